@@ -1,0 +1,133 @@
+/*
+ * tmcore_gpu.h -- C-ABI of libtmgpu.so, the MI355X (gfx950) HIP path of the
+ * TRiSK shallow-water RK stage.
+ *
+ * The reference has no FFI: its seams are Fortran procedures.  Each entry
+ * point below names the reference interface it replaces.  Host arrays are in
+ * reference order (cell/edge/vertex numbering of the mesh descriptor) and are
+ * borrowed only for the duration of a call; the context owns all device
+ * memory.  Every function returns 0 (TM_OK) or a negative TM_E* code from
+ * tmcore_host.h; tm_last_error() gives the text.  A context is used from one
+ * host thread at a time.
+ */
+#ifndef TMCORE_GPU_H
+#define TMCORE_GPU_H
+
+#include <stdint.h>
+
+#include "tmcore_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tm_ctx tm_ctx;
+
+/* Scheme switches: the reference's &tmcore namelist group
+ * (src/core/params_mod.F90:35-58) plus the device options. */
+typedef struct tm_opts {
+  int time_scheme;      /* 2 = CRK4 (time_scheme_mod.F90:109-154); others rejected */
+  int pv_scheme;        /* 3 = high-order upwind PV (operators_mod.F90:359-412)  */
+  int interp_pv_order;  /* 2 or 3 for pv_scheme 3                                */
+  int adv_order;        /* 2 (add_upwind_correction_on_cell is then a no-op)     */
+  int adv_mono;         /* logical, only read for adv_order 3                    */
+  int conserve_energy;  /* logical: CRK4 beta rescale                            */
+  double apvm_weight;   /* read by pv_scheme 1                                   */
+  double dt;            /* seconds                                               */
+  double radius;        /* metres; the reference's is 6371220 (const_mod.F90:11) */
+  int ntracers;         /* passive tracers advected by the stage winds           */
+  int precision;        /* 64: FP64 state and stencils; 32: FP32 state/stencils
+                           with FP64 geometry, weights and invariant sums        */
+  int device;           /* HIP device ordinal                                    */
+  int renumber;         /* 1: Hilbert-curve renumbering of cells/edges/vertices  */
+} tm_opts;
+
+/* Fill *o with the reference namelist defaults (run/namelist.tmcore_swm:12-22):
+ * CRK4, pv_scheme 3, interp_pv_order 3, adv_order 2, conserve_energy, FP64. */
+void tm_opts_default(tm_opts *o);
+
+/* Replaces mesh_init + poly_fit_init + state_init/tend_init
+ * (reference tmcore_swm_mod.F90:37-41): uploads the mesh (reference layout of
+ * tmcore_host.h), derives nSignEdge/tSignEdge/areaEdge/fVertex as
+ * mesh_mod.F90:169-224 does, renumbers along a Hilbert curve and allocates the
+ * device state.  f may be NULL when pv_scheme 3 is not used. */
+int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts *o);
+int tm_ctx_destroy(tm_ctx *ctx);
+const char *tm_last_error(const tm_ctx *ctx);
+
+/* Prognostic upload (reference state(old), tmcore_swm_mod.F90:62-64):
+ * gd[nCells] and ghs[nCells]; either iap_u[nEdges], or u[nEdges] from which
+ * iap_u = sqrt(gd_edge)*u is formed on the device as the reference's run
+ * prologue does; tracers[ntracers][nCells] may be NULL when ntracers == 0. */
+int tm_state_upload(tm_ctx *ctx, const double *gd, const double *u, const double *iap_u,
+                    const double *ghs, const double *tracers);
+/* Download of state(old): any pointer may be NULL.  u is the inverse IAP
+ * wind (tmcore_swm_mod.F90:80). */
+int tm_state_download(tm_ctx *ctx, double *gd, double *u, double *iap_u, double *tracers);
+
+/* Replaces the time loop body of tmcore_swm_run (tmcore_swm_mod.F90:75-87):
+ * nsteps of time_integrate_CRK4 + time_advance + the diagnostic refresh.
+ * invariants_out[4*nsteps] (mass, energy, potential enstrophy, absolute
+ * vorticity after each step, tmcore_swm_mod.F90:215-255) and beta_out[nsteps]
+ * may be NULL.  Synchronous at return. */
+int tm_step(tm_ctx *ctx, int nsteps, double *invariants_out, double *beta_out);
+
+/* Same as tm_step but asynchronous and without outputs: enqueues nsteps on
+ * the context's stream (replayed from a captured hipGraph).  tm_sync waits. */
+int tm_step_async(tm_ctx *ctx, int nsteps);
+int tm_sync(tm_ctx *ctx);
+
+/* The four invariants of the current state (diag_run, diag_mod.F90:76-91). */
+int tm_invariants(tm_ctx *ctx, double out[4]);
+
+/* Test path: one spatial_operators call (tmcore_swm_mod.F90:92-125) on the
+ * current state plus the tracer tendencies, outputs in reference order; any
+ * output pointer may be NULL.  Diagnostics mirror state/tend fields of the
+ * reference: gd_edge, u (edge), ke (cell), vor/gd_vertex/pv_vertex (vertex),
+ * v/pv_edge/pv_flx/u_tend (edge), tend_gd (cell), tend_iap (edge). */
+typedef struct tm_stage_out {
+  double *tend_gd, *tend_iap, *tend_q;
+  double *gd_edge, *u, *ke, *vor, *gd_vertex, *pv_vertex, *v, *pv_edge, *pv_flx, *u_tend;
+} tm_stage_out;
+int tm_stage(tm_ctx *ctx, tm_stage_out *out);
+
+/* Test path: single operators with the argument order of operators_mod.F90,
+ * device round trip per call (not used in timed runs). */
+int tm_op_div(tm_ctx *ctx, const double *f_edge, double *div);                 /* :40-55   */
+int tm_op_curl(tm_ctx *ctx, const double *f_edge, double *curl);               /* :76-91   */
+int tm_op_c2e(tm_ctx *ctx, const double *f_cell, double *f_edge);              /* :93-104  */
+int tm_op_c2v(tm_ctx *ctx, const double *f_cell, double *f_vertex);            /* :106-123 */
+int tm_op_v2c(tm_ctx *ctx, const double *f_vertex, double *f_cell);            /* :125-141 */
+int tm_op_v2e(tm_ctx *ctx, const double *f_vertex, double *f_edge);            /* :143-156 */
+int tm_op_ke(tm_ctx *ctx, const double *u_edge, double *ke_cell);              /* :206-221 */
+int tm_op_tangent_wind(tm_ctx *ctx, const double *u_edge, double *v_edge);     /* :223-237 */
+int tm_op_tangent_vor_flux(tm_ctx *ctx, const double *u_edge, const double *gd_edge,
+                           const double *pv_edge, double *flux);               /* :239-260 */
+
+/* Device timing of the last tm_step/tm_step_async batch, in milliseconds,
+ * measured with HIP events on the context's stream. */
+int tm_last_elapsed_ms(tm_ctx *ctx, double *ms);
+
+/* Element counts and algorithmic bytes of one CRK4 step for this context:
+ * bytes[0] = per stage, bytes[1] = finalize, bytes[2] = per step. */
+int tm_step_bytes(tm_ctx *ctx, double bytes[3]);
+
+/* Instrumentation (no reference counterpart): runs nsteps CRK4 steps with
+ * HIP events around every kernel launch on the context's stream and returns,
+ * per kernel kind (TM_K_*), the summed device time in ms and the number of
+ * launches; kernel_bytes gives the compulsory bytes one launch of each kind
+ * moves (each array it reads or writes counted once). */
+#define TM_K_EDGE_PREP 0
+#define TM_K_CELL_VERTEX 1
+#define TM_K_EDGE_PV 2
+#define TM_K_EDGE_TEND 3
+#define TM_K_REDUCE 4
+#define TM_K_FINALIZE 5
+#define TM_K_COUNT 6
+int tm_profile_steps(tm_ctx *ctx, int nsteps, double ms[TM_K_COUNT], int launches[TM_K_COUNT]);
+int tm_kernel_bytes(tm_ctx *ctx, double bytes[TM_K_COUNT]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,186 @@
+"""HIP path (libtmgpu.so) against the oracle on identical seeded inputs.
+
+Bars (stated in the north star): the plain FP64 path agrees with the FP64
+oracle to <= 1e-12 relative L2 after one step and <= 1e-9 after one simulated
+day, with the same invariant drift.  Stage operators are compared to the same
+1e-12 bar (they are expected to agree to the last bit or two: same operation
+order, no FMA contraction, IEEE division and square root).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tmcore_amd.mesh import Mesh, seeded_state, steady_geostrophic_state
+from tmcore_amd.swm import SwmGpu
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    d = np.linalg.norm(b)
+    return np.linalg.norm(a - b) / (d if d > 0 else 1.0)
+
+
+_MESHES = {}
+
+
+def mesh(level):
+    if level not in _MESHES:
+        _MESHES[level] = Mesh(level)
+    return _MESHES[level]
+
+
+def setup(level, ntracers=0, dt=600.0, seed=2, **kw):
+    m = mesh(level)
+    gd, u, ghs, q = seeded_state(m, seed, [200 + i for i in range(1, ntracers + 1)])
+    o = Oracle(m, m.fit(), dt=dt)
+    iap = o.initial_iap(gd, u)
+    g = SwmGpu(m, m.fit(), dt=dt, ntracers=ntracers, **kw)
+    g.upload(gd, ghs, u=u, tracers=q if ntracers else None)
+    return m, o, g, (gd, u, iap, ghs, q)
+
+
+@pytest.mark.parametrize("level", [2, 5])
+def test_initial_iap_and_download(level):
+    m, o, g, (gd, u, iap, ghs, q) = setup(level)
+    st = g.download()
+    assert rel_l2(st["iap_u"], iap) <= 1e-15
+    np.testing.assert_array_equal(st["gd"], gd)
+    assert rel_l2(st["u"], u) <= 1e-14
+    g.close()
+
+
+@pytest.mark.parametrize("level,ntracers", [(2, 0), (4, 2), (6, 4)])
+def test_stage_operators_match_oracle(level, ntracers):
+    m, o, g, (gd, u, iap, ghs, q) = setup(level, ntracers)
+    tg, ti, d = o.stage(gd, iap, ghs, q)
+    r = g.stage()
+    checks = {"tend_gd": tg, "tend_iap": ti, "gd_edge": d["gd_edge"], "u": d["u"], "ke": d["ke"],
+              "vor": d["vor"], "gd_vertex": d["gd_vertex"], "pv_vertex": d["pv_vertex"], "v": d["v"],
+              "pv_edge": d["pv_edge"], "pv_flx": d["pv_flx"], "u_tend": d["u_tend"]}
+    for k, ref in checks.items():
+        err = rel_l2(r[k], ref)
+        assert err <= 1e-12, (k, err)
+    if ntracers:
+        assert rel_l2(r["tend_q"], d["tend_q"]) <= 1e-12
+    g.close()
+
+
+def test_single_operators_match_oracle():
+    m, o, g, (gd, u, iap, ghs, q) = setup(4)
+    rng = np.random.default_rng(7)
+    fe, fc, fv = rng.standard_normal(m.nEdges), rng.standard_normal(m.nCells), rng.standard_normal(m.nVertices)
+    gde = 1.0 + np.abs(rng.standard_normal(m.nEdges))
+    pairs = [
+        (g.div_operator(fe), o.op("div", fe, out_where="c")),
+        (g.curl_operator(fe), o.op("curl", fe, out_where="v")),
+        (g.scalar_c2e_interp_operator(fc), o.op("c2e", fc, out_where="e")),
+        (g.scalar_c2v_interp_operator(fc), o.op("c2v", fc, out_where="v")),
+        (g.scalar_v2c_interp_operator(fv), o.op("v2c", fv, out_where="c")),
+        (g.scalar_v2e_interp_operator(fv), o.op("v2e", fv, out_where="e")),
+        (g.calc_kinetic_energy(fe), o.op("ke", fe, out_where="c")),
+        (g.calc_tangent_wind(fe), o.op("tangent_wind", fe, out_where="e")),
+        (g.calc_tangent_vor_flux(fe, gde, fv[:0].sum() + fe), o.op("tangent_vor_flux", fe, gde, fe, out_where="e")),
+    ]
+    for i, (a, b) in enumerate(pairs):
+        assert rel_l2(a, b) <= 1e-13, (i, rel_l2(a, b))
+    g.close()
+
+
+@pytest.mark.parametrize("level", [4, 8])
+def test_one_step_matches_oracle(level):
+    dt = 600.0 if level < 8 else 90.0
+    m, o, g, (gd, u, iap, ghs, q) = setup(level, ntracers=4 if level == 8 else 1, dt=dt)
+    gd1, iap1, q1, inv, beta = o.run(1, gd, iap, ghs, q)
+    ginv, gbeta = g.step(1)
+    st = g.download()
+    assert rel_l2(st["gd"], gd1) <= 1e-12
+    assert rel_l2(st["iap_u"], iap1) <= 1e-12
+    assert rel_l2(st["tracers"], q1) <= 1e-12
+    assert abs(gbeta[0] - beta[0]) <= 1e-12
+    np.testing.assert_allclose(ginv[0], inv[0], rtol=1e-12)
+    g.close()
+
+
+def test_one_day_matches_oracle_and_invariant_drift():
+    """Level 5, dt = 600 s, 144 steps (one simulated day), 1 tracer."""
+    m, o, g, (gd, u, iap, ghs, q) = setup(5, ntracers=1, dt=600.0)
+    gdn, iapn, qn, inv, beta = o.run(144, gd, iap, ghs, q)
+    ginv, gbeta = g.step(144)
+    st = g.download()
+    assert rel_l2(st["gd"], gdn) <= 1e-9
+    assert rel_l2(st["iap_u"], iapn) <= 1e-9
+    assert rel_l2(st["tracers"], qn) <= 1e-9
+    np.testing.assert_allclose(gbeta, beta, rtol=1e-9)
+    # same drift of the four invariants relative to the initial state
+    inv0 = o.invariants(gd, iap, ghs)
+    drift_o = inv[:, :3] / inv0[:3] - 1
+    drift_g = ginv[:, :3] / inv0[:3] - 1
+    assert np.abs(drift_g - drift_o).max() <= 1e-12
+    assert np.abs(ginv[:, 3] - inv[:, 3]).max() <= 1e-12 * np.abs(inv0[3]) + 1e-20
+    g.close()
+
+
+def test_bitwise_run_to_run_determinism():
+    _, _, g1, _ = setup(5, ntracers=1)
+    _, _, g2, _ = setup(5, ntracers=1)
+    i1, b1 = g1.step(10)
+    i2, b2 = g2.step(10)
+    s1, s2 = g1.download(), g2.download()
+    np.testing.assert_array_equal(s1["gd"], s2["gd"])
+    np.testing.assert_array_equal(s1["iap_u"], s2["iap_u"])
+    np.testing.assert_array_equal(b1, b2)
+    g1.close()
+    g2.close()
+
+
+def test_renumbering_does_not_change_results():
+    _, _, g1, _ = setup(4, renumber=True)
+    _, _, g2, _ = setup(4, renumber=False)
+    r1, r2 = g1.stage(), g2.stage()
+    for k in ("tend_gd", "tend_iap", "pv_edge"):
+        np.testing.assert_array_equal(r1[k], r2[k])
+    g1.step(5)
+    g2.step(5)
+    assert rel_l2(g1.download()["gd"], g2.download()["gd"]) <= 1e-14
+    g1.close()
+    g2.close()
+
+
+def test_steady_geostrophic_flow_on_device():
+    m = mesh(4)
+    gd, u, ghs = steady_geostrophic_state(m)
+    g = SwmGpu(m, m.fit(), dt=900.0)
+    g.upload(gd, ghs, u=u)
+    inv, beta = g.step(96)
+    st = g.download()
+    assert rel_l2(st["gd"], gd) < 5e-3
+    assert abs(inv[-1, 0] / inv[0, 0] - 1) < 1e-13
+    g.close()
+
+
+def test_fp32_form_tracks_fp64():
+    m, o, g64, (gd, u, iap, ghs, q) = setup(5, ntracers=1)
+    g32 = SwmGpu(m, m.fit(), dt=600.0, ntracers=1, precision=32)
+    g32.upload(gd, ghs, u=u, tracers=q)
+    inv64, _ = g64.step(144)
+    inv32, _ = g32.step(144)
+    s64, s32 = g64.download(), g32.download()
+    assert rel_l2(s32["gd"], s64["gd"]) < 1e-5
+    assert rel_l2(s32["iap_u"], s64["iap_u"]) < 1e-4
+    # compensated FP64 sums keep the FP32 mass drift at the FP32 rounding level
+    assert abs(inv32[-1, 0] / inv64[0, 0] - 1) < 1e-6
+    g64.close()
+    g32.close()
+
+
+def test_nonfinite_upload_is_rejected():
+    m = mesh(2)
+    gd, u, ghs, _ = seeded_state(m, 1)
+    g = SwmGpu(m, m.fit(), dt=600.0)
+    bad = gd.copy()
+    bad[3] = np.nan
+    with pytest.raises(Exception, match="non-finite"):
+        g.upload(bad, ghs, u=u)
+    g.close()

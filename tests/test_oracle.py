@@ -1,0 +1,223 @@
+"""Pins the oracle (oracle/trisk_oracle.c, oracle/poly_fit_oracle.c).
+
+The reference ships no golden vectors, fixtures or tests, and cannot be built
+here (its modules need the absent flogger/fiona libraries).  Each oracle
+function is therefore pinned by what the mathematics fixes: telescoping
+identities, constants preserved by the interpolations, the discrete energy
+and mass conservation the scheme is built for, a library routine
+(numpy.linalg.inv) and an independent numpy least-squares fit.  Each check is
+chosen so that a dropped term, a wrong sign or a transposed index fails it.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, lib, oracle_poly_fit
+from tmcore_amd.mesh import G, RADIUS, Mesh, seeded_state, steady_geostrophic_state
+
+
+@pytest.fixture(scope="module")
+def m4():
+    return Mesh(4)
+
+
+@pytest.fixture(scope="module")
+def o4(m4):
+    return Oracle(m4, m4.fit(), dt=600.0)
+
+
+def _scaled(m):
+    return m.dvEdge * RADIUS, m.dcEdge * RADIUS, m.areaCell * RADIUS**2, m.areaTriangle * RADIUS**2
+
+
+def test_interpolations_preserve_constants(o4, m4):
+    np.testing.assert_array_equal(o4.op("c2e", np.full(m4.nCells, 3.5), out_where="e"), 3.5)
+    np.testing.assert_array_equal(o4.op("v2e", np.full(m4.nVertices, -2.0), out_where="e"), -2.0)
+    np.testing.assert_allclose(o4.op("c2v", np.full(m4.nCells, 7.0), out_where="v"), 7.0, rtol=1e-14)
+    np.testing.assert_allclose(o4.op("v2c", np.full(m4.nVertices, 7.0), out_where="c"), 7.0, rtol=1e-13)
+
+
+def test_div_of_streamfunction_wind_vanishes(o4, m4):
+    """div_operator of u = -dpsi/dt telescopes to zero round every cell."""
+    rng = np.random.default_rng(0)
+    psi = rng.standard_normal(m4.nVertices) * 1e7
+    dv, dc, A, At = _scaled(m4)
+    v1, v2 = m4.verticesOnEdge[:, 0] - 1, m4.verticesOnEdge[:, 1] - 1
+    u = -(psi[v2] - psi[v1]) / dv
+    div = o4.op("div", u, out_where="c")
+    scale = np.abs(u).max() * dv.max() / A.min()
+    assert np.abs(div).max() < 1e-13 * scale
+    # and a non-solenoidal field is not annihilated (the test can fail)
+    assert np.abs(o4.op("div", rng.standard_normal(m4.nEdges), out_where="c")).max() > 1e-3 * dv.max() / A.max()
+
+
+def test_curl_of_gradient_vanishes(o4, m4):
+    rng = np.random.default_rng(1)
+    phi = rng.standard_normal(m4.nCells) * 1e5
+    dv, dc, A, At = _scaled(m4)
+    c1, c2 = m4.cellsOnEdge[:, 0] - 1, m4.cellsOnEdge[:, 1] - 1
+    u = (phi[c2] - phi[c1]) / dc
+    curl = o4.op("curl", u, out_where="v")
+    assert np.abs(curl).max() < 1e-13 * np.abs(u).max() * dc.max() / At.min()
+
+
+def test_div_and_curl_signs(o4, m4):
+    """Outward unit flux on one cell's edges gives -1/A (div_operator returns
+    minus the divergence); a counter-clockwise unit circulation round one
+    triangle gives +1 per unit area."""
+    dv, dc, A, At = _scaled(m4)
+    c = 5
+    f = np.zeros(m4.nEdges)
+    for i in range(m4.nEdgesOnCell[c]):
+        e = m4.edgesOnCell[c, i] - 1
+        f[e] = 1.0 / dv[e] if m4.cellsOnEdge[e, 0] - 1 == c else -1.0 / dv[e]
+    div = o4.op("div", f, out_where="c")
+    assert div[c] == pytest.approx(-m4.nEdgesOnCell[c] / A[c], rel=1e-14)
+    v = 9
+    u = np.zeros(m4.nEdges)
+    for i in range(3):
+        e = m4.edgesOnVertex[v, i] - 1
+        # tangent from vertex 1 to 2 is k x n: circulation ccw round vertex 1 runs along -n
+        u[e] = (-1.0 if m4.verticesOnEdge[e, 0] - 1 == v else 1.0) / dc[e]
+    assert o4.op("curl", u, out_where="v")[v] == pytest.approx(3.0 / At[v], rel=1e-14)
+
+
+def test_ke_closed_forms(o4, m4):
+    # areaEdge = dv*dc (no 1/2, mesh_mod.F90:178): unit normal speed on every
+    # edge gives sum(dv*dc)/4/A = 1 up to the sphere's curvature
+    ke = o4.op("ke", np.ones(m4.nEdges), out_where="c")
+    np.testing.assert_allclose(ke, 1.0, rtol=1e-3)
+    # solid-body flow u0*cos(lat): ke -> u^2/2
+    gd, u, ghs, _ = seeded_state(m4, 1)
+    ke = o4.op("ke", u, out_where="c")
+    exact = 0.5 * (20.0 * np.cos(m4.latCell)) ** 2
+    assert np.abs(ke - exact).max() < 0.05 * 200.0
+
+
+def test_pv_flux_is_energy_neutral(o4, m4):
+    """sum(areaEdge * u*gd_e * pv_flx) = 0 for any inputs (TRiSK antisymmetry)."""
+    rng = np.random.default_rng(2)
+    u, gde, pve = rng.standard_normal((3, m4.nEdges))
+    gde = 1.0 + np.abs(gde)
+    flx = o4.op("tangent_vor_flux", u, gde, pve, out_where="e")
+    aE = m4.dvEdge * m4.dcEdge * RADIUS**2
+    terms = aE * u * gde * flx
+    assert abs(terms.sum()) < 1e-13 * np.abs(terms).sum()
+
+
+def test_math_inv_matrix_matches_numpy():
+    rng = np.random.default_rng(4)
+    for n in (5, 9, 14):
+        A = rng.standard_normal((n, n)) + n * np.eye(n)
+        Af = np.asfortranarray(A.copy())
+        B = np.zeros((n, n), order="F")
+        lib().om_inv_matrix.argtypes = [C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        lib().om_inv_matrix(n, Af.ctypes.data_as(C.POINTER(C.c_double)), B.ctypes.data_as(C.POINTER(C.c_double)))
+        np.testing.assert_allclose(B, np.linalg.inv(A), rtol=1e-10, atol=1e-13)
+
+
+def _numpy_vertex_d2_weights(m, v, fit_row, nfit, edge_slot):
+    """Independent restatement of the second-order vertex fit: local azimuthal
+    coordinates from atan2, normal equations solved by numpy.linalg.lstsq."""
+    P = np.stack([m.xVertex, m.yVertex, m.zVertex], 1)
+    idx = fit_row[:nfit] - 1
+    p0, p1 = P[idx[0]], P[idx[1]]
+    e1 = p1 - np.dot(p1, p0) * p0
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(p0, e1)
+    xs, ys = [], []
+    for q in idx[1:]:
+        p = P[q]
+        d = RADIUS * np.arccos(np.clip(np.dot(p0, p), -1, 1))
+        th = np.arctan2(np.dot(p, e2), np.dot(p, e1))
+        xs.append(np.cos(th) * d)
+        ys.append(np.sin(th) * d)
+    x, y = np.array(xs), np.array(ys)
+    Pm = np.stack([x, y, x * x, x * y, y * y], 1)
+    B = np.linalg.lstsq(Pm, np.eye(len(x)), rcond=None)[0]  # 5 x m
+    th = np.arctan2(ys[edge_slot], xs[edge_slot])
+    ct, st = np.cos(th), np.sin(th)
+    return 2 * B[2] * ct**2 + 2 * B[3] * ct * st + 2 * B[4] * st**2
+
+
+def test_poly_fit_vertex_weights_match_independent_lstsq():
+    m = Mesh(3)
+    r = oracle_poly_fit(m, max_fit=31, order4=False)
+    for v in (0, 17, 400, 1279):
+        for i in range(3):
+            e = m.edgesOnVertex[v, i] - 1
+            k = 0 if m.verticesOnEdge[e, 0] - 1 == v else 1
+            n = r["nFit2Vertices"][v]
+            w_or = r["deriv2OnVertex"][e, k, 1:n]
+            w_np = _numpy_vertex_d2_weights(m, v, r["fit2Vertices"][v], n, i)
+            np.testing.assert_allclose(w_or, w_np, rtol=1e-6, atol=1e-6 * np.abs(w_np).max())
+
+
+def test_poly_fit_stencils():
+    m = Mesh(3)
+    r = oracle_poly_fit(m, max_fit=31, order4=True)
+    # 12 neighbours round a vertex away from pentagons (3 hexagons share 13 vertices)
+    assert set(np.unique(r["nFit2Vertices"])) <= {12, 13}
+    assert (r["nFit2Vertices"] == 13).sum() > 0.9 * m.nVertices
+    # first three stencil points are the vertices across edgesOnVertex(1:3)
+    for v in (0, 99):
+        for i in range(3):
+            e = m.edgesOnVertex[v, i] - 1
+            other = set(m.verticesOnEdge[e]) - {v + 1}
+            assert r["fit2Vertices"][v, i + 1] in other
+    # cell second ring: 1 + 6 + 12 = 19 for hexagons
+    assert np.bincount(r["nFit4Cells"])[19] > 0.85 * m.nCells
+    assert r["nFit4Cells"].min() >= 15  # 14 parameters need at least 14 neighbours
+
+
+def test_product_poly_fit_equals_oracle():
+    m = Mesh(3)
+    r = oracle_poly_fit(m, max_fit=31, order4=True)
+    f = m.fit(max_fit=31, order4=True)
+    for k in ("deriv2OnVertex", "deriv4OnVertex", "deriv2OnCell", "deriv4OnCell"):
+        np.testing.assert_array_equal(getattr(f, k), r[k])
+    for k in ("fit2Vertices", "fit4Vertices", "fit2Cells", "fit4Cells"):
+        np.testing.assert_array_equal(getattr(f, k), r[k])
+
+
+def test_stage_mass_tendency_sums_to_zero(o4, m4):
+    gd, u, ghs, _ = seeded_state(m4, 1)
+    iap = o4.initial_iap(gd, u)
+    tg, ti, d = o4.stage(gd, iap, ghs)
+    A = m4.areaCell * RADIUS**2
+    assert abs((tg * A).sum()) < 1e-14 * np.abs(tg * A).sum()
+    # absolute vorticity: curl of any edge field integrates to zero
+    At = m4.areaTriangle * RADIUS**2
+    assert abs((d["vor"] * At).sum()) < 1e-14 * np.abs(d["vor"] * At).sum()
+
+
+def test_crk4_conserves_mass_and_energy_and_beta_matters():
+    m = Mesh(4)
+    gd, u, ghs, q = seeded_state(m, 1, [101])
+    o = Oracle(m, m.fit(), dt=600.0)
+    iap = o.initial_iap(gd, u)
+    inv0 = o.invariants(gd, iap, ghs)
+    _, _, q2, inv, beta = o.run(12, gd, iap, ghs, q)
+    assert np.abs(inv[:, 0] / inv0[0] - 1).max() < 1e-14
+    assert np.abs(inv[:, 1] / inv0[1] - 1).max() < 1e-13
+    assert np.all(np.abs(beta - 1) < 0.05) and np.any(beta != 1.0)
+    A = m.areaCell
+    assert abs(((q2 - q) * A).sum()) < 1e-14 * (q * A).sum()
+    # without the beta rescale energy drifts orders of magnitude more
+    o2 = Oracle(m, m.fit(), dt=600.0, conserve_energy=False)
+    _, _, _, inv2, beta2 = o2.run(12, gd, iap, ghs)
+    assert np.all(beta2 == 1.0)
+    assert np.abs(inv2[:, 1] / inv0[1] - 1).max() > 100 * np.abs(inv[:, 1] / inv0[1] - 1).max()
+
+
+def test_steady_geostrophic_flow_stays_steady():
+    """Williamson case 2 is an exact steady state; with a wrong Coriolis sign
+    or PV interpolation it is torn apart within hours."""
+    m = Mesh(4)
+    gd, u, ghs = steady_geostrophic_state(m)
+    o = Oracle(m, m.fit(), dt=900.0)
+    iap = o.initial_iap(gd, u)
+    gd2, _, _, inv, _ = o.run(96, gd, iap, ghs)
+    assert np.linalg.norm(gd2 - gd) / np.linalg.norm(gd) < 5e-3
+    assert np.abs(gd2 - gd).max() / G < 60.0

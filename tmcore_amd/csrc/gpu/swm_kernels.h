@@ -1,0 +1,449 @@
+// Fused TRiSK RK-stage kernels for gfx950, templated on the state type R
+// (double: plain FP64 form; float: precision form with FP64 geometry and
+// weights, and FP64 compensated global sums).
+//
+// One RK stage = four launches:
+//   k_edge_prep     gd_edge, u, and the per-edge products the cell/vertex sums
+//                   gather (scalar_c2e + inverse_iap, operators_mod.F90:93-104,
+//                   168-176; flux of calc_gd_tend_on_cell tmcore_swm_mod.F90:
+//                   135; tracer flux adv_scheme_sg11_mod.F90:42-45)
+//   k_cell_vertex   cells: div_operator (:40-55), calc_kinetic_energy
+//                   (:206-221), E = ke+gd+ghs (tmcore_swm_mod.F90:177), tracer
+//                   div, RK stage update of gd/tracers, beta partial products;
+//                   vertices: curl_operator (:76-91), scalar_c2v (:106-123),
+//                   calc_pv_on_vertex (:262-270)
+//   k_edge_pv       calc_tangent_wind (:223-237) + calc_pv_on_edge scheme 3
+//                   (:359-412)
+//   k_edge_tend     calc_tangent_vor_flux (:239-260), calc_u_tend_on_edge
+//                   (tmcore_swm_mod.F90:155-201), RK stage update of iap_u
+// and per step k_reduce (inner_product, beta, invariants) and k_finalize
+// (update_state(beta*dt), time_scheme_mod.F90:152).
+//
+// Each expression keeps the reference's association so that the FP64 form,
+// built with -ffp-contract=off, reproduces the reference arithmetic.
+#pragma once
+
+#include "tm_device.h"
+
+namespace tmk {
+
+__device__ __forceinline__ double tm_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float tm_sqrt(float x) { return sqrtf(x); }
+template <typename R>
+__device__ __forceinline__ R sign1(R x) {  // Fortran sign(1.0d0, x)
+  return signbit(x) ? R(-1) : R(1);
+}
+__device__ __forceinline__ int unsign(int idx) { return idx ^ (idx >> 31); }
+
+template <typename R>
+struct StageBufs {
+  const R *gd, *iap, *q;                 // stage input state X_k
+  const R *gd_base, *iap_base, *q_base;  // X_1 (state(one))
+  R *gd_next, *iap_next, *q_next;        // X_{k+1}
+  R *acc_gd, *acc_iap, *acc_q;           // t1 + 2 t2 + 2 t3 (+ t4)/6
+  const R *tprev_gd, *tprev_iap;         // tendency of stage k-1
+  R *tcur_gd, *tcur_iap;                 // tendency of stage k
+  R *gd_e, *u, *Fdv, *kee, *udc, *qFdv, *E, *tend_gd, *pv_v, *pv_e;
+  const R *ghs;
+  int nT;
+  int k;          // 1..4: CRK4 stage; 0: diagnostics/invariants only
+  double cdt;     // c_k * dt of update_state (c = 1/2, 1/2, 1)
+  int want_inv;   // accumulate the invariants of X_k
+  int pv_order;   // interp_pv_order (2..4)
+  double2 *part;  // [kNumSums][part_stride] block partial sums (hi, lo)
+  int part_stride;
+  // optional diagnostics (test path), NULL in timed runs
+  R *d_ke, *d_vor, *d_gdv, *d_v, *d_pvflx, *d_utend, *d_tiap, *d_tq;
+};
+
+// ---------------------------------------------------------------- reductions
+// Double-double accumulation (Knuth TwoSum) for the compensated global sums.
+__device__ __forceinline__ double2 dd_add(double2 a, double2 b) {
+  double s = a.x + b.x;
+  double bb = s - a.x;
+  double err = (a.x - (s - bb)) + (b.x - bb);
+  double lo = err + (a.y + b.y);
+  double hi = s + lo;
+  return make_double2(hi, lo - (hi - s));
+}
+
+template <int N>
+__device__ __forceinline__ void block_sums(const double (&v)[N], const int (&ids)[N], double2 *part, int stride,
+                                           int tile) {
+  __shared__ double2 sh[N][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __syncthreads();  // a previous call with the same N may still be reading sh
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    double2 a = make_double2(v[i], 0.0);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      double2 o = make_double2(__shfl_xor(a.x, off, 64), __shfl_xor(a.y, off, 64));
+      a = dd_add(a, o);
+    }
+    if (lane == 0) sh[i][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x < N) {
+    const int i = threadIdx.x;
+    double2 a = sh[i][0];
+#pragma unroll
+    for (int q = 1; q < kBlock / 64; ++q) a = dd_add(a, sh[i][q]);
+    part[ids[i] * stride + tile] = a;
+  }
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
+  const int tile = xcd_tile(blockIdx.x, m.nbE);
+  const int e = tile * kBlock + threadIdx.x;
+  double en = 0.0;
+  if (e < m.nE) {
+    const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+    const R gde = R(0.5) * (b.gd[c1] + b.gd[c2]);
+    const R iap = b.iap[e];
+    const R u = iap / tm_sqrt(gde);
+    const R dv = (R)m.dv[e];
+    b.gd_e[e] = gde;
+    b.u[e] = u;
+    b.Fdv[e] = (u * gde) * dv;
+    b.kee[e] = (u * u) * (R)m.aE[e];
+    b.udc[e] = u * (R)m.dc[e];
+    for (int t = 0; t < b.nT; ++t) {
+      const R qe = R(0.5) * (b.q[(size_t)t * m.nC + c1] + b.q[(size_t)t * m.nC + c2]);
+      b.qFdv[(size_t)t * m.nE + e] = (u * qe) * dv;
+    }
+    if (b.want_inv) en = ((double)iap * (double)iap) * m.aE[e];
+  }
+  if (b.want_inv) {
+    const double v[1] = {en};
+    const int ids[1] = {kEnergyE};
+    block_sums<1>(v, ids, b.part, b.part_stride, tile);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b, int tile) {
+  const int c = tile * kBlock + threadIdx.x;
+  double mass = 0.0, enc = 0.0, rp = 0.0, r44 = 0.0;
+  if (c < m.nC) {
+    const int n = m.nEoC[c];
+    int idx[kMaxEoC];
+#pragma unroll
+    for (int s = 0; s < kMaxEoC; ++s) idx[s] = s < n ? m.eocS[s * m.nC + c] : 0;
+    R sdiv = R(0), ske = R(0);
+#pragma unroll
+    for (int s = 0; s < kMaxEoC; ++s)
+      if (s < n) {
+        const int e = unsign(idx[s]);
+        const R f = b.Fdv[e];
+        sdiv += idx[s] < 0 ? -f : f;
+        ske += b.kee[e];
+      }
+    const double A = m.areaCell[c];
+    const R tend = (-sdiv) / (R)A;
+    const R ke = (ske * R(0.25)) / (R)A;
+    const R gd = b.gd[c];
+    const R ghs = b.ghs[c];
+    b.E[c] = (ke + gd) + ghs;
+    b.tend_gd[c] = tend;
+    if (b.d_ke) b.d_ke[c] = ke;
+    if (b.want_inv) {
+      const double g = gd;
+      mass = g * A;
+      enc = ((g * g) + (2.0 * g) * (double)ghs) * A;
+    }
+    if (b.k >= 1) {
+      if (b.k < 4) b.gd_next[c] = b.gd_base[c] + (R)b.cdt * tend;
+      R acc;
+      if (b.k == 1)
+        acc = tend;
+      else if (b.k < 4)
+        acc = b.acc_gd[c] + R(2) * tend;
+      else
+        acc = (b.acc_gd[c] + tend) / R(6);
+      b.acc_gd[c] = acc;
+      if (b.k >= 2) rp = ((double)b.tprev_gd[c] * (double)tend) * A;
+      if (b.k == 4) r44 = ((double)acc * (double)acc) * A;
+      if (b.k < 4) b.tcur_gd[c] = tend;
+    }
+    for (int t = 0; t < b.nT; ++t) {
+      const R *qf = b.qFdv + (size_t)t * m.nE;
+      R sq = R(0);
+#pragma unroll
+      for (int s = 0; s < kMaxEoC; ++s)
+        if (s < n) {
+          const R f = qf[unsign(idx[s])];
+          sq += idx[s] < 0 ? -f : f;
+        }
+      const R tq = (-sq) / (R)A;
+      const size_t o = (size_t)t * m.nC + c;
+      if (b.d_tq) b.d_tq[o] = tq;
+      if (b.k >= 1) {
+        if (b.k < 4) b.q_next[o] = b.q_base[o] + (R)b.cdt * tq;
+        R acc;
+        if (b.k == 1)
+          acc = tq;
+        else if (b.k < 4)
+          acc = b.acc_q[o] + R(2) * tq;
+        else
+          acc = (b.acc_q[o] + tq) / R(6);
+        b.acc_q[o] = acc;
+      }
+    }
+  }
+  if (b.want_inv) {
+    const double v[2] = {mass, enc};
+    const int ids[2] = {kMass, kEnergyC};
+    block_sums<2>(v, ids, b.part, b.part_stride, tile);
+  }
+  if (b.k >= 2) {
+    const int rid = b.k == 2 ? kR12c : (b.k == 3 ? kR23c : kR34c);
+    if (b.k == 4) {
+      const double v[2] = {rp, r44};
+      const int ids[2] = {rid, kR44c};
+      block_sums<2>(v, ids, b.part, b.part_stride, tile);
+    } else {
+      const double v[1] = {rp};
+      const int ids[1] = {rid};
+      block_sums<1>(v, ids, b.part, b.part_stride, tile);
+    }
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void vertex_part(const DMesh &m, const StageBufs<R> &b, int tile) {
+  const int v = tile * kBlock + threadIdx.x;
+  double ens = 0.0, av = 0.0;
+  if (v < m.nV) {
+    R s = R(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int idx = m.eovS[j * m.nV + v];
+      const R f = b.udc[unsign(idx)];
+      s += idx < 0 ? -f : f;
+    }
+    const double At = m.areaTri[v];
+    const R vor = (-s) / (R)At;
+    R g = R(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g = g + b.gd[m.cov[j * m.nV + v]] * (R)m.kite[j * m.nV + v];
+    const R gdv = g / (R)At;
+    const R pv = ((R)m.fV[v] + vor) / gdv;
+    b.pv_v[v] = pv;
+    if (b.d_vor) b.d_vor[v] = vor;
+    if (b.d_gdv) b.d_gdv[v] = gdv;
+    if (b.want_inv) {
+      ens = ((double)gdv * ((double)pv * (double)pv)) * At;
+      av = ((double)vor + m.fV[v]) * At;
+    }
+  }
+  if (b.want_inv) {
+    const double vv[2] = {ens, av};
+    const int ids[2] = {kEnstrophy, kAbsVor};
+    block_sums<2>(vv, ids, b.part, b.part_stride, tile);
+  }
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b) {
+  if ((int)blockIdx.x < m.nbC)
+    cell_part<R>(m, b, xcd_tile(blockIdx.x, m.nbC));
+  else
+    vertex_part<R>(m, b, xcd_tile(blockIdx.x - m.nbC, m.nbV));
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
+  const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
+  if (e >= m.nE) return;
+  const int n = m.nEoE[e];
+  R v = R(0);
+#pragma unroll
+  for (int s = 0; s < kMaxEoE; ++s)
+    if (s < n) v += b.u[m.eoe[s * m.nE + e]] * (R)m.woe[s * m.nE + e];
+  const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
+  const R p1 = b.pv_v[v1], p2 = b.pv_v[v2];
+  R pve = R(0.5) * (p1 + p2);
+  if (b.pv_order == 3 || b.pv_order == 4) {
+    const R coef2 = b.pv_order == 3 ? R(1) : R(0);
+    const int n1 = m.nFit2V[v1] - 1, n2 = m.nFit2V[v2] - 1;
+    R d1 = R(0), d2 = R(0);
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      if (j < n1) d1 += (R)m.d2v[(size_t)j * m.nE + e] * (b.pv_v[m.fit2V[j * m.nV + v1]] - p1);
+      if (j < n2) d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (b.pv_v[m.fit2V[j * m.nV + v2]] - p2);
+    }
+    const R dv = (R)m.dv[e];
+    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1(v) * coef2) * (d2 - d1))) / R(12);
+  }
+  b.pv_e[e] = pve;
+  if (b.d_v) b.d_v[e] = v;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
+  const int tile = xcd_tile(blockIdx.x, m.nbE);
+  const int e = tile * kBlock + threadIdx.x;
+  double rp = 0.0, r44 = 0.0;
+  if (e < m.nE) {
+    const int n = m.nEoE[e];
+    const R pve = b.pv_e[e];
+    R s = R(0);
+#pragma unroll
+    for (int q = 0; q < kMaxEoE; ++q)
+      if (q < n) {
+        const int ep = m.eoe[q * m.nE + e];
+        s = s + ((((R)m.woe[q * m.nE + e] * b.u[ep]) * b.gd_e[ep]) * R(0.5)) * (pve + b.pv_e[ep]);
+      }
+    const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+    const R dEdx = (b.E[c2] - b.E[c1]) / (R)m.dc[e];
+    const R ut = s - dEdx;
+    const R sg = tm_sqrt(b.gd_e[e]);
+    const R tge = R(0.5) * (b.tend_gd[c1] + b.tend_gd[c2]);
+    const R it = sg * ut + ((R(0.5) * b.u[e]) / sg) * tge;
+    if (b.d_pvflx) b.d_pvflx[e] = s;
+    if (b.d_utend) b.d_utend[e] = ut;
+    if (b.d_tiap) b.d_tiap[e] = it;
+    if (b.k >= 1) {
+      const double aE = m.aE[e];
+      if (b.k < 4) b.iap_next[e] = b.iap_base[e] + (R)b.cdt * it;
+      R acc;
+      if (b.k == 1)
+        acc = it;
+      else if (b.k < 4)
+        acc = b.acc_iap[e] + R(2) * it;
+      else
+        acc = (b.acc_iap[e] + it) / R(6);
+      b.acc_iap[e] = acc;
+      if (b.k >= 2) rp = ((double)b.tprev_iap[e] * (double)it) * aE;
+      if (b.k == 4) r44 = ((double)acc * (double)acc) * aE;
+      if (b.k < 4) b.tcur_iap[e] = it;
+    }
+  }
+  if (b.k >= 2) {
+    const int rid = b.k == 2 ? kR12e : (b.k == 3 ? kR23e : kR34e);
+    if (b.k == 4) {
+      const double v[2] = {rp, r44};
+      const int ids[2] = {rid, kR44e};
+      block_sums<2>(v, ids, b.part, b.part_stride, tile);
+    } else {
+      const double v[1] = {rp};
+      const int ids[1] = {rid};
+      block_sums<1>(v, ids, b.part, b.part_stride, tile);
+    }
+  }
+}
+
+// Sums the block partials.  mode 1: end of a CRK4 step -> beta (and the
+// invariants of the step's base state when have_inv); mode 0: invariants only.
+struct ReduceArgs {
+  const double2 *part;
+  int stride, nbE, nbC, nbV;
+  double inv_total_area, inv_sum_area_tri;
+  int conserve_energy;
+  int mode;
+  int have_inv;
+  double *beta;       // scalar read by k_finalize
+  int *hist_idx;      // device step counter of the current batch
+  double *beta_hist;  // [cap]
+  double *inv_hist;   // [cap][4]
+  int hist_cap;
+};
+
+__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs a) {
+  __shared__ double2 sh[kNumSums][kBlock / 64];
+  __shared__ double res[kNumSums];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int i = 0; i < kNumSums; ++i) {
+    const bool edge = i == kR12e || i == kR23e || i == kR34e || i == kR44e || i == kEnergyE;
+    const bool vert = i == kEnstrophy || i == kAbsVor;
+    const bool is_inv = i >= kMass;
+    const int nb = edge ? a.nbE : (vert ? a.nbV : a.nbC);
+    double2 acc = make_double2(0.0, 0.0);
+    if ((is_inv && a.have_inv) || (!is_inv && a.mode == 1))
+      for (int j = threadIdx.x; j < nb; j += kBlock) acc = dd_add(acc, a.part[i * a.stride + j]);
+    for (int off = 32; off > 0; off >>= 1) {
+      double2 o = make_double2(__shfl_xor(acc.x, off, 64), __shfl_xor(acc.y, off, 64));
+      acc = dd_add(acc, o);
+    }
+    if (lane == 0) sh[i][w] = acc;
+  }
+  __syncthreads();
+  if (threadIdx.x < kNumSums) {
+    double2 acc = sh[threadIdx.x][0];
+    for (int q = 1; q < kBlock / 64; ++q) acc = dd_add(acc, sh[threadIdx.x][q]);
+    res[threadIdx.x] = acc.x + acc.y;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const int i = *a.hist_idx;
+    if (a.mode == 1) {
+      double beta = 1.0;
+      if (a.conserve_energy) {
+        // inner_product_tend (operators_mod.F90:187-194) and CRK4 beta
+        // (time_scheme_mod.F90:140-150)
+        const double R44 = (res[kR44e] + res[kR44c]) * a.inv_total_area;
+        const double R12 = (res[kR12e] + res[kR12c]) * a.inv_total_area;
+        const double R23 = (res[kR23e] + res[kR23c]) * a.inv_total_area;
+        const double R34 = (res[kR34e] + res[kR34c]) * a.inv_total_area;
+        beta = (R12 + R23 + R34) / (3.0 * R44);
+      }
+      *a.beta = beta;
+      if (i < a.hist_cap) a.beta_hist[i] = beta;
+    }
+    if (a.have_inv && i >= 1 && i - 1 < a.hist_cap) {
+      double *o = a.inv_hist + 4 * (size_t)(i - 1);
+      o[0] = res[kMass];
+      o[1] = res[kEnergyE] + res[kEnergyC];
+      o[2] = res[kEnstrophy];
+      o[3] = res[kAbsVor] * a.inv_sum_area_tri;
+    }
+    if (a.mode == 1) *a.hist_idx = i + 1;
+  }
+}
+
+template <typename R>
+struct FinalBufs {
+  R *gd, *iap, *q;
+  const R *acc_gd, *acc_iap, *acc_q;
+  const double *beta;
+  double dt;
+  int nT;
+};
+
+// update_state(beta*dt, tend(new), state(old), state(new)) (time_scheme_mod.F90:152);
+// tracers with beta = 1 (tmcore_adv_mod.F90:175 with conserve_energy off).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_finalize(DMesh m, FinalBufs<R> f) {
+  const double bdt = (*f.beta) * f.dt;
+  if ((int)blockIdx.x < m.nbE) {
+    const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
+    if (e < m.nE) f.iap[e] = f.iap[e] + (R)bdt * f.acc_iap[e];
+  } else {
+    const int c = xcd_tile(blockIdx.x - m.nbE, m.nbC) * kBlock + threadIdx.x;
+    if (c < m.nC) {
+      f.gd[c] = f.gd[c] + (R)bdt * f.acc_gd[c];
+      for (int t = 0; t < f.nT; ++t) {
+        const size_t o = (size_t)t * m.nC + c;
+        f.q[o] = f.q[o] + (R)f.dt * f.acc_q[o];
+      }
+    }
+  }
+}
+
+// iap_u = sqrt(c2e(gd)) * u (tmcore_swm_mod.F90:62-64)
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, const R *gd, const R *u, R *iap) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nE) return;
+  const R gde = R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]);
+  iap[e] = tm_sqrt(gde) * u[e];
+}
+
+__global__ void k_reset_counter(int *p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
+}
+
+}  // namespace tmk

@@ -1,0 +1,57 @@
+// Device-side data of the TRiSK stage: mesh constants in slot-major padded
+// SoA (renumbered along a Hilbert curve) and the per-precision state buffers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tmk {
+
+constexpr int kBlock = 256;     // threads per workgroup (4 wave64)
+constexpr int kXcd = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
+constexpr int kMaxEoC = 6;      // maxEdges
+constexpr int kMaxEoE = 10;     // edgesOnEdge slots used by hex/pent Voronoi meshes
+constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding centre)
+constexpr int kNumSums = 13;    // global sums per step (see SumId)
+
+enum SumId {
+  kR12e = 0, kR12c, kR23e, kR23c, kR34e, kR34c, kR44e, kR44c,
+  kMass, kEnergyE, kEnergyC, kEnstrophy, kAbsVor
+};
+
+// Mesh constants, new numbering, 0-based.  Slot-major: array[slot * n + i].
+// Signed connectivity packs nSignEdge / tSignEdge (mesh_mod.F90:182-203) into
+// the index: e for +1, ~e for -1.
+struct DMesh {
+  int nC, nE, nV;
+  int nbC, nbE, nbV;  // workgroups per element kind, multiples of kXcd
+  const int8_t *nEoC;
+  const int *eocS;        // [6][nC] signed edgesOnCell
+  const double *areaCell; // [nC]
+  const int *coe;         // [2][nE] cellsOnEdge
+  const int *voe;         // [2][nE] verticesOnEdge
+  const double *dv, *dc, *aE;  // [nE] dvEdge, dcEdge, areaEdge = dv*dc (radius-scaled)
+  const int8_t *nEoE;
+  const int *eoe;         // [10][nE] edgesOnEdge
+  const double *woe;      // [10][nE] weightsOnEdge
+  const double *d2v;      // [2][12][nE] deriv2OnVertex slots 1..12
+  const int *eovS;        // [3][nV] signed edgesOnVertex
+  const int *cov;         // [3][nV] cellsOnVertex
+  const double *kite;     // [3][nV] kiteAreasOnVertex
+  const double *areaTri;  // [nV]
+  const double *fV;       // [nV] 2*Omega*sin(latVertex)
+  const int8_t *nFit2V;   // [nV] nFit2Vertices
+  const int *fit2V;       // [12][nV] fit2Vertices slots 1..12
+  const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
+  const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))
+};
+
+// Workgroup id -> logical tile so that each XCD (round-robin by workgroup id)
+// gets one contiguous stretch of the Hilbert order, keeping a tile's stencil
+// neighbours in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int per = nb / kXcd;
+  return (b % kXcd) * per + b / kXcd;
+}
+
+}  // namespace tmk

@@ -1,0 +1,851 @@
+// libtmgpu: MI355X (gfx950) HIP path of the TMCORE TRiSK shallow-water RK
+// stage behind the C-ABI of include/tmcore_gpu.h.
+//
+// Host side: validates the reference-layout mesh, renumbers cells, edges and
+// vertices along a 3-D Hilbert curve (so stencil gathers of neighbouring
+// threads hit neighbouring cache lines), stores connectivity slot-major (SoA,
+// padded to the widest stencil), owns every device buffer, and drives the
+// CRK4 step as a captured hipGraph on one stream.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "swm_kernels.h"
+#include "tmcore_gpu.h"
+
+using namespace tmk;
+
+namespace {
+
+// ------------------------------------------------------------- Hilbert order
+// Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707 (2004):
+// axes -> transposed Hilbert index, then bit interleave.
+uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
+  uint32_t X[3] = {x, y, z};
+  const uint32_t M = 1u << (bits - 1);
+  for (uint32_t Q = M; Q > 1; Q >>= 1) {
+    const uint32_t P = Q - 1;
+    for (int i = 0; i < 3; ++i) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        uint32_t t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; ++i) X[i] ^= X[i - 1];
+  uint32_t t = 0;
+  for (uint32_t Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; ++i) X[i] ^= t;
+  uint64_t key = 0;
+  for (int b = bits - 1; b >= 0; --b)
+    for (int i = 0; i < 3; ++i) key = (key << 1) | ((X[i] >> b) & 1u);
+  return key;
+}
+
+// new -> old permutation sorting points on the unit sphere by Hilbert key.
+std::vector<int> hilbert_order(int n, const double *x, const double *y, const double *z, bool enabled) {
+  std::vector<int> p(n);
+  std::iota(p.begin(), p.end(), 0);
+  if (!enabled) return p;
+  const int bits = 20;
+  const double s = (double)((1u << bits) - 1) / 2.0;
+  std::vector<uint64_t> key(n);
+  for (int i = 0; i < n; ++i) {
+    auto q = [&](double v) { return (uint32_t)std::lround((std::min(1.0, std::max(-1.0, v)) + 1.0) * s); };
+    key[i] = hilbert3(q(x[i]), q(y[i]), q(z[i]), bits);
+  }
+  std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return key[a] < key[b]; });
+  return p;
+}
+
+int round_blocks(int n) {
+  int nb = (n + kBlock - 1) / kBlock;
+  return ((nb + kXcd - 1) / kXcd) * kXcd;
+}
+
+}  // namespace
+
+struct tm_ctx {
+  tm_opts o{};
+  std::string err;
+  hipStream_t stream = nullptr;
+  int nC = 0, nE = 0, nV = 0, nT = 0;
+  size_t rsz = 8;  // bytes per state value
+  std::vector<int> pC, pE, pV;   // new -> old
+  std::vector<int> iC, iE, iV;   // old -> new
+  DMesh dm{};
+  std::vector<void *> allocs;
+  double totalArea = 0.0, sumAreaTri = 0.0;
+  double step_bytes[3] = {0, 0, 0};
+  double kbytes[TM_K_COUNT] = {};
+  // state (element type R = double or float, stored untyped)
+  void *gd_old = nullptr, *iap_old = nullptr, *q_old = nullptr, *ghs = nullptr;
+  void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *q_s = nullptr;
+  void *acc_gd = nullptr, *acc_iap = nullptr, *acc_q = nullptr;
+  void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
+  void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr, *qFdv = nullptr;
+  void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
+  void *scratch_e[4] = {}, *scratch_c[2] = {}, *scratch_v[2] = {};
+  double2 *part = nullptr;
+  int part_stride = 0;
+  double *d_beta = nullptr, *beta_hist = nullptr, *inv_hist = nullptr;
+  int *hist_idx = nullptr;
+  int hist_cap = 4096;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool have_state = false;
+  bool timed = false;
+
+  int fail(int code, const std::string &m) {
+    err = m;
+    return code;
+  }
+  template <typename T>
+  T *alloc(size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    return (T *)p;
+  }
+  void *alloc_r(size_t n) {
+    void *p = nullptr;
+    if (hipMalloc(&p, std::max<size_t>(n, 1) * rsz) != hipSuccess) return nullptr;
+    allocs.push_back(p);
+    return p;
+  }
+};
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return (ctx)->fail(TM_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+template <typename T>
+int upload(tm_ctx *c, T *dst, const std::vector<T> &h) {
+  HIPCHK(c, hipMemcpyAsync(dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+// Convert a reference-order FP64 host array to new order and the state type.
+int upload_state(tm_ctx *c, void *dst, const double *src, const std::vector<int> &perm, int ncopies, int n) {
+  if (c->rsz == 8) {
+    std::vector<double> h((size_t)n * ncopies);
+    for (int t = 0; t < ncopies; ++t)
+      for (int i = 0; i < n; ++i) h[(size_t)t * n + i] = src[(size_t)t * n + perm[i]];
+    return upload(c, (double *)dst, h);
+  }
+  std::vector<float> h((size_t)n * ncopies);
+  for (int t = 0; t < ncopies; ++t)
+    for (int i = 0; i < n; ++i) h[(size_t)t * n + i] = (float)src[(size_t)t * n + perm[i]];
+  return upload(c, (float *)dst, h);
+}
+
+int download_state(tm_ctx *c, double *dst, const void *src, const std::vector<int> &perm, int ncopies, int n) {
+  if (c->rsz == 8) {
+    std::vector<double> h((size_t)n * ncopies);
+    HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int t = 0; t < ncopies; ++t)
+      for (int i = 0; i < n; ++i) dst[(size_t)t * n + perm[i]] = h[(size_t)t * n + i];
+    return TM_OK;
+  }
+  std::vector<float> h((size_t)n * ncopies);
+  HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int t = 0; t < ncopies; ++t)
+    for (int i = 0; i < n; ++i) dst[(size_t)t * n + perm[i]] = h[(size_t)t * n + i];
+  return TM_OK;
+}
+
+template <typename R>
+StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
+  StageBufs<R> b{};
+  auto P = [](void *p) { return (R *)p; };
+  // gd ping-pongs: the vertex part of k_cell_vertex reads neighbour cells of
+  // the stage input while the cell part writes the next stage state.
+  const int in_slot = (k == 2 || k == 4) ? 0 : 1;
+  const int out_slot = (k == 1 || k == 3) ? 0 : 1;
+  b.gd = (k <= 1) ? P(c->gd_old) : P(c->gdS[in_slot]);
+  b.iap = (k <= 1) ? P(c->iap_old) : P(c->iap_s);
+  b.q = (k <= 1) ? P(c->q_old) : P(c->q_s);
+  b.gd_base = P(c->gd_old);
+  b.iap_base = P(c->iap_old);
+  b.q_base = P(c->q_old);
+  b.gd_next = P(c->gdS[out_slot]);
+  b.iap_next = P(c->iap_s);
+  b.q_next = P(c->q_s);
+  b.acc_gd = P(c->acc_gd);
+  b.acc_iap = P(c->acc_iap);
+  b.acc_q = P(c->acc_q);
+  b.tprev_gd = P(c->t_gd[(k + 1) & 1]);
+  b.tprev_iap = P(c->t_iap[(k + 1) & 1]);
+  b.tcur_gd = P(c->t_gd[k & 1]);
+  b.tcur_iap = P(c->t_iap[k & 1]);
+  b.gd_e = P(c->gd_e);
+  b.u = P(c->u);
+  b.Fdv = P(c->Fdv);
+  b.kee = P(c->kee);
+  b.udc = P(c->udc);
+  b.qFdv = P(c->qFdv);
+  b.E = P(c->E);
+  b.tend_gd = P(c->tend_gd);
+  b.pv_v = P(c->pv_v);
+  b.pv_e = P(c->pv_e);
+  b.ghs = P(c->ghs);
+  b.nT = c->nT;
+  b.k = k;
+  const double cf[5] = {0.0, 0.5, 0.5, 1.0, 0.0};
+  b.cdt = cf[k] * c->o.dt;
+  b.want_inv = want_inv ? 1 : 0;
+  b.pv_order = c->o.pv_scheme == 3 ? c->o.interp_pv_order : 2;
+  b.part = c->part;
+  b.part_stride = c->part_stride;
+  return b;
+}
+
+template <typename R>
+int launch_stage(tm_ctx *c, const StageBufs<R> &b, bool tail = true) {
+  const DMesh &m = c->dm;
+  hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+  hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
+  if (tail) {
+    hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+  }
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
+ReduceArgs reduce_args(tm_ctx *c, int mode, bool have_inv) {
+  ReduceArgs a{};
+  a.part = c->part;
+  a.stride = c->part_stride;
+  a.nbE = c->dm.nbE;
+  a.nbC = c->dm.nbC;
+  a.nbV = c->dm.nbV;
+  a.inv_total_area = 1.0 / c->totalArea;
+  a.inv_sum_area_tri = 1.0 / c->sumAreaTri;
+  a.conserve_energy = c->o.conserve_energy;
+  a.mode = mode;
+  a.have_inv = have_inv ? 1 : 0;
+  a.beta = c->d_beta;
+  a.hist_idx = c->hist_idx;
+  a.beta_hist = c->beta_hist;
+  a.inv_hist = c->inv_hist;
+  a.hist_cap = c->hist_cap;
+  return a;
+}
+
+// inner_product divides by totalArea; the reference divides, so do the same.
+template <typename R>
+int enqueue_step(tm_ctx *c) {
+  for (int k = 1; k <= 4; ++k) {
+    int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 1, true));
+  FinalBufs<R> f{(R *)c->gd_old, (R *)c->iap_old, (R *)c->q_old, (const R *)c->acc_gd, (const R *)c->acc_iap,
+                 (const R *)c->acc_q, c->d_beta, c->o.dt, c->nT};
+  hipLaunchKernelGGL(k_finalize<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, f);
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
+// Invariants of state(old) (the post-step refresh of tmcore_swm_mod.F90:78-85).
+template <typename R>
+int enqueue_refresh(tm_ctx *c) {
+  int rc = launch_stage<R>(c, stage_bufs<R>(c, 0, true), false);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 0, true));
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
+int build_graph(tm_ctx *c) {
+  if (c->gexec) return TM_OK;
+  HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = c->rsz == 8 ? enqueue_step<double>(c) : enqueue_step<float>(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (rc) return rc;
+  if (e != hipSuccess) return c->fail(TM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  c->graph = g;
+  HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  return TM_OK;
+}
+
+int run_steps(tm_ctx *c, int nsteps, bool refresh) {
+  int rc = build_graph(c);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  c->timed = true;
+  if (refresh) {
+    rc = c->rsz == 8 ? enqueue_refresh<double>(c) : enqueue_refresh<float>(c);
+    if (rc) return rc;
+  }
+  return TM_OK;
+}
+
+bool finite_all(const double *a, size_t n) {
+  for (size_t i = 0; i < n; ++i)
+    if (!std::isfinite(a[i])) return false;
+  return true;
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+extern "C" void tm_opts_default(tm_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->time_scheme = 2;
+  o->pv_scheme = 3;
+  o->interp_pv_order = 3;
+  o->adv_order = 2;
+  o->adv_mono = 1;
+  o->conserve_energy = 1;
+  o->apvm_weight = 0.5;
+  o->dt = 600.0;
+  o->radius = 6371220.0;
+  o->ntracers = 0;
+  o->precision = 64;
+  o->device = 0;
+  o->renumber = 1;
+}
+
+extern "C" const char *tm_last_error(const tm_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int tm_ctx_destroy(tm_ctx *c) {
+  if (!c) return TM_EINVAL;
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  for (void *p : c->allocs) (void)hipFree(p);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return TM_OK;
+}
+
+static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
+  const tm_opts &o = c->o;
+  if (o.time_scheme != 2) return c->fail(TM_EINVAL, "only time_scheme = 2 (CRK4) is implemented on the device");
+  if (o.pv_scheme != 3) return c->fail(TM_EINVAL, "only pv_scheme = 3 is implemented on the device");
+  if (o.interp_pv_order < 2 || o.interp_pv_order > 4)
+    return c->fail(TM_EINVAL, "interp_pv_order must be 2, 3 or 4 on the device");
+  if (o.adv_order != 2) return c->fail(TM_EINVAL, "only adv_order = 2 is implemented on the device");
+  if (o.precision != 64 && o.precision != 32) return c->fail(TM_EINVAL, "precision must be 64 or 32");
+  if (o.ntracers < 0 || o.ntracers > 64) return c->fail(TM_EINVAL, "ntracers out of range");
+  if (!(o.dt > 0.0) || !(o.radius > 0.0)) return c->fail(TM_EINVAL, "dt and radius must be positive");
+  if (!m || m->nCells <= 0 || m->nEdges <= 0 || m->nVertices <= 0 || m->vertexDegree != 3 || m->maxEdges > 6 ||
+      m->maxEdges < 3 || m->maxEdges2 < 1)
+    return c->fail(TM_EINVAL, "bad mesh dimensions");
+  const bool need_fit = o.interp_pv_order >= 3;
+  if (need_fit && (!f || !f->nFit2Vertices || !f->fit2Vertices || !f->deriv2OnVertex || f->maxFit < 2))
+    return c->fail(TM_EINVAL, "interp_pv_order >= 3 needs nFit2Vertices, fit2Vertices and deriv2OnVertex");
+  const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices, ME = m->maxEdges, ME2 = m->maxEdges2;
+  c->nC = nC;
+  c->nE = nE;
+  c->nV = nV;
+  c->nT = o.ntracers;
+  c->rsz = o.precision == 64 ? 8 : 4;
+  auto inr = [](int v, int n) { return v >= 1 && v <= n; };
+  for (int i = 0; i < nC; ++i) {
+    int n = m->nEdgesOnCell[i];
+    if (n < 3 || n > ME) return c->fail(TM_ERANGE, "nEdgesOnCell out of range");
+    for (int s = 0; s < n; ++s)
+      if (!inr(m->edgesOnCell[(size_t)i * ME + s], nE) || !inr(m->verticesOnCell[(size_t)i * ME + s], nV))
+        return c->fail(TM_ERANGE, "edgesOnCell/verticesOnCell index out of range");
+  }
+  for (int e = 0; e < nE; ++e) {
+    if (!inr(m->cellsOnEdge[2 * (size_t)e], nC) || !inr(m->cellsOnEdge[2 * (size_t)e + 1], nC) ||
+        !inr(m->verticesOnEdge[2 * (size_t)e], nV) || !inr(m->verticesOnEdge[2 * (size_t)e + 1], nV))
+      return c->fail(TM_ERANGE, "cellsOnEdge/verticesOnEdge index out of range");
+    int n = m->nEdgesOnEdge[e];
+    if (n < 0 || n > kMaxEoE || n > ME2) return c->fail(TM_ESTENCIL, "nEdgesOnEdge exceeds 10");
+    for (int s = 0; s < n; ++s)
+      if (!inr(m->edgesOnEdge[(size_t)e * ME2 + s], nE)) return c->fail(TM_ERANGE, "edgesOnEdge index out of range");
+  }
+  for (int v = 0; v < nV; ++v)
+    for (int j = 0; j < 3; ++j)
+      if (!inr(m->edgesOnVertex[(size_t)v * 3 + j], nE) || !inr(m->cellsOnVertex[(size_t)v * 3 + j], nC))
+        return c->fail(TM_ERANGE, "edgesOnVertex/cellsOnVertex index out of range");
+  if (need_fit)
+    for (int v = 0; v < nV; ++v) {
+      int n = f->nFit2Vertices[v];
+      if (n < 1 || n - 1 > kFit2 || n > f->maxFit) return c->fail(TM_ESTENCIL, "nFit2Vertices exceeds 13");
+      for (int j = 1; j < n; ++j)
+        if (!inr(f->fit2Vertices[(size_t)v * f->maxFit + j], nV))
+          return c->fail(TM_ERANGE, "fit2Vertices index out of range");
+    }
+
+  // Hilbert renumbering: new -> old and old -> new.
+  const bool ren = o.renumber != 0;
+  c->pC = hilbert_order(nC, m->xCell, m->yCell, m->zCell, ren);
+  c->pE = hilbert_order(nE, m->xEdge, m->yEdge, m->zEdge, ren);
+  c->pV = hilbert_order(nV, m->xVertex, m->yVertex, m->zVertex, ren);
+  c->iC.resize(nC);
+  c->iE.resize(nE);
+  c->iV.resize(nV);
+  for (int i = 0; i < nC; ++i) c->iC[c->pC[i]] = i;
+  for (int i = 0; i < nE; ++i) c->iE[c->pE[i]] = i;
+  for (int i = 0; i < nV; ++i) c->iV[c->pV[i]] = i;
+  const double a = o.radius, a2 = a * a;
+
+  // ---- cells
+  std::vector<int8_t> nEoC(nC);
+  std::vector<int> eocS((size_t)kMaxEoC * nC, 0), vocS((size_t)kMaxEoC * nC, 0);
+  std::vector<double> areaCell(nC), kiteC((size_t)kMaxEoC * nC, 0.0);
+  double tot = 0.0;
+  for (int i = 0; i < nC; ++i) tot += m->areaCell[i] * a2;  // mesh_mod.F90:224 order
+  c->totalArea = tot;
+  for (int ni = 0; ni < nC; ++ni) {
+    const int oi = c->pC[ni];
+    const int n = m->nEdgesOnCell[oi];
+    nEoC[ni] = (int8_t)n;
+    areaCell[ni] = m->areaCell[oi] * a2;
+    for (int s = 0; s < n; ++s) {
+      const int oe = m->edgesOnCell[(size_t)oi * ME + s] - 1;
+      const int ne = c->iE[oe];
+      const bool plus = m->cellsOnEdge[2 * (size_t)oe] - 1 == oi;  // nSignEdge = +1
+      eocS[(size_t)s * nC + ni] = plus ? ne : ~ne;
+      const int ov = m->verticesOnCell[(size_t)oi * ME + s] - 1;
+      vocS[(size_t)s * nC + ni] = c->iV[ov];
+      double k = 0.0;
+      for (int j = 0; j < 3; ++j)
+        if (m->cellsOnVertex[(size_t)ov * 3 + j] - 1 == oi) k = m->kiteAreasOnVertex[(size_t)ov * 3 + j] * a2;
+      kiteC[(size_t)s * nC + ni] = k;
+    }
+  }
+  // ---- edges
+  std::vector<int> coe(2 * (size_t)nE), voe(2 * (size_t)nE), eoe((size_t)kMaxEoE * nE, 0);
+  std::vector<int8_t> nEoE(nE);
+  std::vector<double> dv(nE), dc(nE), aE(nE), woe((size_t)kMaxEoE * nE, 0.0), d2v((size_t)2 * kFit2 * nE, 0.0);
+  for (int ne = 0; ne < nE; ++ne) {
+    const int oe = c->pE[ne];
+    coe[ne] = c->iC[m->cellsOnEdge[2 * (size_t)oe] - 1];
+    coe[(size_t)nE + ne] = c->iC[m->cellsOnEdge[2 * (size_t)oe + 1] - 1];
+    voe[ne] = c->iV[m->verticesOnEdge[2 * (size_t)oe] - 1];
+    voe[(size_t)nE + ne] = c->iV[m->verticesOnEdge[2 * (size_t)oe + 1] - 1];
+    dv[ne] = m->dvEdge[oe] * a;
+    dc[ne] = m->dcEdge[oe] * a;
+    aE[ne] = (m->dvEdge[oe] * m->dcEdge[oe]) * a2;  // mesh_mod.F90:178,221
+    const int n = m->nEdgesOnEdge[oe];
+    nEoE[ne] = (int8_t)n;
+    for (int s = 0; s < kMaxEoE; ++s) {
+      // padding slots point at the edge itself with weight 0 (never read)
+      eoe[(size_t)s * nE + ne] = s < n ? c->iE[m->edgesOnEdge[(size_t)oe * ME2 + s] - 1] : ne;
+      woe[(size_t)s * nE + ne] = s < n ? m->weightsOnEdge[(size_t)oe * ME2 + s] : 0.0;
+    }
+    if (need_fit)
+      for (int k = 0; k < 2; ++k)
+        for (int j = 0; j < kFit2; ++j) {
+          const int slot = j + 1;
+          d2v[((size_t)k * kFit2 + j) * nE + ne] =
+              slot < f->maxFit ? f->deriv2OnVertex[((size_t)oe * 2 + k) * f->maxFit + slot] : 0.0;
+        }
+  }
+  // ---- vertices
+  std::vector<int> eovS((size_t)3 * nV), cov((size_t)3 * nV), fit2V((size_t)kFit2 * nV, 0);
+  std::vector<int8_t> nFit2V(nV, 1);
+  std::vector<double> kite((size_t)3 * nV), areaTri(nV), fV(nV);
+  double sat = 0.0;
+  for (int v = 0; v < nV; ++v) sat += m->areaTriangle[v] * a2;
+  c->sumAreaTri = sat;
+  for (int nv = 0; nv < nV; ++nv) {
+    const int ov = c->pV[nv];
+    areaTri[nv] = m->areaTriangle[ov] * a2;
+    fV[nv] = 2.0 * 7.292e-5 * std::sin(m->latVertex[ov]);  // mesh_mod.F90:180, const_mod.F90:13
+    for (int j = 0; j < 3; ++j) {
+      const int oe = m->edgesOnVertex[(size_t)ov * 3 + j] - 1;
+      const bool plus = m->verticesOnEdge[2 * (size_t)oe] - 1 == ov;  // tSignEdge = +1
+      const int ne = c->iE[oe];
+      eovS[(size_t)j * nV + nv] = plus ? ne : ~ne;
+      cov[(size_t)j * nV + nv] = c->iC[m->cellsOnVertex[(size_t)ov * 3 + j] - 1];
+      kite[(size_t)j * nV + nv] = m->kiteAreasOnVertex[(size_t)ov * 3 + j] * a2;
+    }
+    if (need_fit) {
+      const int n = f->nFit2Vertices[ov];
+      nFit2V[nv] = (int8_t)n;
+      for (int j = 0; j < kFit2; ++j)
+        fit2V[(size_t)j * nV + nv] = j + 1 < n ? c->iV[f->fit2Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
+    }
+  }
+
+  // ---- device upload
+  HIPCHK(c, hipSetDevice(o.device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreate(&c->ev0));
+  HIPCHK(c, hipEventCreate(&c->ev1));
+  DMesh &d = c->dm;
+  d.nC = nC;
+  d.nE = nE;
+  d.nV = nV;
+  d.nbC = round_blocks(nC);
+  d.nbE = round_blocks(nE);
+  d.nbV = round_blocks(nV);
+#define UP(field, vec, T)                                  \
+  do {                                                     \
+    T *p_ = c->alloc<T>(vec.size());                       \
+    if (!p_) return c->fail(TM_ENOMEM, "hipMalloc " #vec); \
+    int rc_ = upload(c, p_, vec);                          \
+    if (rc_) return rc_;                                   \
+    d.field = p_;                                          \
+  } while (0)
+  UP(nEoC, nEoC, int8_t);
+  UP(eocS, eocS, int);
+  UP(areaCell, areaCell, double);
+  UP(coe, coe, int);
+  UP(voe, voe, int);
+  UP(dv, dv, double);
+  UP(dc, dc, double);
+  UP(aE, aE, double);
+  UP(nEoE, nEoE, int8_t);
+  UP(eoe, eoe, int);
+  UP(woe, woe, double);
+  UP(d2v, d2v, double);
+  UP(eovS, eovS, int);
+  UP(cov, cov, int);
+  UP(kite, kite, double);
+  UP(areaTri, areaTri, double);
+  UP(fV, fV, double);
+  UP(nFit2V, nFit2V, int8_t);
+  UP(fit2V, fit2V, int);
+  UP(vocS, vocS, int);
+  UP(kiteC, kiteC, double);
+#undef UP
+
+  const size_t nq = (size_t)c->nT * nC;
+  void **rbufs_c[] = {&c->gd_old, &c->ghs, &c->gdS[0], &c->gdS[1], &c->acc_gd, &c->t_gd[0], &c->t_gd[1],
+                      &c->E, &c->tend_gd, &c->scratch_c[0], &c->scratch_c[1]};
+  for (void **p : rbufs_c)
+    if (!(*p = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc cell state");
+  void **rbufs_e[] = {&c->iap_old, &c->iap_s, &c->acc_iap, &c->t_iap[0], &c->t_iap[1], &c->gd_e, &c->u,
+                      &c->Fdv, &c->kee, &c->udc, &c->pv_e, &c->scratch_e[0], &c->scratch_e[1],
+                      &c->scratch_e[2], &c->scratch_e[3]};
+  for (void **p : rbufs_e)
+    if (!(*p = c->alloc_r(nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
+  void **rbufs_v[] = {&c->pv_v, &c->scratch_v[0], &c->scratch_v[1]};
+  for (void **p : rbufs_v)
+    if (!(*p = c->alloc_r(nV))) return c->fail(TM_ENOMEM, "hipMalloc vertex state");
+  void **rbufs_q[] = {&c->q_old, &c->q_s, &c->acc_q};
+  for (void **p : rbufs_q)
+    if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
+  if (!(c->qFdv = c->alloc_r(std::max<size_t>((size_t)c->nT * nE, 1)))) return c->fail(TM_ENOMEM, "hipMalloc qFdv");
+  c->part_stride = std::max(d.nbE, std::max(d.nbC, d.nbV));
+  if (!(c->part = c->alloc<double2>((size_t)kNumSums * c->part_stride))) return c->fail(TM_ENOMEM, "hipMalloc part");
+  HIPCHK(c, hipMemsetAsync(c->part, 0, sizeof(double2) * kNumSums * c->part_stride, c->stream));
+  if (!(c->d_beta = c->alloc<double>(1)) || !(c->hist_idx = c->alloc<int>(1)) ||
+      !(c->beta_hist = c->alloc<double>(c->hist_cap)) || !(c->inv_hist = c->alloc<double>(4 * (size_t)c->hist_cap)))
+    return c->fail(TM_ENOMEM, "hipMalloc history");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+
+  // Algorithmic bytes (compulsory traffic, actual stencil nnz): constants of
+  // one stage + 56 B (FP64) per prognostic element, finalize 24 B per element.
+  double sumEoE = 0.0, sumFitE = 0.0, sumFitV = 0.0, sumEoC = 0.0;
+  for (int e = 0; e < nE; ++e) sumEoE += m->nEdgesOnEdge[e];
+  for (int i = 0; i < nC; ++i) sumEoC += m->nEdgesOnCell[i];
+  if (need_fit) {
+    for (int e = 0; e < nE; ++e)
+      sumFitE += (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e] - 1] - 1) +
+                 (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e + 1] - 1] - 1);
+    for (int v = 0; v < nV; ++v) sumFitV += f->nFit2Vertices[v] - 1;
+  }
+  const double cst = nE * (8.0 + 8.0 + 8.0 + 8.0) + sumEoE * 12.0 + sumFitE * 8.0 +
+                     nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0) + sumFitV * 4.0 + sumEoC * 4.0 + nC * (8.0 + 8.0);
+  const double rs = (double)c->rsz;
+  const double prog = (double)nE + (double)nC * (1 + c->nT);
+  c->step_bytes[0] = cst + 7.0 * rs * prog;
+  c->step_bytes[1] = 3.0 * rs * prog;
+  c->step_bytes[2] = 4.0 * c->step_bytes[0] + c->step_bytes[1];
+  // Compulsory bytes of one launch of each kernel (stage 2/3 pattern).
+  const double T = c->nT;
+  c->kbytes[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs + T * rs) + nC * rs * (1.0 + T);
+  c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * (2.0 + T) +
+                                T * nC * 4.0 * rs + nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
+  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + 8.0 + 2.0 * rs) + 12.0 * sumEoE + 8.0 * sumFitE +
+                            nV * (rs + 1.0) + 4.0 * sumFitV;
+  c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + 8.0 + 9.0 * rs) + 12.0 * sumEoE + nC * 2.0 * rs;
+  c->kbytes[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
+  c->kbytes[TM_K_FINALIZE] = 3.0 * rs * prog;
+  return TM_OK;
+}
+
+extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts *o) {
+  if (!out || !m || !o) return TM_EINVAL;
+  *out = nullptr;
+  tm_ctx *c = new tm_ctx();
+  c->o = *o;
+  int rc = create_impl(c, m, f);
+  if (rc) {
+    std::fprintf(stderr, "tm_ctx_create: %s\n", c->err.c_str());
+    tm_ctx_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return TM_OK;
+}
+
+extern "C" int tm_state_upload(tm_ctx *c, const double *gd, const double *u, const double *iap_u,
+                               const double *ghs, const double *tracers) {
+  if (!c) return TM_EINVAL;
+  if (!gd || !ghs || (!u && !iap_u) || (c->nT > 0 && !tracers)) return c->fail(TM_EINVAL, "missing state array");
+  if (!finite_all(gd, c->nC) || !finite_all(ghs, c->nC) || (iap_u && !finite_all(iap_u, c->nE)) ||
+      (!iap_u && !finite_all(u, c->nE)) || (c->nT > 0 && !finite_all(tracers, (size_t)c->nT * c->nC)))
+    return c->fail(TM_ENONFINITE, "non-finite value in the uploaded state");
+  for (int i = 0; i < c->nC; ++i)
+    if (!(gd[i] > 0.0)) return c->fail(TM_EINVAL, "gd must be positive (sqrt of gd_edge)");
+  int rc;
+  if ((rc = upload_state(c, c->gd_old, gd, c->pC, 1, c->nC))) return rc;
+  if ((rc = upload_state(c, c->ghs, ghs, c->pC, 1, c->nC))) return rc;
+  if (c->nT > 0 && (rc = upload_state(c, c->q_old, tracers, c->pC, c->nT, c->nC))) return rc;
+  if (iap_u) {
+    if ((rc = upload_state(c, c->iap_old, iap_u, c->pE, 1, c->nE))) return rc;
+  } else {
+    if ((rc = upload_state(c, c->scratch_e[0], u, c->pE, 1, c->nE))) return rc;
+    const int nb = (c->nE + kBlock - 1) / kBlock;
+    if (c->rsz == 8)
+      hipLaunchKernelGGL(k_iap_from_u<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, (const double *)c->gd_old,
+                         (const double *)c->scratch_e[0], (double *)c->iap_old);
+    else
+      hipLaunchKernelGGL(k_iap_from_u<float>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, (const float *)c->gd_old,
+                         (const float *)c->scratch_e[0], (float *)c->iap_old);
+    HIPCHK(c, hipGetLastError());
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  c->have_state = true;
+  return TM_OK;
+}
+
+extern "C" int tm_state_download(tm_ctx *c, double *gd, double *u, double *iap_u, double *tracers) {
+  if (!c) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
+  int rc;
+  if (gd && (rc = download_state(c, gd, c->gd_old, c->pC, 1, c->nC))) return rc;
+  if (iap_u && (rc = download_state(c, iap_u, c->iap_old, c->pE, 1, c->nE))) return rc;
+  if (tracers && c->nT > 0 && (rc = download_state(c, tracers, c->q_old, c->pC, c->nT, c->nC))) return rc;
+  if (u) {
+    rc = c->rsz == 8 ? launch_stage<double>(c, stage_bufs<double>(c, 0, false), false)
+                     : launch_stage<float>(c, stage_bufs<float>(c, 0, false), false);
+    if (rc) return rc;
+    if ((rc = download_state(c, u, c->u, c->pE, 1, c->nE))) return rc;
+  }
+  return TM_OK;
+}
+
+static int step_impl(tm_ctx *c, int nsteps, double *inv_out, double *beta_out) {
+  if (!c) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "tm_step before tm_state_upload");
+  if (nsteps < 0) return c->fail(TM_EINVAL, "nsteps < 0");
+  int done = 0;
+  double ms_total = 0.0;
+  while (done < nsteps) {
+    const int n = std::min(nsteps - done, c->hist_cap);
+    int rc = run_steps(c, n, true);
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->ev0, c->ev1));
+    ms_total += ms;
+    std::vector<double> bh(n), ih(4 * (size_t)n);
+    HIPCHK(c, hipMemcpy(bh.data(), c->beta_hist, n * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(ih.data(), c->inv_hist, 4 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    if (beta_out) std::memcpy(beta_out + done, bh.data(), n * sizeof(double));
+    if (inv_out) std::memcpy(inv_out + 4 * (size_t)done, ih.data(), 4 * (size_t)n * sizeof(double));
+    if (!finite_all(ih.data(), ih.size()) || !finite_all(bh.data(), bh.size()))
+      return c->fail(TM_ENONFINITE, "non-finite invariant or beta: the state blew up");
+    done += n;
+  }
+  c->err.clear();
+  (void)ms_total;
+  return TM_OK;
+}
+
+extern "C" int tm_step(tm_ctx *c, int nsteps, double *invariants_out, double *beta_out) {
+  return step_impl(c, nsteps, invariants_out, beta_out);
+}
+
+extern "C" int tm_step_async(tm_ctx *c, int nsteps) {
+  if (!c) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "tm_step_async before tm_state_upload");
+  if (nsteps < 0 || nsteps > c->hist_cap) return c->fail(TM_EINVAL, "nsteps out of range for one batch");
+  return run_steps(c, nsteps, false);
+}
+
+extern "C" int tm_sync(tm_ctx *c) {
+  if (!c) return TM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+extern "C" int tm_last_elapsed_ms(tm_ctx *c, double *ms) {
+  if (!c || !ms) return TM_EINVAL;
+  if (!c->timed) return c->fail(TM_ESTATE, "nothing timed yet");
+  HIPCHK(c, hipEventSynchronize(c->ev1));
+  float f = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&f, c->ev0, c->ev1));
+  *ms = f;
+  return TM_OK;
+}
+
+extern "C" int tm_step_bytes(tm_ctx *c, double bytes[3]) {
+  if (!c || !bytes) return TM_EINVAL;
+  for (int i = 0; i < 3; ++i) bytes[i] = c->step_bytes[i];
+  return TM_OK;
+}
+
+extern "C" int tm_kernel_bytes(tm_ctx *c, double bytes[TM_K_COUNT]) {
+  if (!c || !bytes) return TM_EINVAL;
+  for (int i = 0; i < TM_K_COUNT; ++i) bytes[i] = c->kbytes[i];
+  return TM_OK;
+}
+
+// Eager launches of one step with an event pair round every kernel.
+template <typename R>
+static int profile_one_step(tm_ctx *c, std::vector<hipEvent_t> &ev, double *ms, int *launches) {
+  struct Rec {
+    int kind, e0;
+  };
+  std::vector<Rec> recs;
+  int next = 0;
+  auto mark = [&](int kind) -> int {
+    recs.push_back({kind, next});
+    HIPCHK(c, hipEventRecord(ev[next++], c->stream));
+    return TM_OK;
+  };
+  auto done = [&]() -> int {
+    HIPCHK(c, hipEventRecord(ev[next++], c->stream));
+    return TM_OK;
+  };
+  const DMesh &m = c->dm;
+  int rc = TM_OK;
+  for (int k = 1; k <= 4; ++k) {
+    StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
+    if ((rc = mark(TM_K_EDGE_PREP))) return rc;
+    hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    if ((rc = done()) || (rc = mark(TM_K_CELL_VERTEX))) return rc;
+    hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
+    if ((rc = done()) || (rc = mark(TM_K_EDGE_PV))) return rc;
+    hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    if ((rc = done()) || (rc = mark(TM_K_EDGE_TEND))) return rc;
+    hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    if ((rc = done())) return rc;
+  }
+  if ((rc = mark(TM_K_REDUCE))) return rc;
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 1, true));
+  if ((rc = done())) return rc;
+  FinalBufs<R> f{(R *)c->gd_old, (R *)c->iap_old, (R *)c->q_old, (const R *)c->acc_gd, (const R *)c->acc_iap,
+                 (const R *)c->acc_q, c->d_beta, c->o.dt, c->nT};
+  if ((rc = mark(TM_K_FINALIZE))) return rc;
+  hipLaunchKernelGGL(k_finalize<R>, dim3(m.nbE + m.nbC), dim3(kBlock), 0, c->stream, m, f);
+  if ((rc = done())) return rc;
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (const Rec &r : recs) {
+    float t = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&t, ev[r.e0], ev[r.e0 + 1]));
+    ms[r.kind] += t;
+    launches[r.kind] += 1;
+  }
+  return TM_OK;
+}
+
+extern "C" int tm_profile_steps(tm_ctx *c, int nsteps, double ms[TM_K_COUNT], int launches[TM_K_COUNT]) {
+  if (!c || !ms || !launches) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "tm_profile_steps before tm_state_upload");
+  for (int i = 0; i < TM_K_COUNT; ++i) {
+    ms[i] = 0.0;
+    launches[i] = 0;
+  }
+  std::vector<hipEvent_t> ev(40, nullptr);
+  for (auto &e : ev) HIPCHK(c, hipEventCreate(&e));
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
+  int rc = TM_OK;
+  for (int s = 0; s < nsteps && rc == TM_OK; ++s)
+    rc = c->rsz == 8 ? profile_one_step<double>(c, ev, ms, launches) : profile_one_step<float>(c, ev, ms, launches);
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  return rc;
+}
+
+extern "C" int tm_invariants(tm_ctx *c, double out[4]) {
+  if (!c || !out) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
+  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
+  // counter 1 so that k_reduce's mode-0 pass writes slot 0
+  int one = 1;
+  HIPCHK(c, hipMemcpyAsync(c->hist_idx, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
+  int rc = c->rsz == 8 ? enqueue_refresh<double>(c) : enqueue_refresh<float>(c);
+  if (rc) return rc;
+  HIPCHK(c, hipMemcpyAsync(out, c->inv_hist, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+template <typename R>
+static int stage_impl(tm_ctx *c, tm_stage_out *out) {
+  StageBufs<R> b = stage_bufs<R>(c, 1, false);
+  // diagnostics into scratch / dedicated buffers
+  b.d_ke = (R *)c->scratch_c[0];
+  b.d_vor = (R *)c->scratch_v[0];
+  b.d_gdv = (R *)c->scratch_v[1];
+  b.d_v = (R *)c->scratch_e[0];
+  b.d_pvflx = (R *)c->scratch_e[1];
+  b.d_utend = (R *)c->scratch_e[2];
+  b.d_tiap = (R *)c->scratch_e[3];
+  void *tq = nullptr;
+  if (c->nT > 0) {
+    if (!(tq = c->alloc_r((size_t)c->nT * c->nC))) return c->fail(TM_ENOMEM, "hipMalloc tend_q");
+    b.d_tq = (R *)tq;
+  }
+  int rc = launch_stage<R>(c, b);
+  if (rc) return rc;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  struct {
+    double *dst;
+    void *src;
+    const std::vector<int> *perm;
+    int n, copies;
+  } outs[] = {
+      {out->tend_gd, c->tend_gd, &c->pC, c->nC, 1},   {out->tend_iap, c->scratch_e[3], &c->pE, c->nE, 1},
+      {out->tend_q, tq, &c->pC, c->nC, c->nT},        {out->gd_edge, c->gd_e, &c->pE, c->nE, 1},
+      {out->u, c->u, &c->pE, c->nE, 1},               {out->ke, c->scratch_c[0], &c->pC, c->nC, 1},
+      {out->vor, c->scratch_v[0], &c->pV, c->nV, 1},  {out->gd_vertex, c->scratch_v[1], &c->pV, c->nV, 1},
+      {out->pv_vertex, c->pv_v, &c->pV, c->nV, 1},    {out->v, c->scratch_e[0], &c->pE, c->nE, 1},
+      {out->pv_edge, c->pv_e, &c->pE, c->nE, 1},      {out->pv_flx, c->scratch_e[1], &c->pE, c->nE, 1},
+      {out->u_tend, c->scratch_e[2], &c->pE, c->nE, 1},
+  };
+  for (auto &x : outs)
+    if (x.dst && x.src && x.copies > 0 && (rc = download_state(c, x.dst, x.src, *x.perm, x.copies, x.n))) return rc;
+  if (tq) {
+    (void)hipFree(tq);
+    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), tq));
+  }
+  return TM_OK;
+}
+
+extern "C" int tm_stage(tm_ctx *c, tm_stage_out *out) {
+  if (!c || !out) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "tm_stage before tm_state_upload");
+  return c->rsz == 8 ? stage_impl<double>(c, out) : stage_impl<float>(c, out);
+}
+
+#include "ops_kernels.h"

@@ -1,0 +1,230 @@
+"""Python host mirror of libtmgpu (include/tmcore_gpu.h).
+
+``SwmGpu`` follows the reference driver's shape (tmcore_swm_mod.F90:29-90):
+create = mesh_init + poly_fit_init + state_init, ``upload`` = the initial
+condition + the run prologue, ``step`` = the time loop with per-step
+invariants and beta.  The single operators keep the names and argument order
+of operators_mod.F90.  Errors raise TmError with the library's message, the
+counterpart of the reference's log_error + stop.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _ffi
+from .mesh import RADIUS, Fit, Mesh
+
+_dp = C.POINTER(C.c_double)
+_vp = C.c_void_p
+
+
+class Opts(C.Structure):
+    _fields_ = [("time_scheme", C.c_int), ("pv_scheme", C.c_int), ("interp_pv_order", C.c_int),
+                ("adv_order", C.c_int), ("adv_mono", C.c_int), ("conserve_energy", C.c_int),
+                ("apvm_weight", C.c_double), ("dt", C.c_double), ("radius", C.c_double),
+                ("ntracers", C.c_int), ("precision", C.c_int), ("device", C.c_int), ("renumber", C.c_int)]
+
+
+class StageOut(C.Structure):
+    _fields_ = [(k, _dp) for k in ("tend_gd", "tend_iap", "tend_q", "gd_edge", "u", "ke", "vor", "gd_vertex",
+                                   "pv_vertex", "v", "pv_edge", "pv_flx", "u_tend")]
+
+
+STAGE_WHERE = {"tend_gd": "c", "tend_iap": "e", "gd_edge": "e", "u": "e", "ke": "c", "vor": "v",
+               "gd_vertex": "v", "pv_vertex": "v", "v": "e", "pv_edge": "e", "pv_flx": "e", "u_tend": "e"}
+
+_gpu = None
+
+
+def gpu_lib() -> C.CDLL:
+    """libtmgpu.so; raises if it is not built (there is no fallback path)."""
+    global _gpu
+    if _gpu is None:
+        L = _ffi._load("libtmgpu.so")
+        L.tm_opts_default.argtypes = [C.POINTER(Opts)]
+        L.tm_opts_default.restype = None
+        L.tm_ctx_create.argtypes = [C.POINTER(_vp), C.POINTER(_ffi.MeshDesc), C.POINTER(_ffi.FitDesc),
+                                    C.POINTER(Opts)]
+        L.tm_ctx_destroy.argtypes = [_vp]
+        L.tm_last_error.argtypes = [_vp]
+        L.tm_last_error.restype = C.c_char_p
+        L.tm_state_upload.argtypes = [_vp, _dp, _dp, _dp, _dp, _dp]
+        L.tm_state_download.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        L.tm_step.argtypes = [_vp, C.c_int, _dp, _dp]
+        L.tm_step_async.argtypes = [_vp, C.c_int]
+        L.tm_sync.argtypes = [_vp]
+        L.tm_invariants.argtypes = [_vp, _dp]
+        L.tm_stage.argtypes = [_vp, C.POINTER(StageOut)]
+        L.tm_last_elapsed_ms.argtypes = [_vp, _dp]
+        L.tm_step_bytes.argtypes = [_vp, _dp]
+        L.tm_profile_steps.argtypes = [_vp, C.c_int, _dp, C.POINTER(C.c_int)]
+        L.tm_kernel_bytes.argtypes = [_vp, _dp]
+        L.tm_profile_steps.restype = C.c_int
+        L.tm_kernel_bytes.restype = C.c_int
+        for n in ("div", "curl", "c2e", "c2v", "v2c", "v2e", "ke", "tangent_wind"):
+            getattr(L, "tm_op_" + n).argtypes = [_vp, _dp, _dp]
+        L.tm_op_tangent_vor_flux.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        for f in ("tm_ctx_create", "tm_ctx_destroy", "tm_state_upload", "tm_state_download", "tm_step",
+                  "tm_step_async", "tm_sync", "tm_invariants", "tm_stage", "tm_last_elapsed_ms", "tm_step_bytes"):
+            getattr(L, f).restype = C.c_int
+        _gpu = L
+    return _gpu
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(_dp)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+class SwmGpu:
+    """One shallow-water model on the device (CRK4, pv_scheme 3 by default)."""
+
+    def __init__(self, mesh: Mesh, fit: Fit | None = None, *, dt: float = 600.0, radius: float = RADIUS,
+                 ntracers: int = 0, precision: int = 64, interp_pv_order: int = 3, conserve_energy: bool = True,
+                 renumber: bool = True, device: int = 0, **scheme):
+        L = gpu_lib()
+        o = Opts()
+        L.tm_opts_default(C.byref(o))
+        o.dt, o.radius, o.ntracers, o.precision = dt, radius, ntracers, precision
+        o.interp_pv_order, o.conserve_energy, o.renumber, o.device = interp_pv_order, int(conserve_energy), \
+            int(renumber), device
+        for k, v in scheme.items():
+            setattr(o, k, v)
+        self.opts = o
+        self.mesh = mesh
+        if fit is None and interp_pv_order >= 3:
+            fit = mesh.fit(radius)
+        self.fit = fit
+        self.nC, self.nE, self.nV, self.nT = mesh.nCells, mesh.nEdges, mesh.nVertices, ntracers
+        h = _vp()
+        md = mesh.desc()
+        fd = fit.desc() if fit is not None else None
+        rc = L.tm_ctx_create(C.byref(h), C.byref(md), C.byref(fd) if fd is not None else None, C.byref(o))
+        if rc != 0:
+            raise _ffi.TmError(rc, "tm_ctx_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            gpu_lib().tm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, where):
+        if rc != 0:
+            msg = gpu_lib().tm_last_error(self.h)
+            raise _ffi.TmError(rc, where, msg.decode() if msg else "")
+
+    # -- state
+    def upload(self, gd, ghs, u=None, iap_u=None, tracers=None):
+        gd, ghs, u, iap_u, tracers = (_f64(x) for x in (gd, ghs, u, iap_u, tracers))
+        self._check(gpu_lib().tm_state_upload(self.h, _p(gd), _p(u), _p(iap_u), _p(ghs), _p(tracers)),
+                    "tm_state_upload")
+
+    def download(self):
+        """Returns dict gd, u, iap_u, tracers (reference order)."""
+        gd = np.zeros(self.nC)
+        u = np.zeros(self.nE)
+        iap = np.zeros(self.nE)
+        tr = np.zeros((self.nT, self.nC))
+        self._check(gpu_lib().tm_state_download(self.h, _p(gd), _p(u), _p(iap), _p(tr) if self.nT else None),
+                    "tm_state_download")
+        return {"gd": gd, "u": u, "iap_u": iap, "tracers": tr}
+
+    # -- time loop
+    def step(self, nsteps: int):
+        """nsteps CRK4 steps; returns (invariants[nsteps, 4], beta[nsteps])."""
+        inv = np.zeros((nsteps, 4))
+        beta = np.zeros(nsteps)
+        self._check(gpu_lib().tm_step(self.h, nsteps, _p(inv), _p(beta)), "tm_step")
+        return inv, beta
+
+    def step_async(self, nsteps: int):
+        self._check(gpu_lib().tm_step_async(self.h, nsteps), "tm_step_async")
+
+    def sync(self):
+        self._check(gpu_lib().tm_sync(self.h), "tm_sync")
+
+    def last_elapsed_ms(self) -> float:
+        ms = C.c_double()
+        self._check(gpu_lib().tm_last_elapsed_ms(self.h, C.byref(ms)), "tm_last_elapsed_ms")
+        return ms.value
+
+    def step_bytes(self):
+        b = np.zeros(3)
+        self._check(gpu_lib().tm_step_bytes(self.h, _p(b)), "tm_step_bytes")
+        return {"stage": b[0], "finalize": b[1], "step": b[2]}
+
+    KERNELS = ("k_edge_prep", "k_cell_vertex", "k_edge_pv", "k_edge_tend", "k_reduce", "k_finalize")
+
+    def profile(self, nsteps: int):
+        """Per-kernel device time over nsteps eager steps (HIP events on the
+        context's stream): {kernel: (avg_ms, launches, bytes_per_launch)}."""
+        ms = np.zeros(6)
+        n = (C.c_int * 6)()
+        self._check(gpu_lib().tm_profile_steps(self.h, nsteps, _p(ms), n), "tm_profile_steps")
+        by = np.zeros(6)
+        self._check(gpu_lib().tm_kernel_bytes(self.h, _p(by)), "tm_kernel_bytes")
+        return {k: (ms[i] / max(n[i], 1), n[i], by[i]) for i, k in enumerate(self.KERNELS)}
+
+    def invariants(self):
+        out = np.zeros(4)
+        self._check(gpu_lib().tm_invariants(self.h, _p(out)), "tm_invariants")
+        return out
+
+    def stage(self):
+        """One spatial_operators call on the current state: dict of outputs."""
+        n = {"c": self.nC, "e": self.nE, "v": self.nV}
+        res = {k: np.zeros(n[w]) for k, w in STAGE_WHERE.items()}
+        res["tend_q"] = np.zeros((self.nT, self.nC))
+        so = StageOut(**{k: _p(v) for k, v in res.items() if k != "tend_q"})
+        so.tend_q = _p(res["tend_q"]) if self.nT else None
+        self._check(gpu_lib().tm_stage(self.h, C.byref(so)), "tm_stage")
+        return res
+
+    # -- single operators (operators_mod.F90 names)
+    def _op(self, name, out_where, *args):
+        arrs = [_f64(a) for a in args]
+        out = np.zeros({"c": self.nC, "e": self.nE, "v": self.nV}[out_where])
+        self._check(getattr(gpu_lib(), "tm_op_" + name)(self.h, *[_p(a) for a in arrs], _p(out)), "tm_op_" + name)
+        return out
+
+    def div_operator(self, f_edge):
+        return self._op("div", "c", f_edge)
+
+    def curl_operator(self, f_edge):
+        return self._op("curl", "v", f_edge)
+
+    def scalar_c2e_interp_operator(self, f_cell):
+        return self._op("c2e", "e", f_cell)
+
+    def scalar_c2v_interp_operator(self, f_cell):
+        return self._op("c2v", "v", f_cell)
+
+    def scalar_v2c_interp_operator(self, f_vertex):
+        return self._op("v2c", "c", f_vertex)
+
+    def scalar_v2e_interp_operator(self, f_vertex):
+        return self._op("v2e", "e", f_vertex)
+
+    def calc_kinetic_energy(self, u_edge):
+        return self._op("ke", "c", u_edge)
+
+    def calc_tangent_wind(self, u_edge):
+        return self._op("tangent_wind", "e", u_edge)
+
+    def calc_tangent_vor_flux(self, u_edge, gd_edge, pv_edge):
+        return self._op("tangent_vor_flux", "e", u_edge, gd_edge, pv_edge)
