@@ -99,7 +99,10 @@ def test_one_step_matches_oracle(level):
     assert rel_l2(st["iap_u"], iap1) <= 1e-12
     assert rel_l2(st["tracers"], q1) <= 1e-12
     assert abs(gbeta[0] - beta[0]) <= 1e-12
-    np.testing.assert_allclose(ginv[0], inv[0], rtol=1e-12)
+    np.testing.assert_allclose(ginv[0, :3], inv[0, :3], rtol=1e-12)
+    # total absolute vorticity is zero up to round-off (f is antisymmetric and
+    # the curl telescopes): compare on the scale of 2*Omega
+    assert abs(ginv[0, 3] - inv[0, 3]) <= 1e-12 * 2 * 7.292e-5
     g.close()
 
 
@@ -118,7 +121,7 @@ def test_one_day_matches_oracle_and_invariant_drift():
     drift_o = inv[:, :3] / inv0[:3] - 1
     drift_g = ginv[:, :3] / inv0[:3] - 1
     assert np.abs(drift_g - drift_o).max() <= 1e-12
-    assert np.abs(ginv[:, 3] - inv[:, 3]).max() <= 1e-12 * np.abs(inv0[3]) + 1e-20
+    assert np.abs(ginv[:, 3] - inv[:, 3]).max() <= 1e-12 * 2 * 7.292e-5
     g.close()
 
 

@@ -128,19 +128,27 @@ __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b,
   const int c = tile * kBlock + threadIdx.x;
   double mass = 0.0, enc = 0.0, rp = 0.0, r44 = 0.0;
   if (c < m.nC) {
+    // Unused slots of pentagons hold index 0 (a valid edge): every gather is
+    // issued unconditionally so they overlap, and the unused terms are
+    // replaced by an exact +0 afterwards.
     const int n = m.nEoC[c];
     int idx[kMaxEoC];
 #pragma unroll
-    for (int s = 0; s < kMaxEoC; ++s) idx[s] = s < n ? m.eocS[s * m.nC + c] : 0;
+    for (int s = 0; s < kMaxEoC; ++s) idx[s] = m.eocS[s * m.nC + c];
+    R fv[kMaxEoC], kv[kMaxEoC];
+#pragma unroll
+    for (int s = 0; s < kMaxEoC; ++s) {
+      const int e = unsign(idx[s]);
+      fv[s] = b.Fdv[e];
+      kv[s] = b.kee[e];
+    }
     R sdiv = R(0), ske = R(0);
 #pragma unroll
-    for (int s = 0; s < kMaxEoC; ++s)
-      if (s < n) {
-        const int e = unsign(idx[s]);
-        const R f = b.Fdv[e];
-        sdiv += idx[s] < 0 ? -f : f;
-        ske += b.kee[e];
-      }
+    for (int s = 0; s < kMaxEoC; ++s) {
+      const R f = idx[s] < 0 ? -fv[s] : fv[s];
+      sdiv += s < n ? f : R(0);
+      ske += s < n ? kv[s] : R(0);
+    }
     const double A = m.areaCell[c];
     const R tend = (-sdiv) / (R)A;
     const R ke = (ske * R(0.25)) / (R)A;
@@ -170,13 +178,15 @@ __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b,
     }
     for (int t = 0; t < b.nT; ++t) {
       const R *qf = b.qFdv + (size_t)t * m.nE;
+      R qv[kMaxEoC];
+#pragma unroll
+      for (int s = 0; s < kMaxEoC; ++s) qv[s] = qf[unsign(idx[s])];
       R sq = R(0);
 #pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s)
-        if (s < n) {
-          const R f = qf[unsign(idx[s])];
-          sq += idx[s] < 0 ? -f : f;
-        }
+      for (int s = 0; s < kMaxEoC; ++s) {
+        const R f = idx[s] < 0 ? -qv[s] : qv[s];
+        sq += s < n ? f : R(0);
+      }
       const R tq = (-sq) / (R)A;
       const size_t o = (size_t)t * m.nC + c;
       if (b.d_tq) b.d_tq[o] = tq;
@@ -258,22 +268,41 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
   const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
   if (e >= m.nE) return;
-  const int n = m.nEoE[e];
+  // Padded stencil slots point at the element itself with weight 0, so every
+  // load is issued unconditionally and a padded term adds an exact +0.
   R v = R(0);
+  {
+    int ep[kMaxEoE];
+    R uv[kMaxEoE];
 #pragma unroll
-  for (int s = 0; s < kMaxEoE; ++s)
-    if (s < n) v += b.u[m.eoe[s * m.nE + e]] * (R)m.woe[s * m.nE + e];
+    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) uv[s] = b.u[ep[s]];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
+  }
   const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
   const R p1 = b.pv_v[v1], p2 = b.pv_v[v2];
   R pve = R(0.5) * (p1 + p2);
   if (b.pv_order == 3 || b.pv_order == 4) {
     const R coef2 = b.pv_order == 3 ? R(1) : R(0);
-    const int n1 = m.nFit2V[v1] - 1, n2 = m.nFit2V[v2] - 1;
+    int f1[kFit2], f2[kFit2];
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      f1[j] = m.fit2V[j * m.nV + v1];
+      f2[j] = m.fit2V[j * m.nV + v2];
+    }
+    R q1[kFit2], q2[kFit2];
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      q1[j] = b.pv_v[f1[j]];
+      q2[j] = b.pv_v[f2[j]];
+    }
     R d1 = R(0), d2 = R(0);
 #pragma unroll
     for (int j = 0; j < kFit2; ++j) {
-      if (j < n1) d1 += (R)m.d2v[(size_t)j * m.nE + e] * (b.pv_v[m.fit2V[j * m.nV + v1]] - p1);
-      if (j < n2) d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (b.pv_v[m.fit2V[j * m.nV + v2]] - p2);
+      d1 += (R)m.d2v[(size_t)j * m.nE + e] * (q1[j] - p1);
+      d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (q2[j] - p2);
     }
     const R dv = (R)m.dv[e];
     pve = pve - ((dv * dv) * ((d1 + d2) - (sign1(v) * coef2) * (d2 - d1))) / R(12);
@@ -288,15 +317,21 @@ __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   const int e = tile * kBlock + threadIdx.x;
   double rp = 0.0, r44 = 0.0;
   if (e < m.nE) {
-    const int n = m.nEoE[e];
     const R pve = b.pv_e[e];
+    int ep[kMaxEoE];
+    R uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
+#pragma unroll
+    for (int q = 0; q < kMaxEoE; ++q) ep[q] = m.eoe[q * m.nE + e];
+#pragma unroll
+    for (int q = 0; q < kMaxEoE; ++q) {
+      uv[q] = b.u[ep[q]];
+      gv[q] = b.gd_e[ep[q]];
+      pv[q] = b.pv_e[ep[q]];
+    }
     R s = R(0);
 #pragma unroll
     for (int q = 0; q < kMaxEoE; ++q)
-      if (q < n) {
-        const int ep = m.eoe[q * m.nE + e];
-        s = s + ((((R)m.woe[q * m.nE + e] * b.u[ep]) * b.gd_e[ep]) * R(0.5)) * (pve + b.pv_e[ep]);
-      }
+      s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
     const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
     const R dEdx = (b.E[c2] - b.E[c1]) / (R)m.dc[e];
     const R ut = s - dEdx;
@@ -336,88 +371,112 @@ __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   }
 }
 
-// Sums the block partials.  mode 1: end of a CRK4 step -> beta (and the
-// invariants of the step's base state when have_inv); mode 0: invariants only.
+// Global sums: block i of k_reduce folds the block partials of sum i
+// (double-double) into sums[i].  beta and the invariants are formed from
+// sums[] by whoever consumes them (k_finalize, k_inv_out), so no extra launch
+// sits between the last stage and the state update.
+constexpr int kReduceBlock = 512;
+
 struct ReduceArgs {
   const double2 *part;
   int stride, nbE, nbC, nbV;
+  int mode;      // 1: end of a CRK4 step (beta sums); 0: invariants only
+  int have_inv;  // the stage-1 partials hold the invariants of the base state
+  double *sums;  // [kNumSums]
+};
+
+__global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs a) {
+  __shared__ double2 sh[kReduceBlock / 64];
+  const int i = blockIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool edge = i == kR12e || i == kR23e || i == kR34e || i == kR44e || i == kEnergyE;
+  const bool vert = i == kEnstrophy || i == kAbsVor;
+  const bool is_inv = i >= kMass;
+  const int nb = edge ? a.nbE : (vert ? a.nbV : a.nbC);
+  double2 acc = make_double2(0.0, 0.0);
+  if ((is_inv && a.have_inv) || (!is_inv && a.mode == 1))
+    for (int j = threadIdx.x; j < nb; j += kReduceBlock) acc = dd_add(acc, a.part[i * a.stride + j]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    double2 o = make_double2(__shfl_xor(acc.x, off, 64), __shfl_xor(acc.y, off, 64));
+    acc = dd_add(acc, o);
+  }
+  if (lane == 0) sh[w] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t = sh[0];
+    for (int q = 1; q < kReduceBlock / 64; ++q) t = dd_add(t, sh[q]);
+    a.sums[i] = t.x + t.y;
+  }
+}
+
+struct StepOut {
+  const double *sums;
   double inv_total_area, inv_sum_area_tri;
   int conserve_energy;
-  int mode;
   int have_inv;
-  double *beta;       // scalar read by k_finalize
   int *hist_idx;      // device step counter of the current batch
   double *beta_hist;  // [cap]
   double *inv_hist;   // [cap][4]
   int hist_cap;
 };
 
-__global__ __launch_bounds__(kBlock) void k_reduce(ReduceArgs a) {
-  __shared__ double2 sh[kNumSums][kBlock / 64];
-  __shared__ double res[kNumSums];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int i = 0; i < kNumSums; ++i) {
-    const bool edge = i == kR12e || i == kR23e || i == kR34e || i == kR44e || i == kEnergyE;
-    const bool vert = i == kEnstrophy || i == kAbsVor;
-    const bool is_inv = i >= kMass;
-    const int nb = edge ? a.nbE : (vert ? a.nbV : a.nbC);
-    double2 acc = make_double2(0.0, 0.0);
-    if ((is_inv && a.have_inv) || (!is_inv && a.mode == 1))
-      for (int j = threadIdx.x; j < nb; j += kBlock) acc = dd_add(acc, a.part[i * a.stride + j]);
-    for (int off = 32; off > 0; off >>= 1) {
-      double2 o = make_double2(__shfl_xor(acc.x, off, 64), __shfl_xor(acc.y, off, 64));
-      acc = dd_add(acc, o);
-    }
-    if (lane == 0) sh[i][w] = acc;
-  }
-  __syncthreads();
-  if (threadIdx.x < kNumSums) {
-    double2 acc = sh[threadIdx.x][0];
-    for (int q = 1; q < kBlock / 64; ++q) acc = dd_add(acc, sh[threadIdx.x][q]);
-    res[threadIdx.x] = acc.x + acc.y;
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const int i = *a.hist_idx;
-    if (a.mode == 1) {
-      double beta = 1.0;
-      if (a.conserve_energy) {
-        // inner_product_tend (operators_mod.F90:187-194) and CRK4 beta
-        // (time_scheme_mod.F90:140-150)
-        const double R44 = (res[kR44e] + res[kR44c]) * a.inv_total_area;
-        const double R12 = (res[kR12e] + res[kR12c]) * a.inv_total_area;
-        const double R23 = (res[kR23e] + res[kR23c]) * a.inv_total_area;
-        const double R34 = (res[kR34e] + res[kR34c]) * a.inv_total_area;
-        beta = (R12 + R23 + R34) / (3.0 * R44);
-      }
-      *a.beta = beta;
-      if (i < a.hist_cap) a.beta_hist[i] = beta;
-    }
-    if (a.have_inv && i >= 1 && i - 1 < a.hist_cap) {
-      double *o = a.inv_hist + 4 * (size_t)(i - 1);
-      o[0] = res[kMass];
-      o[1] = res[kEnergyE] + res[kEnergyC];
-      o[2] = res[kEnstrophy];
-      o[3] = res[kAbsVor] * a.inv_sum_area_tri;
-    }
-    if (a.mode == 1) *a.hist_idx = i + 1;
-  }
+// inner_product_tend (operators_mod.F90:187-194) and the CRK4 beta
+// (time_scheme_mod.F90:140-150).
+__device__ __forceinline__ double crk4_beta(const StepOut &o) {
+  if (!o.conserve_energy) return 1.0;
+  const double *r = o.sums;
+  const double R44 = (r[kR44e] + r[kR44c]) * o.inv_total_area;
+  const double R12 = (r[kR12e] + r[kR12c]) * o.inv_total_area;
+  const double R23 = (r[kR23e] + r[kR23c]) * o.inv_total_area;
+  const double R34 = (r[kR34e] + r[kR34c]) * o.inv_total_area;
+  return (R12 + R23 + R34) / (3.0 * R44);
+}
+
+// calc_total_mass/energy/potential_enstropy/absolute_vorticity
+// (tmcore_swm_mod.F90:215-255) into inv_hist[slot].
+__device__ __forceinline__ void write_invariants(const StepOut &o, int slot) {
+  if (slot < 0 || slot >= o.hist_cap) return;
+  const double *r = o.sums;
+  double *d = o.inv_hist + 4 * (size_t)slot;
+  d[0] = r[kMass];
+  d[1] = r[kEnergyE] + r[kEnergyC];
+  d[2] = r[kEnstrophy];
+  d[3] = r[kAbsVor] * o.inv_sum_area_tri;
+}
+
+// Invariants-only pass (the post-loop refresh): slot hist_idx - 1.
+__global__ void k_inv_out(StepOut o) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) write_invariants(o, *o.hist_idx - 1);
 }
 
 template <typename R>
 struct FinalBufs {
   R *gd, *iap, *q;
   const R *acc_gd, *acc_iap, *acc_q;
-  const double *beta;
   double dt;
   int nT;
+  StepOut out;
 };
 
 // update_state(beta*dt, tend(new), state(old), state(new)) (time_scheme_mod.F90:152);
 // tracers with beta = 1 (tmcore_adv_mod.F90:175 with conserve_energy off).
+// Every block forms beta from the same sums; block 0 records the history.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_finalize(DMesh m, FinalBufs<R> f) {
-  const double bdt = (*f.beta) * f.dt;
+  __shared__ double sbeta;
+  if (threadIdx.x == 0) {
+    const double beta = crk4_beta(f.out);
+    sbeta = beta;
+    if (blockIdx.x == 0) {
+      const int i = *f.out.hist_idx;
+      if (i < f.out.hist_cap) f.out.beta_hist[i] = beta;
+      if (f.out.have_inv) write_invariants(f.out, i - 1);
+      *f.out.hist_idx = i + 1;
+    }
+  }
+  __syncthreads();
+  const double bdt = sbeta * f.dt;
   if ((int)blockIdx.x < m.nbE) {
     const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
     if (e < m.nE) f.iap[e] = f.iap[e] + (R)bdt * f.acc_iap[e];

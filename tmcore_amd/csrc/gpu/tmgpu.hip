@@ -99,7 +99,7 @@ struct tm_ctx {
   void *scratch_e[4] = {}, *scratch_c[2] = {}, *scratch_v[2] = {};
   double2 *part = nullptr;
   int part_stride = 0;
-  double *d_beta = nullptr, *beta_hist = nullptr, *inv_hist = nullptr;
+  double *sums = nullptr, *beta_hist = nullptr, *inv_hist = nullptr;
   int *hist_idx = nullptr;
   int hist_cap = 4096;
   hipGraph_t graph = nullptr;
@@ -239,30 +239,49 @@ ReduceArgs reduce_args(tm_ctx *c, int mode, bool have_inv) {
   a.nbE = c->dm.nbE;
   a.nbC = c->dm.nbC;
   a.nbV = c->dm.nbV;
-  a.inv_total_area = 1.0 / c->totalArea;
-  a.inv_sum_area_tri = 1.0 / c->sumAreaTri;
-  a.conserve_energy = c->o.conserve_energy;
   a.mode = mode;
   a.have_inv = have_inv ? 1 : 0;
-  a.beta = c->d_beta;
-  a.hist_idx = c->hist_idx;
-  a.beta_hist = c->beta_hist;
-  a.inv_hist = c->inv_hist;
-  a.hist_cap = c->hist_cap;
+  a.sums = c->sums;
   return a;
 }
 
-// inner_product divides by totalArea; the reference divides, so do the same.
+StepOut step_out(tm_ctx *c, bool have_inv) {
+  StepOut o{};
+  o.sums = c->sums;
+  o.inv_total_area = 1.0 / c->totalArea;  // inner_product divides by totalArea
+  o.inv_sum_area_tri = 1.0 / c->sumAreaTri;
+  o.conserve_energy = c->o.conserve_energy;
+  o.have_inv = have_inv ? 1 : 0;
+  o.hist_idx = c->hist_idx;
+  o.beta_hist = c->beta_hist;
+  o.inv_hist = c->inv_hist;
+  o.hist_cap = c->hist_cap;
+  return o;
+}
+
+template <typename R>
+FinalBufs<R> final_bufs(tm_ctx *c) {
+  FinalBufs<R> f{};
+  f.gd = (R *)c->gd_old;
+  f.iap = (R *)c->iap_old;
+  f.q = (R *)c->q_old;
+  f.acc_gd = (const R *)c->acc_gd;
+  f.acc_iap = (const R *)c->acc_iap;
+  f.acc_q = (const R *)c->acc_q;
+  f.dt = c->o.dt;
+  f.nT = c->nT;
+  f.out = step_out(c, true);
+  return f;
+}
+
 template <typename R>
 int enqueue_step(tm_ctx *c) {
   for (int k = 1; k <= 4; ++k) {
     int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
     if (rc) return rc;
   }
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 1, true));
-  FinalBufs<R> f{(R *)c->gd_old, (R *)c->iap_old, (R *)c->q_old, (const R *)c->acc_gd, (const R *)c->acc_iap,
-                 (const R *)c->acc_q, c->d_beta, c->o.dt, c->nT};
-  hipLaunchKernelGGL(k_finalize<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, f);
+  hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 1, true));
+  hipLaunchKernelGGL(k_finalize<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, final_bufs<R>(c));
   HIPCHK(c, hipGetLastError());
   return TM_OK;
 }
@@ -272,7 +291,8 @@ template <typename R>
 int enqueue_refresh(tm_ctx *c) {
   int rc = launch_stage<R>(c, stage_bufs<R>(c, 0, true), false);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 0, true));
+  hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 0, true));
+  hipLaunchKernelGGL(k_inv_out, dim3(1), dim3(64), 0, c->stream, step_out(c, true));
   HIPCHK(c, hipGetLastError());
   return TM_OK;
 }
@@ -457,12 +477,16 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
       woe[(size_t)s * nE + ne] = s < n ? m->weightsOnEdge[(size_t)oe * ME2 + s] : 0.0;
     }
     if (need_fit)
-      for (int k = 0; k < 2; ++k)
+      for (int k = 0; k < 2; ++k) {
+        // slots past the vertex's stencil are zeroed whatever the caller's
+        // array holds there (the reference leaves them unset)
+        const int nfit = f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)oe + k] - 1];
         for (int j = 0; j < kFit2; ++j) {
           const int slot = j + 1;
           d2v[((size_t)k * kFit2 + j) * nE + ne] =
-              slot < f->maxFit ? f->deriv2OnVertex[((size_t)oe * 2 + k) * f->maxFit + slot] : 0.0;
+              slot < nfit ? f->deriv2OnVertex[((size_t)oe * 2 + k) * f->maxFit + slot] : 0.0;
         }
+      }
   }
   // ---- vertices
   std::vector<int> eovS((size_t)3 * nV), cov((size_t)3 * nV), fit2V((size_t)kFit2 * nV, 0);
@@ -554,7 +578,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->part_stride = std::max(d.nbE, std::max(d.nbC, d.nbV));
   if (!(c->part = c->alloc<double2>((size_t)kNumSums * c->part_stride))) return c->fail(TM_ENOMEM, "hipMalloc part");
   HIPCHK(c, hipMemsetAsync(c->part, 0, sizeof(double2) * kNumSums * c->part_stride, c->stream));
-  if (!(c->d_beta = c->alloc<double>(1)) || !(c->hist_idx = c->alloc<int>(1)) ||
+  if (!(c->sums = c->alloc<double>(kNumSums)) || !(c->hist_idx = c->alloc<int>(1)) ||
       !(c->beta_hist = c->alloc<double>(c->hist_cap)) || !(c->inv_hist = c->alloc<double>(4 * (size_t)c->hist_cap)))
     return c->fail(TM_ENOMEM, "hipMalloc history");
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -751,12 +775,10 @@ static int profile_one_step(tm_ctx *c, std::vector<hipEvent_t> &ev, double *ms, 
     if ((rc = done())) return rc;
   }
   if ((rc = mark(TM_K_REDUCE))) return rc;
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, c->stream, reduce_args(c, 1, true));
+  hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 1, true));
   if ((rc = done())) return rc;
-  FinalBufs<R> f{(R *)c->gd_old, (R *)c->iap_old, (R *)c->q_old, (const R *)c->acc_gd, (const R *)c->acc_iap,
-                 (const R *)c->acc_q, c->d_beta, c->o.dt, c->nT};
   if ((rc = mark(TM_K_FINALIZE))) return rc;
-  hipLaunchKernelGGL(k_finalize<R>, dim3(m.nbE + m.nbC), dim3(kBlock), 0, c->stream, m, f);
+  hipLaunchKernelGGL(k_finalize<R>, dim3(m.nbE + m.nbC), dim3(kBlock), 0, c->stream, m, final_bufs<R>(c));
   if ((rc = done())) return rc;
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -789,8 +811,7 @@ extern "C" int tm_profile_steps(tm_ctx *c, int nsteps, double ms[TM_K_COUNT], in
 extern "C" int tm_invariants(tm_ctx *c, double out[4]) {
   if (!c || !out) return TM_EINVAL;
   if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
-  hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
-  // counter 1 so that k_reduce's mode-0 pass writes slot 0
+  // counter 1 so that the invariants-only pass writes slot 0
   int one = 1;
   HIPCHK(c, hipMemcpyAsync(c->hist_idx, &one, sizeof(int), hipMemcpyHostToDevice, c->stream));
   int rc = c->rsz == 8 ? enqueue_refresh<double>(c) : enqueue_refresh<float>(c);
