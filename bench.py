@@ -110,6 +110,10 @@ def main():
     log(f"kernels: {json.dumps(kernels)}")
     g64.close()
 
+    from tmcore_amd.swm import hbm_copy_gbps
+    copy_gbs = hbm_copy_gbps(2.0 * 2**30, 20)
+    log(f"HBM copy probe (2 GiB, read+write): {copy_gbs:.1f} GB/s")
+
     forms = {"fp64": {"sim_days_per_hour": round(sim_days_h, 2), "ms_per_step": round(ms_per_step, 4)}}
     if not args.no_fp32:
         g32, wall32, ev32, st32 = run_form(32)
@@ -148,6 +152,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "bytes_per_launch": kbytes, "avg_launch_us": round(avg_ms * 1e3, 2)},
+        "hbm_copy_probe_GBps": round(copy_gbs, 1),
         "step_algorithmic_GBps": round(step_gbs, 1),
         "step_algorithmic_frac": round(step_gbs / HBM_PEAK_GBS, 4),
         "kernels": kernels,

@@ -127,6 +127,13 @@ int tm_step_bytes(tm_ctx *ctx, double bytes[3]);
 int tm_profile_steps(tm_ctx *ctx, int nsteps, double ms[TM_K_COUNT], int launches[TM_K_COUNT]);
 int tm_kernel_bytes(tm_ctx *ctx, double bytes[TM_K_COUNT]);
 
+/* Instrumentation (no reference counterpart): a plain device-to-device copy
+ * (kernel tm_copy_probe_kernel) of `bytes` bytes, timed with HIP events over
+ * `iters` launches; *gbps = 2*bytes / average launch time (read + write).
+ * The practical HBM ceiling the stage kernels are compared with, and the
+ * known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE counters. */
+int tm_hbm_copy_probe(int device, double bytes, int iters, double *gbps);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,7 +43,7 @@ struct StageBufs {
   R *acc_gd, *acc_iap, *acc_q;           // t1 + 2 t2 + 2 t3 (+ t4)/6
   const R *tprev_gd, *tprev_iap;         // tendency of stage k-1
   R *tcur_gd, *tcur_iap;                 // tendency of stage k
-  R *gd_e, *u, *Fdv, *kee, *udc, *qFdv, *E, *tend_gd, *pv_v, *pv_e;
+  R *gd_e, *u, *Fdv, *kee, *udc, *E, *tend_gd, *pv_v, *pv_e;
   const R *ghs;
   int nT;
   int k;          // 1..4: CRK4 stage; 0: diagnostics/invariants only
@@ -110,10 +110,6 @@ __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
     b.Fdv[e] = (u * gde) * dv;
     b.kee[e] = (u * u) * (R)m.aE[e];
     b.udc[e] = u * (R)m.dc[e];
-    for (int t = 0; t < b.nT; ++t) {
-      const R qe = R(0.5) * (b.q[(size_t)t * m.nC + c1] + b.q[(size_t)t * m.nC + c2]);
-      b.qFdv[(size_t)t * m.nE + e] = (u * qe) * dv;
-    }
     if (b.want_inv) en = ((double)iap * (double)iap) * m.aE[e];
   }
   if (b.want_inv) {
@@ -176,15 +172,30 @@ __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b,
       if (b.k == 4) r44 = ((double)acc * (double)acc) * A;
       if (b.k < 4) b.tcur_gd[c] = tend;
     }
-    for (int t = 0; t < b.nT; ++t) {
-      const R *qf = b.qFdv + (size_t)t * m.nE;
-      R qv[kMaxEoC];
+    int nb[kMaxEoC];
+    R uv[kMaxEoC], dvv[kMaxEoC];
+    if (b.nT > 0) {
 #pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s) qv[s] = qf[unsign(idx[s])];
+      for (int s = 0; s < kMaxEoC; ++s) {
+        const int e = unsign(idx[s]);
+        nb[s] = m.cocN[s * m.nC + c];
+        uv[s] = b.u[e];
+        dvv[s] = (R)m.dv[e];
+      }
+    }
+    for (int t = 0; t < b.nT; ++t) {
+      // adv_scheme_sg11_run: f_edge = c2e(q), flux = u*f_edge, div_operator
+      const R *qt = b.q + (size_t)t * m.nC;
+      const R q0 = qt[c];
+      R qn[kMaxEoC];
+#pragma unroll
+      for (int s = 0; s < kMaxEoC; ++s) qn[s] = qt[nb[s]];
       R sq = R(0);
 #pragma unroll
       for (int s = 0; s < kMaxEoC; ++s) {
-        const R f = idx[s] < 0 ? -qv[s] : qv[s];
+        const R qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
+        const R fl = (uv[s] * qe) * dvv[s];
+        const R f = idx[s] < 0 ? -fl : fl;
         sq += s < n ? f : R(0);
       }
       const R tq = (-sq) / (R)A;

@@ -43,6 +43,7 @@ struct DMesh {
   const int8_t *nFit2V;   // [nV] nFit2Vertices
   const int *fit2V;       // [12][nV] fit2Vertices slots 1..12
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
+  const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
   const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))
 };
 

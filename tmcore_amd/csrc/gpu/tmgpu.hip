@@ -91,10 +91,10 @@ struct tm_ctx {
   double kbytes[TM_K_COUNT] = {};
   // state (element type R = double or float, stored untyped)
   void *gd_old = nullptr, *iap_old = nullptr, *q_old = nullptr, *ghs = nullptr;
-  void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *q_s = nullptr;
+  void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *qS[2] = {nullptr, nullptr};
   void *acc_gd = nullptr, *acc_iap = nullptr, *acc_q = nullptr;
   void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
-  void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr, *qFdv = nullptr;
+  void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr;
   void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
   void *scratch_e[4] = {}, *scratch_c[2] = {}, *scratch_v[2] = {};
   double2 *part = nullptr;
@@ -177,19 +177,19 @@ template <typename R>
 StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   StageBufs<R> b{};
   auto P = [](void *p) { return (R *)p; };
-  // gd ping-pongs: the vertex part of k_cell_vertex reads neighbour cells of
-  // the stage input while the cell part writes the next stage state.
+  // gd and the tracers ping-pong: k_cell_vertex reads neighbour cells of the
+  // stage input while it writes the next stage state.
   const int in_slot = (k == 2 || k == 4) ? 0 : 1;
   const int out_slot = (k == 1 || k == 3) ? 0 : 1;
   b.gd = (k <= 1) ? P(c->gd_old) : P(c->gdS[in_slot]);
   b.iap = (k <= 1) ? P(c->iap_old) : P(c->iap_s);
-  b.q = (k <= 1) ? P(c->q_old) : P(c->q_s);
+  b.q = (k <= 1) ? P(c->q_old) : P(c->qS[in_slot]);
   b.gd_base = P(c->gd_old);
   b.iap_base = P(c->iap_old);
   b.q_base = P(c->q_old);
   b.gd_next = P(c->gdS[out_slot]);
   b.iap_next = P(c->iap_s);
-  b.q_next = P(c->q_s);
+  b.q_next = P(c->qS[out_slot]);
   b.acc_gd = P(c->acc_gd);
   b.acc_iap = P(c->acc_iap);
   b.acc_q = P(c->acc_q);
@@ -202,7 +202,6 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.Fdv = P(c->Fdv);
   b.kee = P(c->kee);
   b.udc = P(c->udc);
-  b.qFdv = P(c->qFdv);
   b.E = P(c->E);
   b.tend_gd = P(c->tend_gd);
   b.pv_v = P(c->pv_v);
@@ -433,7 +432,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
 
   // ---- cells
   std::vector<int8_t> nEoC(nC);
-  std::vector<int> eocS((size_t)kMaxEoC * nC, 0), vocS((size_t)kMaxEoC * nC, 0);
+  std::vector<int> eocS((size_t)kMaxEoC * nC, 0), vocS((size_t)kMaxEoC * nC, 0), cocN((size_t)kMaxEoC * nC, 0);
   std::vector<double> areaCell(nC), kiteC((size_t)kMaxEoC * nC, 0.0);
   double tot = 0.0;
   for (int i = 0; i < nC; ++i) tot += m->areaCell[i] * a2;  // mesh_mod.F90:224 order
@@ -443,11 +442,15 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     const int n = m->nEdgesOnCell[oi];
     nEoC[ni] = (int8_t)n;
     areaCell[ni] = m->areaCell[oi] * a2;
+    for (int s = n; s < kMaxEoC; ++s) cocN[(size_t)s * nC + ni] = ni;
     for (int s = 0; s < n; ++s) {
       const int oe = m->edgesOnCell[(size_t)oi * ME + s] - 1;
       const int ne = c->iE[oe];
       const bool plus = m->cellsOnEdge[2 * (size_t)oe] - 1 == oi;  // nSignEdge = +1
       eocS[(size_t)s * nC + ni] = plus ? ne : ~ne;
+      // the cell across the edge (cellsOnCell(s) lies across edgesOnCell(s))
+      const int other = plus ? m->cellsOnEdge[2 * (size_t)oe + 1] - 1 : m->cellsOnEdge[2 * (size_t)oe] - 1;
+      cocN[(size_t)s * nC + ni] = c->iC[other];
       const int ov = m->verticesOnCell[(size_t)oi * ME + s] - 1;
       vocS[(size_t)s * nC + ni] = c->iV[ov];
       double k = 0.0;
@@ -555,6 +558,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   UP(nFit2V, nFit2V, int8_t);
   UP(fit2V, fit2V, int);
   UP(vocS, vocS, int);
+  UP(cocN, cocN, int);
   UP(kiteC, kiteC, double);
 #undef UP
 
@@ -571,10 +575,9 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   void **rbufs_v[] = {&c->pv_v, &c->scratch_v[0], &c->scratch_v[1]};
   for (void **p : rbufs_v)
     if (!(*p = c->alloc_r(nV))) return c->fail(TM_ENOMEM, "hipMalloc vertex state");
-  void **rbufs_q[] = {&c->q_old, &c->q_s, &c->acc_q};
+  void **rbufs_q[] = {&c->q_old, &c->qS[0], &c->qS[1], &c->acc_q};
   for (void **p : rbufs_q)
     if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
-  if (!(c->qFdv = c->alloc_r(std::max<size_t>((size_t)c->nT * nE, 1)))) return c->fail(TM_ENOMEM, "hipMalloc qFdv");
   c->part_stride = std::max(d.nbE, std::max(d.nbC, d.nbV));
   if (!(c->part = c->alloc<double2>((size_t)kNumSums * c->part_stride))) return c->fail(TM_ENOMEM, "hipMalloc part");
   HIPCHK(c, hipMemsetAsync(c->part, 0, sizeof(double2) * kNumSums * c->part_stride, c->stream));
@@ -603,9 +606,10 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->step_bytes[2] = 4.0 * c->step_bytes[0] + c->step_bytes[1];
   // Compulsory bytes of one launch of each kernel (stage 2/3 pattern).
   const double T = c->nT;
-  c->kbytes[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs + T * rs) + nC * rs * (1.0 + T);
-  c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * (2.0 + T) +
-                                T * nC * 4.0 * rs + nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
+  c->kbytes[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs) + nC * rs;
+  c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * 2.0 +
+                                (T > 0 ? 4.0 * sumEoC + nE * (rs + 8.0) : 0.0) + T * nC * 5.0 * rs +
+                                nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
   c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + 8.0 + 2.0 * rs) + 12.0 * sumEoE + 8.0 * sumFitE +
                             nV * (rs + 1.0) + 4.0 * sumFitV;
   c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + 8.0 + 9.0 * rs) + 12.0 * sumEoE + nC * 2.0 * rs;
@@ -870,3 +874,44 @@ extern "C" int tm_stage(tm_ctx *c, tm_stage_out *out) {
 }
 
 #include "ops_kernels.h"
+
+// ------------------------------------------------------------- copy probe
+__global__ __launch_bounds__(256) void tm_copy_probe_kernel(const double2 *__restrict__ a, double2 *__restrict__ b,
+                                                            size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int tm_hbm_copy_probe(int device, double bytes, int iters, double *gbps) {
+  if (!gbps || !(bytes >= 1 << 20) || bytes > 64.0 * (1ull << 30) || iters < 1) return TM_EINVAL;
+  const size_t n = (size_t)(bytes / sizeof(double2));
+  void *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = TM_OK;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = TM_EDEVICE;
+  if (!rc && (hipMalloc(&a, n * sizeof(double2)) != hipSuccess || hipMalloc(&b, n * sizeof(double2)) != hipSuccess))
+    rc = TM_ENOMEM;
+  if (!rc) {
+    (void)hipMemsetAsync(a, 0, n * sizeof(double2), s);
+    const int grid = 8 * 1024;
+    hipLaunchKernelGGL(tm_copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+    (void)hipEventRecord(e0, s);
+    for (int i = 0; i < iters; ++i)
+      hipLaunchKernelGGL(tm_copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
+    (void)hipEventRecord(e1, s);
+    float ms = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+      rc = TM_EDEVICE;
+    else
+      *gbps = 2.0 * (double)n * sizeof(double2) / ((double)ms * 1e-3 / iters) / 1e9;
+  }
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  return rc;
+}

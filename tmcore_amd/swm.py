@@ -63,6 +63,8 @@ def gpu_lib() -> C.CDLL:
         L.tm_kernel_bytes.argtypes = [_vp, _dp]
         L.tm_profile_steps.restype = C.c_int
         L.tm_kernel_bytes.restype = C.c_int
+        L.tm_hbm_copy_probe.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
+        L.tm_hbm_copy_probe.restype = C.c_int
         for n in ("div", "curl", "c2e", "c2v", "v2c", "v2e", "ke", "tangent_wind"):
             getattr(L, "tm_op_" + n).argtypes = [_vp, _dp, _dp]
         L.tm_op_tangent_vor_flux.argtypes = [_vp, _dp, _dp, _dp, _dp]
@@ -75,6 +77,15 @@ def gpu_lib() -> C.CDLL:
 
 def _p(a):
     return None if a is None else a.ctypes.data_as(_dp)
+
+
+def hbm_copy_gbps(nbytes: float = 2.0 * 2**30, iters: int = 20, device: int = 0) -> float:
+    """Device-to-device copy bandwidth (read + write GB/s), tm_hbm_copy_probe."""
+    g = C.c_double()
+    rc = gpu_lib().tm_hbm_copy_probe(device, nbytes, iters, C.byref(g))
+    if rc != 0:
+        raise _ffi.TmError(rc, "tm_hbm_copy_probe")
+    return g.value
 
 
 def _f64(a):
