@@ -64,7 +64,7 @@ def build_fortran():
     os.makedirs(bdir, exist_ok=True)
     _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs], "-o",
           os.path.join(LIB, "test_tmcore_sw"), "-L" + LIB, "-ltmgpu", "-ltmhost",
-          "-Wl,-rpath," + LIB, "-Wl,-rpath,/opt/rocm/lib"], cwd=bdir)
+          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"], cwd=bdir)
 
 
 def build_all():

@@ -1,0 +1,216 @@
+! iso_c_binding declarations of the C-ABI in include/tmcore_host.h and
+! include/tmcore_gpu.h.  Host arrays keep the reference's Fortran layout
+! (1-based int32 connectivity, (slot, n) column-major), so they are passed as
+! they are allocated here.
+module tm_c_binding_mod
+
+  use iso_c_binding
+
+  implicit none
+
+  integer(c_int), parameter :: TM_OK = 0
+
+  type, bind(c) :: tm_mesh_desc
+    integer(c_int32_t) :: nCells, nEdges, nVertices, maxEdges, maxEdges2, vertexDegree
+    type(c_ptr) :: latCell, lonCell, xCell, yCell, zCell
+    type(c_ptr) :: latEdge, lonEdge, xEdge, yEdge, zEdge
+    type(c_ptr) :: latVertex, lonVertex, xVertex, yVertex, zVertex
+    type(c_ptr) :: nEdgesOnCell, nEdgesOnEdge
+    type(c_ptr) :: cellsOnCell, edgesOnCell, verticesOnCell
+    type(c_ptr) :: cellsOnEdge, verticesOnEdge
+    type(c_ptr) :: edgesOnEdge, weightsOnEdge
+    type(c_ptr) :: cellsOnVertex, edgesOnVertex, kiteAreasOnVertex
+    type(c_ptr) :: dvEdge, dv1Edge, dv2Edge, dcEdge, angleEdge
+    type(c_ptr) :: areaCell, areaTriangle
+  end type tm_mesh_desc
+
+  type, bind(c) :: tm_fit_desc
+    integer(c_int32_t) :: maxFit
+    type(c_ptr) :: nFit2Cells, fit2Cells, deriv2OnCell
+    type(c_ptr) :: nFit4Cells, fit4Cells, deriv4OnCell
+    type(c_ptr) :: nFit2Vertices, fit2Vertices, deriv2OnVertex
+    type(c_ptr) :: nFit4Vertices, fit4Vertices, deriv4OnVertex
+  end type tm_fit_desc
+
+  type, bind(c) :: tm_opts
+    integer(c_int) :: time_scheme, pv_scheme, interp_pv_order, adv_order, adv_mono, conserve_energy
+    real(c_double) :: apvm_weight, dt, radius
+    integer(c_int) :: ntracers, precision, device, renumber
+  end type tm_opts
+
+  interface
+
+    integer(c_int) function tm_mesh_icos_dims(level, nC, nE, nV) bind(c, name='tm_mesh_icos_dims')
+      import :: c_int, c_int32_t
+      integer(c_int), value :: level
+      integer(c_int32_t) :: nC, nE, nV
+    end function tm_mesh_icos_dims
+
+    integer(c_int) function tm_mesh_icos_generate(level, m) bind(c, name='tm_mesh_icos_generate')
+      import :: c_int, tm_mesh_desc
+      integer(c_int), value :: level
+      type(tm_mesh_desc) :: m
+    end function tm_mesh_icos_generate
+
+    integer(c_int) function tm_poly_fit(m, radius, f) bind(c, name='tm_poly_fit')
+      import :: c_int, c_double, tm_mesh_desc, tm_fit_desc
+      type(tm_mesh_desc) :: m
+      real(c_double), value :: radius
+      type(tm_fit_desc) :: f
+    end function tm_poly_fit
+
+    integer(c_int) function tm_case_seeded(m, radius, seed, ntracers, tracer_seeds, gd, u, ghs, tracers) &
+        bind(c, name='tm_case_seeded')
+      import :: c_int, c_int64_t, c_double, c_ptr, tm_mesh_desc
+      type(tm_mesh_desc) :: m
+      real(c_double), value :: radius
+      integer(c_int64_t), value :: seed
+      integer(c_int), value :: ntracers
+      integer(c_int64_t) :: tracer_seeds(*)
+      real(c_double) :: gd(*), u(*), ghs(*)
+      type(c_ptr), value :: tracers
+    end function tm_case_seeded
+
+    integer(c_int) function tm_case_steady_geostrophic(m, radius, gd, u, ghs) bind(c, name='tm_case_steady_geostrophic')
+      import :: c_int, c_double, tm_mesh_desc
+      type(tm_mesh_desc) :: m
+      real(c_double), value :: radius
+      real(c_double) :: gd(*), u(*), ghs(*)
+    end function tm_case_steady_geostrophic
+
+    subroutine tm_opts_default(o) bind(c, name='tm_opts_default')
+      import :: tm_opts
+      type(tm_opts) :: o
+    end subroutine tm_opts_default
+
+    integer(c_int) function tm_ctx_create(ctx, m, f, o) bind(c, name='tm_ctx_create')
+      import :: c_int, c_ptr, tm_mesh_desc, tm_fit_desc, tm_opts
+      type(c_ptr) :: ctx
+      type(tm_mesh_desc) :: m
+      type(tm_fit_desc) :: f
+      type(tm_opts) :: o
+    end function tm_ctx_create
+
+    integer(c_int) function tm_ctx_destroy(ctx) bind(c, name='tm_ctx_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function tm_ctx_destroy
+
+    type(c_ptr) function tm_last_error(ctx) bind(c, name='tm_last_error')
+      import :: c_ptr
+      type(c_ptr), value :: ctx
+    end function tm_last_error
+
+    integer(c_int) function tm_state_upload(ctx, gd, u, iap_u, ghs, tracers) bind(c, name='tm_state_upload')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gd, u, iap_u, ghs, tracers
+    end function tm_state_upload
+
+    integer(c_int) function tm_state_download(ctx, gd, u, iap_u, tracers) bind(c, name='tm_state_download')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, gd, u, iap_u, tracers
+    end function tm_state_download
+
+    integer(c_int) function tm_step(ctx, nsteps, invariants, beta) bind(c, name='tm_step')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      real(c_double) :: invariants(*), beta(*)
+    end function tm_step
+
+    integer(c_int) function tm_invariants(ctx, out) bind(c, name='tm_invariants')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: out(4)
+    end function tm_invariants
+
+    integer(c_int) function tm_last_elapsed_ms(ctx, ms) bind(c, name='tm_last_elapsed_ms')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: ms
+    end function tm_last_elapsed_ms
+
+    integer(c_int) function tm_op_div(ctx, f_in, f_out) bind(c, name='tm_op_div')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_div
+
+    integer(c_int) function tm_op_curl(ctx, f_in, f_out) bind(c, name='tm_op_curl')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_curl
+
+    integer(c_int) function tm_op_c2e(ctx, f_in, f_out) bind(c, name='tm_op_c2e')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_c2e
+
+    integer(c_int) function tm_op_c2v(ctx, f_in, f_out) bind(c, name='tm_op_c2v')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_c2v
+
+    integer(c_int) function tm_op_v2c(ctx, f_in, f_out) bind(c, name='tm_op_v2c')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_v2c
+
+    integer(c_int) function tm_op_v2e(ctx, f_in, f_out) bind(c, name='tm_op_v2e')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_v2e
+
+    integer(c_int) function tm_op_ke(ctx, f_in, f_out) bind(c, name='tm_op_ke')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_ke
+
+    integer(c_int) function tm_op_tangent_wind(ctx, f_in, f_out) bind(c, name='tm_op_tangent_wind')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_in(*), f_out(*)
+    end function tm_op_tangent_wind
+
+    integer(c_int) function tm_op_tangent_vor_flux(ctx, u, gd, pv, flux) bind(c, name='tm_op_tangent_vor_flux')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), gd(*), pv(*), flux(*)
+    end function tm_op_tangent_vor_flux
+
+    integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+      import :: c_size_t, c_ptr
+      type(c_ptr), value :: s
+    end function c_strlen
+
+  end interface
+
+contains
+
+  ! Error text of the last failed call on ctx, as a Fortran string.
+  function tm_error_text(ctx) result(res)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: res
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: n, i
+    p = tm_last_error(ctx)
+    if (.not. c_associated(p)) then
+      res = ''
+      return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: res)
+    do i = 1, n
+      res(i:i) = chars(i)
+    end do
+  end function tm_error_text
+
+end module tm_c_binding_mod

@@ -1,0 +1,271 @@
+! Fortran host of the MI355X shallow-water path: the counterpart of the
+! reference's tmcore_swm_mod (reference src/core/tmcore_swm_mod.F90:29-90)
+! with the time loop on the device through libtmgpu.
+!
+!   tm_swm_init   params_parse_namelist + mesh + poly_fit + initial state +
+!                 device context (tmcore_swm_init, :29-47, and
+!                 test_cases_set_initial_condition)
+!   tm_swm_run    the time loop with the per-step invariant line
+!                 (tmcore_swm_run, :60-90)
+!   tm_swm_final  teardown (tmcore_swm_final, :49-58)
+!
+! The operators of operators_mod keep their names and dummy arguments
+! (operators_mod.F90:40-260) and run on the device.  A non-zero return code
+! from the library prints its message and stops, as the reference's
+! log_error does.
+module tm_swm_gpu_mod
+
+  use iso_c_binding
+  use tm_c_binding_mod
+  use tm_params_mod
+
+  implicit none
+
+  private
+
+  public tm_swm_init, tm_swm_run, tm_swm_final
+  public div_operator, curl_operator, scalar_c2e_interp_operator, scalar_c2v_interp_operator
+  public scalar_v2c_interp_operator, scalar_v2e_interp_operator, calc_kinetic_energy
+  public calc_tangent_wind, calc_tangent_vor_flux
+  public nCells, nEdges, nVertices, gd, u, iap_u, ghs, tracers, run_inv, run_beta, run_ms
+
+  ! Mesh (mesh_mod.F90:10-71 names), unit-sphere values as a grid file holds them.
+  integer(c_int32_t) :: nCells, nEdges, nVertices
+  real(c_double), allocatable, target :: latCell(:), lonCell(:), xCell(:), yCell(:), zCell(:)
+  real(c_double), allocatable, target :: latEdge(:), lonEdge(:), xEdge(:), yEdge(:), zEdge(:)
+  real(c_double), allocatable, target :: latVertex(:), lonVertex(:), xVertex(:), yVertex(:), zVertex(:)
+  integer(c_int32_t), allocatable, target :: nEdgesOnCell(:), nEdgesOnEdge(:)
+  integer(c_int32_t), allocatable, target :: cellsOnCell(:,:), edgesOnCell(:,:), verticesOnCell(:,:)
+  integer(c_int32_t), allocatable, target :: cellsOnEdge(:,:), verticesOnEdge(:,:), edgesOnEdge(:,:)
+  real(c_double), allocatable, target :: weightsOnEdge(:,:)
+  integer(c_int32_t), allocatable, target :: cellsOnVertex(:,:), edgesOnVertex(:,:)
+  real(c_double), allocatable, target :: kiteAreasOnVertex(:,:)
+  real(c_double), allocatable, target :: dvEdge(:), dv1Edge(:), dv2Edge(:), dcEdge(:), angleEdge(:)
+  real(c_double), allocatable, target :: areaCell(:), areaTriangle(:)
+  ! poly_fit_mod.F90:35-57 (second-order vertex fit, slots 0:12)
+  integer, parameter :: max_fit = 13
+  integer(c_int32_t), allocatable, target :: nFit2Vertices(:), fit2Vertices(:,:)
+  real(c_double), allocatable, target :: deriv2OnVertex(:,:,:)
+  ! Prognostics of state(old) and static%cell%ghs
+  real(c_double), allocatable, target :: gd(:), u(:), iap_u(:), ghs(:), tracers(:,:)
+  real(c_double), allocatable :: run_inv(:,:), run_beta(:)
+  real(c_double) :: run_ms = 0
+
+  type(tm_mesh_desc), target :: mesh
+  type(tm_fit_desc), target :: fit
+  type(tm_opts) :: opts
+  type(c_ptr) :: ctx = c_null_ptr
+
+contains
+
+  subroutine check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc /= TM_OK) then
+      write(*, '(A,I0,A)') '[Error]: ' // where // ' returned ', rc, ': ' // tm_error_text(ctx)
+      stop 1
+    end if
+  end subroutine check
+
+  subroutine tm_swm_init(namelist_file_path)
+
+    character(*), intent(in) :: namelist_file_path
+    integer(c_int64_t), allocatable :: tseeds(:)
+    integer :: k
+
+    call params_parse_namelist(namelist_file_path)
+    if (trim(mesh_file_path) /= 'N/A') then
+      write(*, '(A)') '[Notice]: mesh_file_path is ignored; the icosahedral mesh of gpu_control%mesh_level is generated.'
+    end if
+    call check(tm_mesh_icos_dims(mesh_level, nCells, nEdges, nVertices), 'tm_mesh_icos_dims')
+    call allocate_mesh()
+    call check(tm_mesh_icos_generate(mesh_level, mesh), 'tm_mesh_icos_generate')
+
+    allocate(nFit2Vertices(nVertices), fit2Vertices(0:max_fit-1, nVertices), deriv2OnVertex(0:max_fit-1, 2, nEdges))
+    fit%maxFit = max_fit
+    fit%nFit2Cells = c_null_ptr;    fit%fit2Cells = c_null_ptr;    fit%deriv2OnCell = c_null_ptr
+    fit%nFit4Cells = c_null_ptr;    fit%fit4Cells = c_null_ptr;    fit%deriv4OnCell = c_null_ptr
+    fit%nFit4Vertices = c_null_ptr; fit%fit4Vertices = c_null_ptr; fit%deriv4OnVertex = c_null_ptr
+    fit%nFit2Vertices = c_loc(nFit2Vertices)
+    fit%fit2Vertices = c_loc(fit2Vertices)
+    fit%deriv2OnVertex = c_loc(deriv2OnVertex)
+    call check(tm_poly_fit(mesh, radius, fit), 'tm_poly_fit')
+
+    allocate(gd(nCells), u(nEdges), iap_u(nEdges), ghs(nCells), tracers(nCells, max(ntracers, 1)))
+    select case (trim(case_name))
+    case ('seeded')
+      allocate(tseeds(max(ntracers, 1)))
+      do k = 1, max(ntracers, 1)
+        tseeds(k) = tracer_seed0 + k - 1
+      end do
+      call check(tm_case_seeded(mesh, radius, seed, ntracers, tseeds, gd, u, ghs, c_loc(tracers)), 'tm_case_seeded')
+    case ('steady_geostrophic_flow')
+      call check(tm_case_steady_geostrophic(mesh, radius, gd, u, ghs), 'tm_case_steady_geostrophic')
+      tracers = 0
+    case default
+      write(*, '(A)') '[Error]: unknown case_name ' // trim(case_name) // ' (seeded, steady_geostrophic_flow)'
+      stop 1
+    end select
+
+    call tm_opts_default(opts)
+    opts%time_scheme = time_scheme
+    opts%pv_scheme = pv_scheme
+    opts%interp_pv_order = interp_pv_order
+    opts%adv_order = adv_order
+    opts%adv_mono = merge(1, 0, adv_mono)
+    opts%conserve_energy = merge(1, 0, conserve_energy)
+    opts%apvm_weight = apvm_weight
+    opts%dt = dt
+    opts%radius = radius
+    opts%ntracers = ntracers
+    opts%precision = precision
+    opts%device = device
+    opts%renumber = renumber
+    if (flux_4th_order_correct) then
+      write(*, '(A)') '[Error]: flux_4th_order_correct = .true. is not reproducible in the reference (deriv3OnCell)'
+      stop 1
+    end if
+    call check(tm_ctx_create(ctx, mesh, fit, opts), 'tm_ctx_create')
+    call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
+
+    write(*, '(A,I0,A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh level ', mesh_level, ': ', nCells, ' cells, ', &
+      nEdges, ' edges, ', nVertices, ' vertices; dt = ', dt, ' s; precision ', precision
+
+  end subroutine tm_swm_init
+
+  subroutine allocate_mesh()
+
+    allocate(latCell(nCells), lonCell(nCells), xCell(nCells), yCell(nCells), zCell(nCells))
+    allocate(latEdge(nEdges), lonEdge(nEdges), xEdge(nEdges), yEdge(nEdges), zEdge(nEdges))
+    allocate(latVertex(nVertices), lonVertex(nVertices), xVertex(nVertices), yVertex(nVertices), zVertex(nVertices))
+    allocate(nEdgesOnCell(nCells), nEdgesOnEdge(nEdges))
+    allocate(cellsOnCell(6, nCells), edgesOnCell(6, nCells), verticesOnCell(6, nCells))
+    allocate(cellsOnEdge(2, nEdges), verticesOnEdge(2, nEdges), edgesOnEdge(12, nEdges), weightsOnEdge(12, nEdges))
+    allocate(cellsOnVertex(3, nVertices), edgesOnVertex(3, nVertices), kiteAreasOnVertex(3, nVertices))
+    allocate(dvEdge(nEdges), dv1Edge(nEdges), dv2Edge(nEdges), dcEdge(nEdges), angleEdge(nEdges))
+    allocate(areaCell(nCells), areaTriangle(nVertices))
+    mesh%nCells = nCells; mesh%nEdges = nEdges; mesh%nVertices = nVertices
+    mesh%maxEdges = 6; mesh%maxEdges2 = 12; mesh%vertexDegree = 3
+    mesh%latCell = c_loc(latCell); mesh%lonCell = c_loc(lonCell)
+    mesh%xCell = c_loc(xCell); mesh%yCell = c_loc(yCell); mesh%zCell = c_loc(zCell)
+    mesh%latEdge = c_loc(latEdge); mesh%lonEdge = c_loc(lonEdge)
+    mesh%xEdge = c_loc(xEdge); mesh%yEdge = c_loc(yEdge); mesh%zEdge = c_loc(zEdge)
+    mesh%latVertex = c_loc(latVertex); mesh%lonVertex = c_loc(lonVertex)
+    mesh%xVertex = c_loc(xVertex); mesh%yVertex = c_loc(yVertex); mesh%zVertex = c_loc(zVertex)
+    mesh%nEdgesOnCell = c_loc(nEdgesOnCell); mesh%nEdgesOnEdge = c_loc(nEdgesOnEdge)
+    mesh%cellsOnCell = c_loc(cellsOnCell); mesh%edgesOnCell = c_loc(edgesOnCell)
+    mesh%verticesOnCell = c_loc(verticesOnCell)
+    mesh%cellsOnEdge = c_loc(cellsOnEdge); mesh%verticesOnEdge = c_loc(verticesOnEdge)
+    mesh%edgesOnEdge = c_loc(edgesOnEdge); mesh%weightsOnEdge = c_loc(weightsOnEdge)
+    mesh%cellsOnVertex = c_loc(cellsOnVertex); mesh%edgesOnVertex = c_loc(edgesOnVertex)
+    mesh%kiteAreasOnVertex = c_loc(kiteAreasOnVertex)
+    mesh%dvEdge = c_loc(dvEdge); mesh%dv1Edge = c_loc(dv1Edge); mesh%dv2Edge = c_loc(dv2Edge)
+    mesh%dcEdge = c_loc(dcEdge); mesh%angleEdge = c_loc(angleEdge)
+    mesh%areaCell = c_loc(areaCell); mesh%areaTriangle = c_loc(areaTriangle)
+
+  end subroutine allocate_mesh
+
+  ! The time loop of tmcore_swm_run (tmcore_swm_mod.F90:75-88): one device
+  ! call per print interval; every step's invariants and beta come back and
+  ! are logged as the reference's log_print_diag does.
+  subroutine tm_swm_run()
+
+    integer :: nsteps, done, n, i
+    real(c_double) :: inv0(4), ms
+    real(c_double), allocatable :: inv(:,:), beta(:)
+
+    nsteps = run_steps()
+    allocate(run_inv(4, nsteps), run_beta(nsteps))
+    call check(tm_invariants(ctx, inv0), 'tm_invariants')
+    write(*, '(A,I8,4(A,ES24.16))') 'step=', 0, ' total_mass=', inv0(1), ' total_energy=', inv0(2), &
+      ' total_potential_enstropy=', inv0(3), ' total_absolute_vorticity=', inv0(4)
+    done = 0
+    run_ms = 0
+    do while (done < nsteps)
+      n = min(max(print_every, 1), nsteps - done)
+      allocate(inv(4, n), beta(n))
+      call check(tm_step(ctx, n, inv, beta), 'tm_step')
+      call check(tm_last_elapsed_ms(ctx, ms), 'tm_last_elapsed_ms')
+      run_ms = run_ms + ms
+      run_inv(:, done+1:done+n) = inv
+      run_beta(done+1:done+n) = beta
+      do i = 1, n
+        if (mod(done + i, max(print_every, 1)) == 0 .or. done + i == nsteps) then
+          write(*, '(A,I8,5(A,ES24.16))') 'step=', done + i, ' total_mass=', inv(1,i), ' total_energy=', inv(2,i), &
+            ' total_potential_enstropy=', inv(3,i), ' total_absolute_vorticity=', inv(4,i), ' beta=', beta(i)
+        end if
+      end do
+      deallocate(inv, beta)
+      done = done + n
+    end do
+    call check(tm_state_download(ctx, c_loc(gd), c_loc(u), c_loc(iap_u), c_loc(tracers)), 'tm_state_download')
+    write(*, '(A,I0,A,F12.3,A,F14.3)') '[Notice]: ', nsteps, ' steps, device time ', run_ms, ' ms, sim-days/hour ', &
+      (nsteps * dt / 86400.0_r8) / (max(run_ms, 1.0e-9_r8) / 3.6e6_r8)
+
+  end subroutine tm_swm_run
+
+  subroutine tm_swm_final()
+
+    integer(c_int) :: rc
+    if (c_associated(ctx)) rc = tm_ctx_destroy(ctx)
+    ctx = c_null_ptr
+
+  end subroutine tm_swm_final
+
+  ! ---------------------------------------------------------------- operators
+  ! operators_mod.F90 names and argument order, computed on the device.
+  subroutine div_operator(f_edge, div)
+    real(c_double), intent(in) :: f_edge(:)
+    real(c_double), intent(out) :: div(:)
+    call check(tm_op_div(ctx, f_edge, div), 'tm_op_div')
+  end subroutine div_operator
+
+  subroutine curl_operator(f_edge, curl)
+    real(c_double), intent(in) :: f_edge(:)
+    real(c_double), intent(out) :: curl(:)
+    call check(tm_op_curl(ctx, f_edge, curl), 'tm_op_curl')
+  end subroutine curl_operator
+
+  subroutine scalar_c2e_interp_operator(f_cell, f_edge)
+    real(c_double), intent(in) :: f_cell(:)
+    real(c_double), intent(out) :: f_edge(:)
+    call check(tm_op_c2e(ctx, f_cell, f_edge), 'tm_op_c2e')
+  end subroutine scalar_c2e_interp_operator
+
+  subroutine scalar_c2v_interp_operator(f_cell, f_vertex)
+    real(c_double), intent(in) :: f_cell(:)
+    real(c_double), intent(out) :: f_vertex(:)
+    call check(tm_op_c2v(ctx, f_cell, f_vertex), 'tm_op_c2v')
+  end subroutine scalar_c2v_interp_operator
+
+  subroutine scalar_v2c_interp_operator(f_vertex, f_cell)
+    real(c_double), intent(in) :: f_vertex(:)
+    real(c_double), intent(out) :: f_cell(:)
+    call check(tm_op_v2c(ctx, f_vertex, f_cell), 'tm_op_v2c')
+  end subroutine scalar_v2c_interp_operator
+
+  subroutine scalar_v2e_interp_operator(f_vertex, f_edge)
+    real(c_double), intent(in) :: f_vertex(:)
+    real(c_double), intent(out) :: f_edge(:)
+    call check(tm_op_v2e(ctx, f_vertex, f_edge), 'tm_op_v2e')
+  end subroutine scalar_v2e_interp_operator
+
+  subroutine calc_kinetic_energy(u_edge, ke_cell)
+    real(c_double), intent(in) :: u_edge(:)
+    real(c_double), intent(out) :: ke_cell(:)
+    call check(tm_op_ke(ctx, u_edge, ke_cell), 'tm_op_ke')
+  end subroutine calc_kinetic_energy
+
+  subroutine calc_tangent_wind(u_edge, v_edge)
+    real(c_double), intent(in) :: u_edge(:)
+    real(c_double), intent(out) :: v_edge(:)
+    call check(tm_op_tangent_wind(ctx, u_edge, v_edge), 'tm_op_tangent_wind')
+  end subroutine calc_tangent_wind
+
+  subroutine calc_tangent_vor_flux(u_edge, gd_edge, pv_edge, tangent_vor_flux)
+    real(c_double), intent(in) :: u_edge(:), gd_edge(:), pv_edge(:)
+    real(c_double), intent(out) :: tangent_vor_flux(:)
+    call check(tm_op_tangent_vor_flux(ctx, u_edge, gd_edge, pv_edge, tangent_vor_flux), 'tm_op_tangent_vor_flux')
+  end subroutine calc_tangent_vor_flux
+
+end module tm_swm_gpu_mod
