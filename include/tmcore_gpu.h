@@ -40,6 +40,9 @@ typedef struct tm_opts {
                            with FP64 geometry, weights and invariant sums        */
   int device;           /* HIP device ordinal                                    */
   int renumber;         /* 1: Hilbert-curve renumbering of cells/edges/vertices  */
+  int geometry_scaled;  /* 1: x/y/z, dvEdge, dcEdge, areas and kites are already
+                           multiplied by radius / radius^2, as mesh_init leaves
+                           them (mesh_mod.F90:205-224); 0: unit sphere        */
 } tm_opts;
 
 /* Fill *o with the reference namelist defaults (run/namelist.tmcore_swm:12-22):

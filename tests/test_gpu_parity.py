@@ -151,6 +151,29 @@ def test_renumbering_does_not_change_results():
     g2.close()
 
 
+def test_radius_scaled_geometry_input():
+    """geometry_scaled = 1 takes the arrays as mesh_init leaves them (radius-scaled)."""
+    import copy
+    from tmcore_amd.mesh import RADIUS
+    m = mesh(4)
+    ms = copy.deepcopy(m)
+    for k in ("xCell", "yCell", "zCell", "xEdge", "yEdge", "zEdge", "xVertex", "yVertex", "zVertex",
+              "dvEdge", "dcEdge", "dv1Edge", "dv2Edge"):
+        setattr(ms, k, getattr(m, k) * RADIUS)
+    for k in ("areaCell", "areaTriangle", "kiteAreasOnVertex"):
+        setattr(ms, k, getattr(m, k) * RADIUS**2)
+    gd, u, ghs, _ = seeded_state(m, 3)
+    g1 = SwmGpu(m, m.fit(), dt=600.0)
+    g2 = SwmGpu(ms, m.fit(), dt=600.0, geometry_scaled=1)
+    for g in (g1, g2):
+        g.upload(gd, ghs, u=u)
+    r1, r2 = g1.stage(), g2.stage()
+    for k in ("tend_gd", "tend_iap", "pv_edge", "ke"):
+        assert rel_l2(r2[k], r1[k]) <= 1e-14, k
+    g1.close()
+    g2.close()
+
+
 def test_steady_geostrophic_flow_on_device():
     m = mesh(4)
     gd, u, ghs = steady_geostrophic_state(m)

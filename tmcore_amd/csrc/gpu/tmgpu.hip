@@ -54,7 +54,8 @@ uint64_t hilbert3(uint32_t x, uint32_t y, uint32_t z, int bits) {
 }
 
 // new -> old permutation sorting points on the unit sphere by Hilbert key.
-std::vector<int> hilbert_order(int n, const double *x, const double *y, const double *z, bool enabled) {
+std::vector<int> hilbert_order(int n, const double *x, const double *y, const double *z, bool enabled,
+                               double pos_scale) {
   std::vector<int> p(n);
   std::iota(p.begin(), p.end(), 0);
   if (!enabled) return p;
@@ -62,7 +63,9 @@ std::vector<int> hilbert_order(int n, const double *x, const double *y, const do
   const double s = (double)((1u << bits) - 1) / 2.0;
   std::vector<uint64_t> key(n);
   for (int i = 0; i < n; ++i) {
-    auto q = [&](double v) { return (uint32_t)std::lround((std::min(1.0, std::max(-1.0, v)) + 1.0) * s); };
+    auto q = [&](double v) {
+      return (uint32_t)std::lround((std::min(1.0, std::max(-1.0, v * pos_scale)) + 1.0) * s);
+    };
     key[i] = hilbert3(q(x[i]), q(y[i]), q(z[i]), bits);
   }
   std::stable_sort(p.begin(), p.end(), [&](int a, int b) { return key[a] < key[b]; });
@@ -417,18 +420,22 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
           return c->fail(TM_ERANGE, "fit2Vertices index out of range");
     }
 
-  // Hilbert renumbering: new -> old and old -> new.
+  // Hilbert renumbering: new -> old and old -> new.  Keys are taken on the
+  // unit sphere whether or not the caller's coordinates are radius-scaled.
   const bool ren = o.renumber != 0;
-  c->pC = hilbert_order(nC, m->xCell, m->yCell, m->zCell, ren);
-  c->pE = hilbert_order(nE, m->xEdge, m->yEdge, m->zEdge, ren);
-  c->pV = hilbert_order(nV, m->xVertex, m->yVertex, m->zVertex, ren);
+  const double pos_scale = o.geometry_scaled ? 1.0 / o.radius : 1.0;
+  c->pC = hilbert_order(nC, m->xCell, m->yCell, m->zCell, ren, pos_scale);
+  c->pE = hilbert_order(nE, m->xEdge, m->yEdge, m->zEdge, ren, pos_scale);
+  c->pV = hilbert_order(nV, m->xVertex, m->yVertex, m->zVertex, ren, pos_scale);
   c->iC.resize(nC);
   c->iE.resize(nE);
   c->iV.resize(nV);
   for (int i = 0; i < nC; ++i) c->iC[c->pC[i]] = i;
   for (int i = 0; i < nE; ++i) c->iE[c->pE[i]] = i;
   for (int i = 0; i < nV; ++i) c->iV[c->pV[i]] = i;
-  const double a = o.radius, a2 = a * a;
+  // Lengths and areas: scale unit-sphere values as mesh_mod.F90:205-224 does,
+  // or take them as they are when the caller has already scaled them.
+  const double a = o.geometry_scaled ? 1.0 : o.radius, a2 = a * a;
 
   // ---- cells
   std::vector<int8_t> nEoC(nC);

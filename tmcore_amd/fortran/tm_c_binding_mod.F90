@@ -35,7 +35,7 @@ module tm_c_binding_mod
   type, bind(c) :: tm_opts
     integer(c_int) :: time_scheme, pv_scheme, interp_pv_order, adv_order, adv_mono, conserve_energy
     real(c_double) :: apvm_weight, dt, radius
-    integer(c_int) :: ntracers, precision, device, renumber
+    integer(c_int) :: ntracers, precision, device, renumber, geometry_scaled
   end type tm_opts
 
   interface

@@ -24,7 +24,8 @@ class Opts(C.Structure):
     _fields_ = [("time_scheme", C.c_int), ("pv_scheme", C.c_int), ("interp_pv_order", C.c_int),
                 ("adv_order", C.c_int), ("adv_mono", C.c_int), ("conserve_energy", C.c_int),
                 ("apvm_weight", C.c_double), ("dt", C.c_double), ("radius", C.c_double),
-                ("ntracers", C.c_int), ("precision", C.c_int), ("device", C.c_int), ("renumber", C.c_int)]
+                ("ntracers", C.c_int), ("precision", C.c_int), ("device", C.c_int), ("renumber", C.c_int),
+                ("geometry_scaled", C.c_int)]
 
 
 class StageOut(C.Structure):
