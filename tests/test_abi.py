@@ -19,10 +19,6 @@ def declared(header):
     return sorted(set(re.findall(r"\b(tm_[a-z0-9_]+)\s*\(", txt)))
 
 
-def exported(lib):
-    return {n for n in dir(lib)}  # placeholder, replaced by direct lookups below
-
-
 @pytest.mark.parametrize("header,libname", [("tmcore_host.h", "libtmhost.so"), ("tmcore_gpu.h", "libtmgpu.so")])
 def test_every_declared_symbol_is_exported(header, libname):
     path = os.path.join(_ffi.LIB_DIR, libname)
@@ -50,16 +46,20 @@ def test_ctx_create_rejects_bad_mesh_before_touching_the_device():
     assert not h.value
 
 
-def test_ctx_create_rejects_unsupported_scheme():
+@pytest.mark.parametrize("field,value", [("time_scheme", 9), ("pv_scheme", 1), ("adv_order", 3),
+                                         ("precision", 16), ("ntracers", -1), ("dt", -1.0)])
+def test_ctx_create_rejects_unsupported_options(field, value):
     from tmcore_amd.swm import Opts, gpu_lib
     m = Mesh(1)
     L = gpu_lib()
     o = Opts()
     L.tm_opts_default(C.byref(o))
-    o.time_scheme = 9
+    setattr(o, field, value)
     h = C.c_void_p()
     md = m.desc()
-    assert L.tm_ctx_create(C.byref(h), C.byref(md), None, C.byref(o)) == -1  # TM_EINVAL
+    fd = m.fit().desc()
+    assert L.tm_ctx_create(C.byref(h), C.byref(md), C.byref(fd), C.byref(o)) == -1  # TM_EINVAL
+    assert not h.value
 
 
 def test_host_library_errors():

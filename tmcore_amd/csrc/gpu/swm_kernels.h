@@ -93,25 +93,263 @@ __device__ __forceinline__ void block_sums(const double (&v)[N], const int (&ids
   }
 }
 
+// ----------------------------------------------------------- element code
+// Layouts: where element i of a state or intermediate array lives.  Solo is
+// one model; Member puts the M ensemble members of element i side by side
+// (member fastest), so the lanes of a wavefront that share an element read
+// its mesh constants once and gather neighbour rows contiguously.
+struct Solo {
+  __device__ __forceinline__ size_t operator()(size_t i) const { return i; }
+};
+struct Member {
+  int M, m;
+  __device__ __forceinline__ size_t operator()(size_t i) const { return i * (size_t)M + (size_t)m; }
+};
+
+// gd_edge, u and the per-edge products the cell/vertex sums gather; returns
+// the edge part of the energy invariant when b.want_inv.
+template <typename R, typename L>
+__device__ __forceinline__ double edge_prep_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+  const R gde = R(0.5) * (b.gd[at(c1)] + b.gd[at(c2)]);
+  const R iap = b.iap[at(e)];
+  const R u = iap / tm_sqrt(gde);
+  const R dv = (R)m.dv[e];
+  b.gd_e[at(e)] = gde;
+  b.u[at(e)] = u;
+  b.Fdv[at(e)] = (u * gde) * dv;
+  b.kee[at(e)] = (u * u) * (R)m.aE[e];
+  b.udc[at(e)] = u * (R)m.dc[e];
+  return b.want_inv ? ((double)iap * (double)iap) * m.aE[e] : 0.0;
+}
+
+// CRK4 bookkeeping of one prognostic value: next stage state, accumulator
+// t1 + 2 t2 + 2 t3 (+ t4)/6, and the previous/current tendency for beta.
+template <typename R>
+__device__ __forceinline__ R rk_accumulate(const StageBufs<R> &b, R *acc_arr, size_t o, R t) {
+  R acc;
+  if (b.k == 1)
+    acc = t;
+  else if (b.k < 4)
+    acc = acc_arr[o] + R(2) * t;
+  else
+    acc = (acc_arr[o] + t) / R(6);
+  acc_arr[o] = acc;
+  return acc;
+}
+
+struct CellSums {
+  double mass, enc, rp, r44;
+};
+
+template <typename R, typename L>
+__device__ __forceinline__ CellSums cell_e(const DMesh &m, const StageBufs<R> &b, L at, int c) {
+  CellSums r{0.0, 0.0, 0.0, 0.0};
+  // Unused slots of pentagons hold index 0 (a valid edge): every gather is
+  // issued unconditionally so they overlap, and the unused terms are
+  // replaced by an exact +0 afterwards.
+  const int n = m.nEoC[c];
+  int idx[kMaxEoC];
+#pragma unroll
+  for (int s = 0; s < kMaxEoC; ++s) idx[s] = m.eocS[s * m.nC + c];
+  R fv[kMaxEoC], kv[kMaxEoC];
+#pragma unroll
+  for (int s = 0; s < kMaxEoC; ++s) {
+    const int e = unsign(idx[s]);
+    fv[s] = b.Fdv[at(e)];
+    kv[s] = b.kee[at(e)];
+  }
+  R sdiv = R(0), ske = R(0);
+#pragma unroll
+  for (int s = 0; s < kMaxEoC; ++s) {
+    const R f = idx[s] < 0 ? -fv[s] : fv[s];
+    sdiv += s < n ? f : R(0);
+    ske += s < n ? kv[s] : R(0);
+  }
+  const double A = m.areaCell[c];
+  const R tend = (-sdiv) / (R)A;
+  const R ke = (ske * R(0.25)) / (R)A;
+  const R gd = b.gd[at(c)];
+  const R ghs = b.ghs[at(c)];
+  b.E[at(c)] = (ke + gd) + ghs;
+  b.tend_gd[at(c)] = tend;
+  if (b.d_ke) b.d_ke[at(c)] = ke;
+  if (b.want_inv) {
+    const double g = gd;
+    r.mass = g * A;
+    r.enc = ((g * g) + (2.0 * g) * (double)ghs) * A;
+  }
+  if (b.k >= 1) {
+    if (b.k < 4) b.gd_next[at(c)] = b.gd_base[at(c)] + (R)b.cdt * tend;
+    const R acc = rk_accumulate(b, b.acc_gd, at(c), tend);
+    if (b.k >= 2) r.rp = ((double)b.tprev_gd[at(c)] * (double)tend) * A;
+    if (b.k == 4) r.r44 = ((double)acc * (double)acc) * A;
+    if (b.k < 4) b.tcur_gd[at(c)] = tend;
+  }
+  if (b.nT > 0) {
+    int nb[kMaxEoC];
+    R uv[kMaxEoC], dvv[kMaxEoC];
+#pragma unroll
+    for (int s = 0; s < kMaxEoC; ++s) {
+      const int e = unsign(idx[s]);
+      nb[s] = m.cocN[s * m.nC + c];
+      uv[s] = b.u[at(e)];
+      dvv[s] = (R)m.dv[e];
+    }
+    for (int t = 0; t < b.nT; ++t) {
+      // adv_scheme_sg11_run: f_edge = c2e(q), flux = u*f_edge, div_operator
+      const size_t tb = (size_t)t * m.nC;
+      const R q0 = b.q[at(tb + c)];
+      R qn[kMaxEoC];
+#pragma unroll
+      for (int s = 0; s < kMaxEoC; ++s) qn[s] = b.q[at(tb + nb[s])];
+      R sq = R(0);
+#pragma unroll
+      for (int s = 0; s < kMaxEoC; ++s) {
+        const R qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
+        const R fl = (uv[s] * qe) * dvv[s];
+        const R f = idx[s] < 0 ? -fl : fl;
+        sq += s < n ? f : R(0);
+      }
+      const R tq = (-sq) / (R)A;
+      const size_t o = at(tb + c);
+      if (b.d_tq) b.d_tq[o] = tq;
+      if (b.k >= 1) {
+        if (b.k < 4) b.q_next[o] = b.q_base[o] + (R)b.cdt * tq;
+        (void)rk_accumulate(b, b.acc_q, o, tq);
+      }
+    }
+  }
+  return r;
+}
+
+struct VertSums {
+  double ens, av;
+};
+
+template <typename R, typename L>
+__device__ __forceinline__ VertSums vertex_e(const DMesh &m, const StageBufs<R> &b, L at, int v) {
+  R s = R(0);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int idx = m.eovS[j * m.nV + v];
+    const R f = b.udc[at(unsign(idx))];
+    s += idx < 0 ? -f : f;
+  }
+  const double At = m.areaTri[v];
+  const R vor = (-s) / (R)At;
+  R g = R(0);
+#pragma unroll
+  for (int j = 0; j < 3; ++j) g = g + b.gd[at(m.cov[j * m.nV + v])] * (R)m.kite[j * m.nV + v];
+  const R gdv = g / (R)At;
+  const R pv = ((R)m.fV[v] + vor) / gdv;
+  b.pv_v[at(v)] = pv;
+  if (b.d_vor) b.d_vor[at(v)] = vor;
+  if (b.d_gdv) b.d_gdv[at(v)] = gdv;
+  VertSums r{0.0, 0.0};
+  if (b.want_inv) {
+    r.ens = ((double)gdv * ((double)pv * (double)pv)) * At;
+    r.av = ((double)vor + m.fV[v]) * At;
+  }
+  return r;
+}
+
+// Tangent wind + scheme-3 PV on the edge.  Padded stencil slots point at the
+// element itself with weight 0, so every load is issued unconditionally and
+// a padded term adds an exact +0.
+template <typename R, typename L>
+__device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+  R v = R(0);
+  {
+    int ep[kMaxEoE];
+    R uv[kMaxEoE];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) uv[s] = b.u[at(ep[s])];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
+  }
+  const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
+  const R p1 = b.pv_v[at(v1)], p2 = b.pv_v[at(v2)];
+  R pve = R(0.5) * (p1 + p2);
+  if (b.pv_order == 3 || b.pv_order == 4) {
+    const R coef2 = b.pv_order == 3 ? R(1) : R(0);
+    int f1[kFit2], f2[kFit2];
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      f1[j] = m.fit2V[j * m.nV + v1];
+      f2[j] = m.fit2V[j * m.nV + v2];
+    }
+    R q1[kFit2], q2[kFit2];
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      q1[j] = b.pv_v[at(f1[j])];
+      q2[j] = b.pv_v[at(f2[j])];
+    }
+    R d1 = R(0), d2 = R(0);
+#pragma unroll
+    for (int j = 0; j < kFit2; ++j) {
+      d1 += (R)m.d2v[(size_t)j * m.nE + e] * (q1[j] - p1);
+      d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (q2[j] - p2);
+    }
+    const R dv = (R)m.dv[e];
+    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1(v) * coef2) * (d2 - d1))) / R(12);
+  }
+  b.pv_e[at(e)] = pve;
+  if (b.d_v) b.d_v[at(e)] = v;
+}
+
+struct EdgeSums {
+  double rp, r44;
+};
+
+// TRiSK PV flux, edge gradient, IAP tendency and the RK update of iap_u.
+template <typename R, typename L>
+__device__ __forceinline__ EdgeSums edge_tend_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+  const R pve = b.pv_e[at(e)];
+  int ep[kMaxEoE];
+  R uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
+#pragma unroll
+  for (int q = 0; q < kMaxEoE; ++q) ep[q] = m.eoe[q * m.nE + e];
+#pragma unroll
+  for (int q = 0; q < kMaxEoE; ++q) {
+    uv[q] = b.u[at(ep[q])];
+    gv[q] = b.gd_e[at(ep[q])];
+    pv[q] = b.pv_e[at(ep[q])];
+  }
+  R s = R(0);
+#pragma unroll
+  for (int q = 0; q < kMaxEoE; ++q)
+    s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
+  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+  const R dEdx = (b.E[at(c2)] - b.E[at(c1)]) / (R)m.dc[e];
+  const R ut = s - dEdx;
+  const R sg = tm_sqrt(b.gd_e[at(e)]);
+  const R tge = R(0.5) * (b.tend_gd[at(c1)] + b.tend_gd[at(c2)]);
+  const R it = sg * ut + ((R(0.5) * b.u[at(e)]) / sg) * tge;
+  if (b.d_pvflx) b.d_pvflx[at(e)] = s;
+  if (b.d_utend) b.d_utend[at(e)] = ut;
+  if (b.d_tiap) b.d_tiap[at(e)] = it;
+  EdgeSums r{0.0, 0.0};
+  if (b.k >= 1) {
+    const double aE = m.aE[e];
+    if (b.k < 4) b.iap_next[at(e)] = b.iap_base[at(e)] + (R)b.cdt * it;
+    const R acc = rk_accumulate(b, b.acc_iap, at(e), it);
+    if (b.k >= 2) r.rp = ((double)b.tprev_iap[at(e)] * (double)it) * aE;
+    if (b.k == 4) r.r44 = ((double)acc * (double)acc) * aE;
+    if (b.k < 4) b.tcur_iap[at(e)] = it;
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------- kernels
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE);
   const int e = tile * kBlock + threadIdx.x;
   double en = 0.0;
-  if (e < m.nE) {
-    const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-    const R gde = R(0.5) * (b.gd[c1] + b.gd[c2]);
-    const R iap = b.iap[e];
-    const R u = iap / tm_sqrt(gde);
-    const R dv = (R)m.dv[e];
-    b.gd_e[e] = gde;
-    b.u[e] = u;
-    b.Fdv[e] = (u * gde) * dv;
-    b.kee[e] = (u * u) * (R)m.aE[e];
-    b.udc[e] = u * (R)m.dc[e];
-    if (b.want_inv) en = ((double)iap * (double)iap) * m.aE[e];
-  }
+  if (e < m.nE) en = edge_prep_e<R>(m, b, Solo{}, e);
   if (b.want_inv) {
     const double v[1] = {en};
     const int ids[1] = {kEnergyE};
@@ -122,111 +360,21 @@ __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
 template <typename R>
 __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b, int tile) {
   const int c = tile * kBlock + threadIdx.x;
-  double mass = 0.0, enc = 0.0, rp = 0.0, r44 = 0.0;
-  if (c < m.nC) {
-    // Unused slots of pentagons hold index 0 (a valid edge): every gather is
-    // issued unconditionally so they overlap, and the unused terms are
-    // replaced by an exact +0 afterwards.
-    const int n = m.nEoC[c];
-    int idx[kMaxEoC];
-#pragma unroll
-    for (int s = 0; s < kMaxEoC; ++s) idx[s] = m.eocS[s * m.nC + c];
-    R fv[kMaxEoC], kv[kMaxEoC];
-#pragma unroll
-    for (int s = 0; s < kMaxEoC; ++s) {
-      const int e = unsign(idx[s]);
-      fv[s] = b.Fdv[e];
-      kv[s] = b.kee[e];
-    }
-    R sdiv = R(0), ske = R(0);
-#pragma unroll
-    for (int s = 0; s < kMaxEoC; ++s) {
-      const R f = idx[s] < 0 ? -fv[s] : fv[s];
-      sdiv += s < n ? f : R(0);
-      ske += s < n ? kv[s] : R(0);
-    }
-    const double A = m.areaCell[c];
-    const R tend = (-sdiv) / (R)A;
-    const R ke = (ske * R(0.25)) / (R)A;
-    const R gd = b.gd[c];
-    const R ghs = b.ghs[c];
-    b.E[c] = (ke + gd) + ghs;
-    b.tend_gd[c] = tend;
-    if (b.d_ke) b.d_ke[c] = ke;
-    if (b.want_inv) {
-      const double g = gd;
-      mass = g * A;
-      enc = ((g * g) + (2.0 * g) * (double)ghs) * A;
-    }
-    if (b.k >= 1) {
-      if (b.k < 4) b.gd_next[c] = b.gd_base[c] + (R)b.cdt * tend;
-      R acc;
-      if (b.k == 1)
-        acc = tend;
-      else if (b.k < 4)
-        acc = b.acc_gd[c] + R(2) * tend;
-      else
-        acc = (b.acc_gd[c] + tend) / R(6);
-      b.acc_gd[c] = acc;
-      if (b.k >= 2) rp = ((double)b.tprev_gd[c] * (double)tend) * A;
-      if (b.k == 4) r44 = ((double)acc * (double)acc) * A;
-      if (b.k < 4) b.tcur_gd[c] = tend;
-    }
-    int nb[kMaxEoC];
-    R uv[kMaxEoC], dvv[kMaxEoC];
-    if (b.nT > 0) {
-#pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s) {
-        const int e = unsign(idx[s]);
-        nb[s] = m.cocN[s * m.nC + c];
-        uv[s] = b.u[e];
-        dvv[s] = (R)m.dv[e];
-      }
-    }
-    for (int t = 0; t < b.nT; ++t) {
-      // adv_scheme_sg11_run: f_edge = c2e(q), flux = u*f_edge, div_operator
-      const R *qt = b.q + (size_t)t * m.nC;
-      const R q0 = qt[c];
-      R qn[kMaxEoC];
-#pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s) qn[s] = qt[nb[s]];
-      R sq = R(0);
-#pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s) {
-        const R qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
-        const R fl = (uv[s] * qe) * dvv[s];
-        const R f = idx[s] < 0 ? -fl : fl;
-        sq += s < n ? f : R(0);
-      }
-      const R tq = (-sq) / (R)A;
-      const size_t o = (size_t)t * m.nC + c;
-      if (b.d_tq) b.d_tq[o] = tq;
-      if (b.k >= 1) {
-        if (b.k < 4) b.q_next[o] = b.q_base[o] + (R)b.cdt * tq;
-        R acc;
-        if (b.k == 1)
-          acc = tq;
-        else if (b.k < 4)
-          acc = b.acc_q[o] + R(2) * tq;
-        else
-          acc = (b.acc_q[o] + tq) / R(6);
-        b.acc_q[o] = acc;
-      }
-    }
-  }
+  CellSums r{0.0, 0.0, 0.0, 0.0};
+  if (c < m.nC) r = cell_e<R>(m, b, Solo{}, c);
   if (b.want_inv) {
-    const double v[2] = {mass, enc};
+    const double v[2] = {r.mass, r.enc};
     const int ids[2] = {kMass, kEnergyC};
     block_sums<2>(v, ids, b.part, b.part_stride, tile);
   }
   if (b.k >= 2) {
     const int rid = b.k == 2 ? kR12c : (b.k == 3 ? kR23c : kR34c);
     if (b.k == 4) {
-      const double v[2] = {rp, r44};
+      const double v[2] = {r.rp, r.r44};
       const int ids[2] = {rid, kR44c};
       block_sums<2>(v, ids, b.part, b.part_stride, tile);
     } else {
-      const double v[1] = {rp};
+      const double v[1] = {r.rp};
       const int ids[1] = {rid};
       block_sums<1>(v, ids, b.part, b.part_stride, tile);
     }
@@ -236,32 +384,10 @@ __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b,
 template <typename R>
 __device__ __forceinline__ void vertex_part(const DMesh &m, const StageBufs<R> &b, int tile) {
   const int v = tile * kBlock + threadIdx.x;
-  double ens = 0.0, av = 0.0;
-  if (v < m.nV) {
-    R s = R(0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      const int idx = m.eovS[j * m.nV + v];
-      const R f = b.udc[unsign(idx)];
-      s += idx < 0 ? -f : f;
-    }
-    const double At = m.areaTri[v];
-    const R vor = (-s) / (R)At;
-    R g = R(0);
-#pragma unroll
-    for (int j = 0; j < 3; ++j) g = g + b.gd[m.cov[j * m.nV + v]] * (R)m.kite[j * m.nV + v];
-    const R gdv = g / (R)At;
-    const R pv = ((R)m.fV[v] + vor) / gdv;
-    b.pv_v[v] = pv;
-    if (b.d_vor) b.d_vor[v] = vor;
-    if (b.d_gdv) b.d_gdv[v] = gdv;
-    if (b.want_inv) {
-      ens = ((double)gdv * ((double)pv * (double)pv)) * At;
-      av = ((double)vor + m.fV[v]) * At;
-    }
-  }
+  VertSums r{0.0, 0.0};
+  if (v < m.nV) r = vertex_e<R>(m, b, Solo{}, v);
   if (b.want_inv) {
-    const double vv[2] = {ens, av};
+    const double vv[2] = {r.ens, r.av};
     const int ids[2] = {kEnstrophy, kAbsVor};
     block_sums<2>(vv, ids, b.part, b.part_stride, tile);
   }
@@ -278,104 +404,23 @@ __global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b)
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
   const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
-  if (e >= m.nE) return;
-  // Padded stencil slots point at the element itself with weight 0, so every
-  // load is issued unconditionally and a padded term adds an exact +0.
-  R v = R(0);
-  {
-    int ep[kMaxEoE];
-    R uv[kMaxEoE];
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) uv[s] = b.u[ep[s]];
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
-  }
-  const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
-  const R p1 = b.pv_v[v1], p2 = b.pv_v[v2];
-  R pve = R(0.5) * (p1 + p2);
-  if (b.pv_order == 3 || b.pv_order == 4) {
-    const R coef2 = b.pv_order == 3 ? R(1) : R(0);
-    int f1[kFit2], f2[kFit2];
-#pragma unroll
-    for (int j = 0; j < kFit2; ++j) {
-      f1[j] = m.fit2V[j * m.nV + v1];
-      f2[j] = m.fit2V[j * m.nV + v2];
-    }
-    R q1[kFit2], q2[kFit2];
-#pragma unroll
-    for (int j = 0; j < kFit2; ++j) {
-      q1[j] = b.pv_v[f1[j]];
-      q2[j] = b.pv_v[f2[j]];
-    }
-    R d1 = R(0), d2 = R(0);
-#pragma unroll
-    for (int j = 0; j < kFit2; ++j) {
-      d1 += (R)m.d2v[(size_t)j * m.nE + e] * (q1[j] - p1);
-      d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (q2[j] - p2);
-    }
-    const R dv = (R)m.dv[e];
-    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1(v) * coef2) * (d2 - d1))) / R(12);
-  }
-  b.pv_e[e] = pve;
-  if (b.d_v) b.d_v[e] = v;
+  if (e < m.nE) edge_pv_e<R>(m, b, Solo{}, e);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE);
   const int e = tile * kBlock + threadIdx.x;
-  double rp = 0.0, r44 = 0.0;
-  if (e < m.nE) {
-    const R pve = b.pv_e[e];
-    int ep[kMaxEoE];
-    R uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
-#pragma unroll
-    for (int q = 0; q < kMaxEoE; ++q) ep[q] = m.eoe[q * m.nE + e];
-#pragma unroll
-    for (int q = 0; q < kMaxEoE; ++q) {
-      uv[q] = b.u[ep[q]];
-      gv[q] = b.gd_e[ep[q]];
-      pv[q] = b.pv_e[ep[q]];
-    }
-    R s = R(0);
-#pragma unroll
-    for (int q = 0; q < kMaxEoE; ++q)
-      s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
-    const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-    const R dEdx = (b.E[c2] - b.E[c1]) / (R)m.dc[e];
-    const R ut = s - dEdx;
-    const R sg = tm_sqrt(b.gd_e[e]);
-    const R tge = R(0.5) * (b.tend_gd[c1] + b.tend_gd[c2]);
-    const R it = sg * ut + ((R(0.5) * b.u[e]) / sg) * tge;
-    if (b.d_pvflx) b.d_pvflx[e] = s;
-    if (b.d_utend) b.d_utend[e] = ut;
-    if (b.d_tiap) b.d_tiap[e] = it;
-    if (b.k >= 1) {
-      const double aE = m.aE[e];
-      if (b.k < 4) b.iap_next[e] = b.iap_base[e] + (R)b.cdt * it;
-      R acc;
-      if (b.k == 1)
-        acc = it;
-      else if (b.k < 4)
-        acc = b.acc_iap[e] + R(2) * it;
-      else
-        acc = (b.acc_iap[e] + it) / R(6);
-      b.acc_iap[e] = acc;
-      if (b.k >= 2) rp = ((double)b.tprev_iap[e] * (double)it) * aE;
-      if (b.k == 4) r44 = ((double)acc * (double)acc) * aE;
-      if (b.k < 4) b.tcur_iap[e] = it;
-    }
-  }
+  EdgeSums r{0.0, 0.0};
+  if (e < m.nE) r = edge_tend_e<R>(m, b, Solo{}, e);
   if (b.k >= 2) {
     const int rid = b.k == 2 ? kR12e : (b.k == 3 ? kR23e : kR34e);
     if (b.k == 4) {
-      const double v[2] = {rp, r44};
+      const double v[2] = {r.rp, r.r44};
       const int ids[2] = {rid, kR44e};
       block_sums<2>(v, ids, b.part, b.part_stride, tile);
     } else {
-      const double v[1] = {rp};
+      const double v[1] = {r.rp};
       const int ids[1] = {rid};
       block_sums<1>(v, ids, b.part, b.part_stride, tile);
     }

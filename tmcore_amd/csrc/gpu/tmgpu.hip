@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "swm_kernels.h"
+#include "tm_mesh_dev.h"
 #include "tmcore_gpu.h"
 
 using namespace tmk;
@@ -86,8 +87,8 @@ struct tm_ctx {
   int nC = 0, nE = 0, nV = 0, nT = 0;
   size_t rsz = 8;  // bytes per state value
   std::vector<int> pC, pE, pV;   // new -> old
-  std::vector<int> iC, iE, iV;   // old -> new
   DMesh dm{};
+  MeshDev md;
   std::vector<void *> allocs;
   double totalArea = 0.0, sumAreaTri = 0.0;
   double step_bytes[3] = {0, 0, 0};
@@ -368,71 +369,80 @@ extern "C" int tm_ctx_destroy(tm_ctx *c) {
   return TM_OK;
 }
 
-static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
-  const tm_opts &o = c->o;
-  if (o.time_scheme != 2) return c->fail(TM_EINVAL, "only time_scheme = 2 (CRK4) is implemented on the device");
-  if (o.pv_scheme != 3) return c->fail(TM_EINVAL, "only pv_scheme = 3 is implemented on the device");
+namespace tmk {
+
+int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o, std::string &err) {
+  auto failv = [&](int code, const char *msg) {
+    err = msg;
+    return code;
+  };
+  if (o.time_scheme != 2) return failv(TM_EINVAL, "only time_scheme = 2 (CRK4) is implemented on the device");
+  if (o.pv_scheme != 3) return failv(TM_EINVAL, "only pv_scheme = 3 is implemented on the device");
   if (o.interp_pv_order < 2 || o.interp_pv_order > 4)
-    return c->fail(TM_EINVAL, "interp_pv_order must be 2, 3 or 4 on the device");
-  if (o.adv_order != 2) return c->fail(TM_EINVAL, "only adv_order = 2 is implemented on the device");
-  if (o.precision != 64 && o.precision != 32) return c->fail(TM_EINVAL, "precision must be 64 or 32");
-  if (o.ntracers < 0 || o.ntracers > 64) return c->fail(TM_EINVAL, "ntracers out of range");
-  if (!(o.dt > 0.0) || !(o.radius > 0.0)) return c->fail(TM_EINVAL, "dt and radius must be positive");
+    return failv(TM_EINVAL, "interp_pv_order must be 2, 3 or 4 on the device");
+  if (o.adv_order != 2) return failv(TM_EINVAL, "only adv_order = 2 is implemented on the device");
+  if (o.precision != 64 && o.precision != 32) return failv(TM_EINVAL, "precision must be 64 or 32");
+  if (o.ntracers < 0 || o.ntracers > 64) return failv(TM_EINVAL, "ntracers out of range");
+  if (!(o.dt > 0.0) || !(o.radius > 0.0)) return failv(TM_EINVAL, "dt and radius must be positive");
   if (!m || m->nCells <= 0 || m->nEdges <= 0 || m->nVertices <= 0 || m->vertexDegree != 3 || m->maxEdges > 6 ||
       m->maxEdges < 3 || m->maxEdges2 < 1)
-    return c->fail(TM_EINVAL, "bad mesh dimensions");
+    return failv(TM_EINVAL, "bad mesh dimensions");
   const bool need_fit = o.interp_pv_order >= 3;
   if (need_fit && (!f || !f->nFit2Vertices || !f->fit2Vertices || !f->deriv2OnVertex || f->maxFit < 2))
-    return c->fail(TM_EINVAL, "interp_pv_order >= 3 needs nFit2Vertices, fit2Vertices and deriv2OnVertex");
+    return failv(TM_EINVAL, "interp_pv_order >= 3 needs nFit2Vertices, fit2Vertices and deriv2OnVertex");
   const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices, ME = m->maxEdges, ME2 = m->maxEdges2;
-  c->nC = nC;
-  c->nE = nE;
-  c->nV = nV;
-  c->nT = o.ntracers;
-  c->rsz = o.precision == 64 ? 8 : 4;
   auto inr = [](int v, int n) { return v >= 1 && v <= n; };
   for (int i = 0; i < nC; ++i) {
     int n = m->nEdgesOnCell[i];
-    if (n < 3 || n > ME) return c->fail(TM_ERANGE, "nEdgesOnCell out of range");
+    if (n < 3 || n > ME) return failv(TM_ERANGE, "nEdgesOnCell out of range");
     for (int s = 0; s < n; ++s)
       if (!inr(m->edgesOnCell[(size_t)i * ME + s], nE) || !inr(m->verticesOnCell[(size_t)i * ME + s], nV))
-        return c->fail(TM_ERANGE, "edgesOnCell/verticesOnCell index out of range");
+        return failv(TM_ERANGE, "edgesOnCell/verticesOnCell index out of range");
   }
   for (int e = 0; e < nE; ++e) {
     if (!inr(m->cellsOnEdge[2 * (size_t)e], nC) || !inr(m->cellsOnEdge[2 * (size_t)e + 1], nC) ||
         !inr(m->verticesOnEdge[2 * (size_t)e], nV) || !inr(m->verticesOnEdge[2 * (size_t)e + 1], nV))
-      return c->fail(TM_ERANGE, "cellsOnEdge/verticesOnEdge index out of range");
+      return failv(TM_ERANGE, "cellsOnEdge/verticesOnEdge index out of range");
     int n = m->nEdgesOnEdge[e];
-    if (n < 0 || n > kMaxEoE || n > ME2) return c->fail(TM_ESTENCIL, "nEdgesOnEdge exceeds 10");
+    if (n < 0 || n > kMaxEoE || n > ME2) return failv(TM_ESTENCIL, "nEdgesOnEdge exceeds 10");
     for (int s = 0; s < n; ++s)
-      if (!inr(m->edgesOnEdge[(size_t)e * ME2 + s], nE)) return c->fail(TM_ERANGE, "edgesOnEdge index out of range");
+      if (!inr(m->edgesOnEdge[(size_t)e * ME2 + s], nE)) return failv(TM_ERANGE, "edgesOnEdge index out of range");
   }
   for (int v = 0; v < nV; ++v)
     for (int j = 0; j < 3; ++j)
       if (!inr(m->edgesOnVertex[(size_t)v * 3 + j], nE) || !inr(m->cellsOnVertex[(size_t)v * 3 + j], nC))
-        return c->fail(TM_ERANGE, "edgesOnVertex/cellsOnVertex index out of range");
+        return failv(TM_ERANGE, "edgesOnVertex/cellsOnVertex index out of range");
   if (need_fit)
     for (int v = 0; v < nV; ++v) {
       int n = f->nFit2Vertices[v];
-      if (n < 1 || n - 1 > kFit2 || n > f->maxFit) return c->fail(TM_ESTENCIL, "nFit2Vertices exceeds 13");
+      if (n < 1 || n - 1 > kFit2 || n > f->maxFit) return failv(TM_ESTENCIL, "nFit2Vertices exceeds 13");
       for (int j = 1; j < n; ++j)
         if (!inr(f->fit2Vertices[(size_t)v * f->maxFit + j], nV))
-          return c->fail(TM_ERANGE, "fit2Vertices index out of range");
+          return failv(TM_ERANGE, "fit2Vertices index out of range");
     }
 
+  return TM_OK;
+}
+
+int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
+                   hipStream_t stream, std::vector<void *> &allocs, std::string &err) {
+  int rc = validate_mesh(m, f, o, err);
+  if (rc) return rc;
+  const bool need_fit = o.interp_pv_order >= 3;
+  const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices, ME = m->maxEdges, ME2 = m->maxEdges2;
   // Hilbert renumbering: new -> old and old -> new.  Keys are taken on the
   // unit sphere whether or not the caller's coordinates are radius-scaled.
   const bool ren = o.renumber != 0;
   const double pos_scale = o.geometry_scaled ? 1.0 / o.radius : 1.0;
-  c->pC = hilbert_order(nC, m->xCell, m->yCell, m->zCell, ren, pos_scale);
-  c->pE = hilbert_order(nE, m->xEdge, m->yEdge, m->zEdge, ren, pos_scale);
-  c->pV = hilbert_order(nV, m->xVertex, m->yVertex, m->zVertex, ren, pos_scale);
-  c->iC.resize(nC);
-  c->iE.resize(nE);
-  c->iV.resize(nV);
-  for (int i = 0; i < nC; ++i) c->iC[c->pC[i]] = i;
-  for (int i = 0; i < nE; ++i) c->iE[c->pE[i]] = i;
-  for (int i = 0; i < nV; ++i) c->iV[c->pV[i]] = i;
+  md.pC = hilbert_order(nC, m->xCell, m->yCell, m->zCell, ren, pos_scale);
+  md.pE = hilbert_order(nE, m->xEdge, m->yEdge, m->zEdge, ren, pos_scale);
+  md.pV = hilbert_order(nV, m->xVertex, m->yVertex, m->zVertex, ren, pos_scale);
+  md.iC.resize(nC);
+  md.iE.resize(nE);
+  md.iV.resize(nV);
+  for (int i = 0; i < nC; ++i) md.iC[md.pC[i]] = i;
+  for (int i = 0; i < nE; ++i) md.iE[md.pE[i]] = i;
+  for (int i = 0; i < nV; ++i) md.iV[md.pV[i]] = i;
   // Lengths and areas: scale unit-sphere values as mesh_mod.F90:205-224 does,
   // or take them as they are when the caller has already scaled them.
   const double a = o.geometry_scaled ? 1.0 : o.radius, a2 = a * a;
@@ -443,23 +453,23 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   std::vector<double> areaCell(nC), kiteC((size_t)kMaxEoC * nC, 0.0);
   double tot = 0.0;
   for (int i = 0; i < nC; ++i) tot += m->areaCell[i] * a2;  // mesh_mod.F90:224 order
-  c->totalArea = tot;
+  md.totalArea = tot;
   for (int ni = 0; ni < nC; ++ni) {
-    const int oi = c->pC[ni];
+    const int oi = md.pC[ni];
     const int n = m->nEdgesOnCell[oi];
     nEoC[ni] = (int8_t)n;
     areaCell[ni] = m->areaCell[oi] * a2;
     for (int s = n; s < kMaxEoC; ++s) cocN[(size_t)s * nC + ni] = ni;
     for (int s = 0; s < n; ++s) {
       const int oe = m->edgesOnCell[(size_t)oi * ME + s] - 1;
-      const int ne = c->iE[oe];
+      const int ne = md.iE[oe];
       const bool plus = m->cellsOnEdge[2 * (size_t)oe] - 1 == oi;  // nSignEdge = +1
       eocS[(size_t)s * nC + ni] = plus ? ne : ~ne;
       // the cell across the edge (cellsOnCell(s) lies across edgesOnCell(s))
       const int other = plus ? m->cellsOnEdge[2 * (size_t)oe + 1] - 1 : m->cellsOnEdge[2 * (size_t)oe] - 1;
-      cocN[(size_t)s * nC + ni] = c->iC[other];
+      cocN[(size_t)s * nC + ni] = md.iC[other];
       const int ov = m->verticesOnCell[(size_t)oi * ME + s] - 1;
-      vocS[(size_t)s * nC + ni] = c->iV[ov];
+      vocS[(size_t)s * nC + ni] = md.iV[ov];
       double k = 0.0;
       for (int j = 0; j < 3; ++j)
         if (m->cellsOnVertex[(size_t)ov * 3 + j] - 1 == oi) k = m->kiteAreasOnVertex[(size_t)ov * 3 + j] * a2;
@@ -471,11 +481,11 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   std::vector<int8_t> nEoE(nE);
   std::vector<double> dv(nE), dc(nE), aE(nE), woe((size_t)kMaxEoE * nE, 0.0), d2v((size_t)2 * kFit2 * nE, 0.0);
   for (int ne = 0; ne < nE; ++ne) {
-    const int oe = c->pE[ne];
-    coe[ne] = c->iC[m->cellsOnEdge[2 * (size_t)oe] - 1];
-    coe[(size_t)nE + ne] = c->iC[m->cellsOnEdge[2 * (size_t)oe + 1] - 1];
-    voe[ne] = c->iV[m->verticesOnEdge[2 * (size_t)oe] - 1];
-    voe[(size_t)nE + ne] = c->iV[m->verticesOnEdge[2 * (size_t)oe + 1] - 1];
+    const int oe = md.pE[ne];
+    coe[ne] = md.iC[m->cellsOnEdge[2 * (size_t)oe] - 1];
+    coe[(size_t)nE + ne] = md.iC[m->cellsOnEdge[2 * (size_t)oe + 1] - 1];
+    voe[ne] = md.iV[m->verticesOnEdge[2 * (size_t)oe] - 1];
+    voe[(size_t)nE + ne] = md.iV[m->verticesOnEdge[2 * (size_t)oe + 1] - 1];
     dv[ne] = m->dvEdge[oe] * a;
     dc[ne] = m->dcEdge[oe] * a;
     aE[ne] = (m->dvEdge[oe] * m->dcEdge[oe]) * a2;  // mesh_mod.F90:178,221
@@ -483,7 +493,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     nEoE[ne] = (int8_t)n;
     for (int s = 0; s < kMaxEoE; ++s) {
       // padding slots point at the edge itself with weight 0 (never read)
-      eoe[(size_t)s * nE + ne] = s < n ? c->iE[m->edgesOnEdge[(size_t)oe * ME2 + s] - 1] : ne;
+      eoe[(size_t)s * nE + ne] = s < n ? md.iE[m->edgesOnEdge[(size_t)oe * ME2 + s] - 1] : ne;
       woe[(size_t)s * nE + ne] = s < n ? m->weightsOnEdge[(size_t)oe * ME2 + s] : 0.0;
     }
     if (need_fit)
@@ -504,33 +514,28 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   std::vector<double> kite((size_t)3 * nV), areaTri(nV), fV(nV);
   double sat = 0.0;
   for (int v = 0; v < nV; ++v) sat += m->areaTriangle[v] * a2;
-  c->sumAreaTri = sat;
+  md.sumAreaTri = sat;
   for (int nv = 0; nv < nV; ++nv) {
-    const int ov = c->pV[nv];
+    const int ov = md.pV[nv];
     areaTri[nv] = m->areaTriangle[ov] * a2;
     fV[nv] = 2.0 * 7.292e-5 * std::sin(m->latVertex[ov]);  // mesh_mod.F90:180, const_mod.F90:13
     for (int j = 0; j < 3; ++j) {
       const int oe = m->edgesOnVertex[(size_t)ov * 3 + j] - 1;
       const bool plus = m->verticesOnEdge[2 * (size_t)oe] - 1 == ov;  // tSignEdge = +1
-      const int ne = c->iE[oe];
+      const int ne = md.iE[oe];
       eovS[(size_t)j * nV + nv] = plus ? ne : ~ne;
-      cov[(size_t)j * nV + nv] = c->iC[m->cellsOnVertex[(size_t)ov * 3 + j] - 1];
+      cov[(size_t)j * nV + nv] = md.iC[m->cellsOnVertex[(size_t)ov * 3 + j] - 1];
       kite[(size_t)j * nV + nv] = m->kiteAreasOnVertex[(size_t)ov * 3 + j] * a2;
     }
     if (need_fit) {
       const int n = f->nFit2Vertices[ov];
       nFit2V[nv] = (int8_t)n;
       for (int j = 0; j < kFit2; ++j)
-        fit2V[(size_t)j * nV + nv] = j + 1 < n ? c->iV[f->fit2Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
+        fit2V[(size_t)j * nV + nv] = j + 1 < n ? md.iV[f->fit2Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
     }
   }
 
-  // ---- device upload
-  HIPCHK(c, hipSetDevice(o.device));
-  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-  HIPCHK(c, hipEventCreate(&c->ev0));
-  HIPCHK(c, hipEventCreate(&c->ev1));
-  DMesh &d = c->dm;
+  DMesh &d = md.dm;
   d.nC = nC;
   d.nE = nE;
   d.nV = nV;
@@ -539,11 +544,18 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   d.nbV = round_blocks(nV);
 #define UP(field, vec, T)                                  \
   do {                                                     \
-    T *p_ = c->alloc<T>(vec.size());                       \
-    if (!p_) return c->fail(TM_ENOMEM, "hipMalloc " #vec); \
-    int rc_ = upload(c, p_, vec);                          \
-    if (rc_) return rc_;                                   \
-    d.field = p_;                                          \
+    void *p_ = nullptr;                                    \
+    if (hipMalloc(&p_, std::max<size_t>(vec.size(), 1) * sizeof(T)) != hipSuccess) { \
+      err = "hipMalloc " #vec;                             \
+      return TM_ENOMEM;                                    \
+    }                                                      \
+    allocs.push_back(p_);                                  \
+    if (hipMemcpyAsync(p_, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice, stream) != hipSuccess || \
+        hipStreamSynchronize(stream) != hipSuccess) {       \
+      err = "upload " #vec;                                \
+      return TM_EDEVICE;                                   \
+    }                                                      \
+    d.field = (T *)p_;                                     \
   } while (0)
   UP(nEoC, nEoC, int8_t);
   UP(eocS, eocS, int);
@@ -569,6 +581,49 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   UP(kiteC, kiteC, double);
 #undef UP
 
+  md.sumEoE = md.sumFitE = md.sumFitV = md.sumEoC = 0.0;
+  for (int e = 0; e < nE; ++e) md.sumEoE += m->nEdgesOnEdge[e];
+  for (int i = 0; i < nC; ++i) md.sumEoC += m->nEdgesOnCell[i];
+  if (need_fit) {
+    for (int e = 0; e < nE; ++e)
+      md.sumFitE += (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e] - 1] - 1) +
+                    (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e + 1] - 1] - 1);
+    for (int v = 0; v < nV; ++v) md.sumFitV += f->nFit2Vertices[v] - 1;
+  }
+  return TM_OK;
+}
+
+}  // namespace tmk
+
+static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
+  const tm_opts &o = c->o;
+  {
+    std::string e;
+    int rc = validate_mesh(m, f, o, e);
+    if (rc) return c->fail(rc, e);
+  }
+  const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices;
+  c->nC = nC;
+  c->nE = nE;
+  c->nV = nV;
+  c->nT = o.ntracers;
+  c->rsz = o.precision == 64 ? 8 : 4;
+  HIPCHK(c, hipSetDevice(o.device));
+  HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+  HIPCHK(c, hipEventCreate(&c->ev0));
+  HIPCHK(c, hipEventCreate(&c->ev1));
+  {
+    std::string e;
+    int rc = build_mesh_dev(c->md, m, f, o, c->stream, c->allocs, e);
+    if (rc) return c->fail(rc, e);
+  }
+  c->dm = c->md.dm;
+  c->pC = c->md.pC;
+  c->pE = c->md.pE;
+  c->pV = c->md.pV;
+  c->totalArea = c->md.totalArea;
+  c->sumAreaTri = c->md.sumAreaTri;
+  const DMesh &d = c->dm;
   const size_t nq = (size_t)c->nT * nC;
   void **rbufs_c[] = {&c->gd_old, &c->ghs, &c->gdS[0], &c->gdS[1], &c->acc_gd, &c->t_gd[0], &c->t_gd[1],
                       &c->E, &c->tend_gd, &c->scratch_c[0], &c->scratch_c[1]};
@@ -595,15 +650,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
 
   // Algorithmic bytes (compulsory traffic, actual stencil nnz): constants of
   // one stage + 56 B (FP64) per prognostic element, finalize 24 B per element.
-  double sumEoE = 0.0, sumFitE = 0.0, sumFitV = 0.0, sumEoC = 0.0;
-  for (int e = 0; e < nE; ++e) sumEoE += m->nEdgesOnEdge[e];
-  for (int i = 0; i < nC; ++i) sumEoC += m->nEdgesOnCell[i];
-  if (need_fit) {
-    for (int e = 0; e < nE; ++e)
-      sumFitE += (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e] - 1] - 1) +
-                 (f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)e + 1] - 1] - 1);
-    for (int v = 0; v < nV; ++v) sumFitV += f->nFit2Vertices[v] - 1;
-  }
+  const double sumEoE = c->md.sumEoE, sumFitE = c->md.sumFitE, sumFitV = c->md.sumFitV, sumEoC = c->md.sumEoC;
   const double cst = nE * (8.0 + 8.0 + 8.0 + 8.0) + sumEoE * 12.0 + sumFitE * 8.0 +
                      nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0) + sumFitV * 4.0 + sumEoC * 4.0 + nC * (8.0 + 8.0);
   const double rs = (double)c->rsz;
