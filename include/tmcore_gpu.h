@@ -137,6 +137,42 @@ int tm_kernel_bytes(tm_ctx *ctx, double bytes[TM_K_COUNT]);
  * known-size kernel that scales rocprof's FETCH_SIZE/WRITE_SIZE counters. */
 int tm_hbm_copy_probe(int device, double bytes, int iters, double *gbps);
 
+/* ------------------------------------------------------------------------
+ * Batched ensemble (no reference counterpart): many members on meshes of
+ * uneven size, each level group advanced with its own dt, all of them in one
+ * persistent kernel launch (replayed from a hipGraph) per tm_ens_run.
+ * Members of a level share its mesh constants; state is stored member-fastest.
+ * Every member runs the same CRK4 + beta scheme as tm_step (precision from
+ * tm_opts, FP64 double-double global sums and invariants per member).
+ * ------------------------------------------------------------------------ */
+typedef struct tm_ens tm_ens;
+
+typedef struct tm_ens_level {
+  const tm_mesh_desc *mesh;  /* reference-layout mesh of this level group   */
+  const tm_fit_desc *fit;    /* poly_fit weights (interp_pv_order >= 3)     */
+  int members;               /* 1..256                                      */
+  double dt;                 /* seconds                                     */
+} tm_ens_level;
+
+int tm_ens_create(tm_ens **out, int nlevels, const tm_ens_level *levels, const tm_opts *o);
+int tm_ens_destroy(tm_ens *e);
+const char *tm_ens_last_error(const tm_ens *e);
+/* State of one member in reference order (u -> iap_u as tm_state_upload). */
+int tm_ens_upload(tm_ens *e, int level, int member, const double *gd, const double *u, const double *ghs,
+                  const double *tracers);
+int tm_ens_download(tm_ens *e, int level, int member, double *gd, double *iap_u, double *tracers);
+/* Advances level l by nsteps[l] CRK4 steps; levels with fewer steps are
+ * spread evenly over the run.  *ms = device time of the launch.  The four
+ * invariants of every member after the run come back through
+ * tm_ens_invariants. */
+int tm_ens_run(tm_ens *e, const int *nsteps, double *ms);
+int tm_ens_invariants(tm_ens *e, int level, int member, double out[4]);
+/* Tuning: elements per work chunk and workgroups per CU of the persistent
+ * grid.  tm_ens_autotune times candidates over `ticks` ticks on a saved copy
+ * of the state (restored afterwards) and keeps the fastest. */
+int tm_ens_set_tuning(tm_ens *e, int chunk, int blocks_per_cu);
+int tm_ens_autotune(tm_ens *e, int ticks, int best[2], double *best_ms_per_tick);
+
 #ifdef __cplusplus
 }
 #endif

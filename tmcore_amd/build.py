@@ -48,7 +48,8 @@ def build_host():
 
 def build_gpu():
     os.makedirs(LIB, exist_ok=True)
-    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "gpu", "tmgpu.hip"), "-o", os.path.join(LIB, "libtmgpu.so")])
+    srcs = [os.path.join(CSRC, "gpu", f) for f in ("tmgpu.hip", "ens.hip")]
+    _run([HIPCC, *HIP_FLAGS, *srcs, "-o", os.path.join(LIB, "libtmgpu.so")])
 
 
 def build_fortran():

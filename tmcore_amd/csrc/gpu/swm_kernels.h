@@ -441,7 +441,7 @@ struct ReduceArgs {
   double *sums;  // [kNumSums]
 };
 
-__global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs a) {
+static __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs a) {
   __shared__ double2 sh[kReduceBlock / 64];
   const int i = blockIdx.x;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -502,7 +502,7 @@ __device__ __forceinline__ void write_invariants(const StepOut &o, int slot) {
 }
 
 // Invariants-only pass (the post-loop refresh): slot hist_idx - 1.
-__global__ void k_inv_out(StepOut o) {
+static __global__ void k_inv_out(StepOut o) {
   if (threadIdx.x == 0 && blockIdx.x == 0) write_invariants(o, *o.hist_idx - 1);
 }
 
@@ -557,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, const R *gd, con
   iap[e] = tm_sqrt(gde) * u[e];
 }
 
-__global__ void k_reset_counter(int *p) {
+static __global__ void k_reset_counter(int *p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0;
 }
 
