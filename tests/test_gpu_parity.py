@@ -153,10 +153,9 @@ def test_renumbering_does_not_change_results():
 
 def test_radius_scaled_geometry_input():
     """geometry_scaled = 1 takes the arrays as mesh_init leaves them (radius-scaled)."""
-    import copy
     from tmcore_amd.mesh import RADIUS
     m = mesh(4)
-    ms = copy.deepcopy(m)
+    ms = Mesh(4)  # a second copy, scaled in place below
     for k in ("xCell", "yCell", "zCell", "xEdge", "yEdge", "zEdge", "xVertex", "yVertex", "zVertex",
               "dvEdge", "dcEdge", "dv1Edge", "dv2Edge"):
         setattr(ms, k, getattr(m, k) * RADIUS)
