@@ -49,6 +49,41 @@ def cpu_baseline(mesh, fit, state, dt, budget_s=15.0):
             "s_per_step": el / n}
 
 
+def ensemble_bench(days=2.0):
+    """Batched ensemble: 128 members on level 6 (dt 300 s), 32 on level 7
+    (dt 150 s), 8 on level 8 (dt 90 s), FP32 state with FP64 invariants, each
+    member seeded 1000+m, all advanced in one persistent launch."""
+    from tmcore_amd.ensemble import Ensemble
+    from tmcore_amd.mesh import Mesh, seeded_state
+    spec = [(6, 128, 300.0), (7, 32, 150.0), (8, 8, 90.0)]
+    meshes = [Mesh(L) for L, _, _ in spec]
+    ens = Ensemble([(m, m.fit(), M, dt) for m, (L, M, dt) in zip(meshes, spec)], precision=32)
+    seed = 1000
+    for li, (m, (L, M, dt)) in enumerate(zip(meshes, spec)):
+        for k in range(M):
+            gd, u, ghs, _ = seeded_state(m, seed)
+            seed += 1
+            ens.upload(li, k, gd, u, ghs)
+    (chunk, bpc), tick_ms = ens.autotune(ticks=2)
+    steps = [int(round(days * 86400 / dt)) for _, _, dt in spec]
+    ms = ens.run(steps)
+    member_days = sum(M for _, M, _ in spec) * days
+    inv = np.array([ens.invariants(li, k) for li, (_, M, _) in enumerate(spec) for k in range(M)])
+    out = {"member_days_per_hour": member_days / (ms / 3.6e6), "run_s": ms / 1e3, "member_days": member_days,
+           "members": {f"L{L}": M for L, M, _ in spec}, "steps": {f"L{L}": s for (L, _, _), s in zip(spec, steps)},
+           "autotune": {"chunk": chunk, "blocks_per_cu": bpc, "ms_per_tick": round(tick_ms, 4)},
+           "finite_invariants": bool(np.isfinite(inv).all())}
+    # per-size step latency: each level group alone, 20 steps
+    lat = {}
+    for li, (L, M, dt) in enumerate(spec):
+        ns = [0] * len(spec)
+        ns[li] = 20
+        lat[f"L{L}x{M}"] = round(ens.run(ns) / 20, 4)
+    out["step_latency_ms_alone"] = lat
+    ens.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +95,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=8)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp32", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-days", type=float, default=2.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -126,6 +163,10 @@ def main():
                          "l2_height_rel_err_vs_fp64": float(norm(h32 - h64) / norm(h64)),
                          "after_steps": args.warmup + args.steps}
         log(f"FP32: {forms['fp32']}")
+
+    if not args.no_ensemble:
+        forms["ensemble_fp32"] = ensemble_bench(args.ensemble_days)
+        log(f"ensemble: {forms['ensemble_fp32']}")
 
     cpu = None
     if not args.no_cpu:
