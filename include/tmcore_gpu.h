@@ -173,6 +173,82 @@ int tm_ens_invariants(tm_ens *e, int level, int member, double out[4]);
 int tm_ens_set_tuning(tm_ens *e, int chunk, int blocks_per_cu);
 int tm_ens_autotune(tm_ens *e, int ticks, int best[2], double *best_ms_per_tick);
 
+/* ------------------------------------------------------------------------
+ * Learned closure (no reference counterpart): a per-edge MLP
+ * 40 -> 128 -> 128 -> 128 -> 2 with GELU, BF16 weights on MFMA and FP32
+ * accumulation, trained and run by the library's own kernels.
+ * Frame: kappa(e) = +1 when the mesh normal cellsOnEdge(1) -> (2) points
+ * east (north if exactly meridional), else -1; velocities and both outputs
+ * are taken along kappa(e) * n(e).
+ * Inputs of edge e, j = field*10 + slot: 4 fields on the edge (slot 0) and on
+ * 9 edgesOnEdge neighbours (slots 1..9: the west cell's other edges
+ * counter-clockwise from e, then the east cell's; the 10th is not used):
+ *   0: kappa(e) * nSign(e,c) * nSign(e',c) * u(e'), c the shared cell
+ *   1: gd_edge(e')   2: pv_edge(e')   3: (tend_gd(c1(e')) + tend_gd(c2(e')))/2
+ * so the inputs do not depend on the mesh's edge orientation; each is
+ * normalised as (value - in_shift[j]) * in_scale[j].
+ * Outputs: y0, a normal-velocity tendency kappa * out_scale[0] * y0 (m s^-2),
+ * and y1, an edge flux F = kappa * out_scale[1] * y1 along the mesh normal
+ * whose divergence corrects the thickness,
+ *   P_c = -(1/A_c) sum_i nSign(i,c) F(e_i) dv(e_i),
+ * so total mass is unchanged.  Attached to a context, every tm_step computes
+ * the inputs from the state at the start of the step and, after the CRK4
+ * update, adds
+ *   iap_u += dt * sqrt(gd_edge_start) * kappa * out_scale[0] * y0,
+ *   gd    += dt * P.
+ * Without it (or with mlp == NULL) the plain method runs unchanged.
+ * ------------------------------------------------------------------------ */
+#define TM_MLP_NIN 40
+#define TM_MLP_NHID 128
+#define TM_MLP_NOUT 2
+#define TM_MLP_NPARAMS 38530
+typedef struct tm_mlp tm_mlp;
+
+/* Parameters: W1[40][128] b1[128] W2[128][128] b2 W3[128][128] b3 W4[128][2]
+ * b4[2], FP32 masters (BF16 MFMA operands are packed from them). */
+int tm_mlp_create(tm_mlp **out, int device, uint64_t seed);
+int tm_mlp_destroy(tm_mlp *p);
+const char *tm_mlp_last_error(const tm_mlp *p);
+int tm_mlp_get_params(tm_mlp *p, float *params);
+int tm_mlp_set_params(tm_mlp *p, const float *params);
+int tm_mlp_set_scales(tm_mlp *p, const float *in_shift /*[40]*/, const float *in_scale /*[40]*/,
+                      const float *out_scale /*[2]*/);
+int tm_mlp_get_scales(tm_mlp *p, float *in_shift, float *in_scale, float *out_scale);
+/* y[n][2] = MLP(x[n][40]), host arrays (x already scaled). */
+int tm_mlp_forward(tm_mlp *p, const float *x, int n, float *y);
+/* Gradients of L = sum(dy * y) w.r.t. the parameters (grad[TM_MLP_NPARAMS])
+ * and, if dx != NULL, w.r.t. x (dx[n][40]). */
+int tm_mlp_backward(tm_mlp *p, const float *x, const float *dy, int n, float *grad, float *dx);
+/* One Adam step (beta 0.9/0.999, eps 1e-8) with the given gradient. */
+int tm_mlp_adam_step(tm_mlp *p, const float *grad, float lr);
+
+/* Normalised inputs of the context's current state, reference edge order:
+ * x[nEdges][40]. */
+int tm_closure_features(tm_ctx *ctx, tm_mlp *mlp, float *x);
+/* Applies the closure in every following tm_step (mlp == NULL detaches).
+ * The MLP must outlive the attachment; its parameters and scales may change
+ * between steps (training) without re-attaching. */
+int tm_closure_attach(tm_ctx *ctx, tm_mlp *mlp);
+/* Adds one training sample held on the device: the inputs of the context's
+ * current state (normalised with mlp's scales at this call) and its one-step
+ * targets in reference order, along the mesh normal,
+ *   t_u[nEdges]  = (iap_target - iap_plain) / (dt * sqrt(gd_edge)),
+ *   t_gd[nCells] = (gd_target - gd_plain) / dt,
+ * where *_plain is one plain step from the current state and gd_edge is of
+ * the current state.  The loss of a sample is
+ *   mean_e (y0 - kappa t_u/out_scale[0])^2 + mean_c ((P_c - t_gd_c)/tg_scale)^2. */
+int tm_closure_dataset_add(tm_ctx *ctx, tm_mlp *mlp, const double *t_u, const double *t_gd, double tg_scale);
+int tm_closure_dataset_size(tm_ctx *ctx, int *n);
+int tm_closure_dataset_clear(tm_ctx *ctx);
+/* Mean loss of every sample in [first, first + count) under the current
+ * parameters, summed per term: loss[0] (velocity) and loss[1] (thickness). */
+int tm_closure_eval(tm_ctx *ctx, tm_mlp *mlp, int first, int count, double *loss);
+/* epochs passes over samples [0, ntrain), one Adam step (lr, beta 0.9/0.999)
+ * per sample in an order shuffled per epoch from seed; loss_hist[2*epochs]
+ * receives the two loss terms averaged over each epoch (before each step). */
+int tm_closure_train(tm_ctx *ctx, tm_mlp *mlp, int ntrain, int epochs, float lr, uint64_t seed,
+                     double *loss_hist);
+
 #ifdef __cplusplus
 }
 #endif

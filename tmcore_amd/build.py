@@ -48,7 +48,7 @@ def build_host():
 
 def build_gpu():
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(CSRC, "gpu", f) for f in ("tmgpu.hip", "ens.hip")]
+    srcs = [os.path.join(CSRC, "gpu", f) for f in ("tmgpu.hip", "ens.hip", "mlp.hip")]
     _run([HIPCC, *HIP_FLAGS, *srcs, "-o", os.path.join(LIB, "libtmgpu.so")])
 
 

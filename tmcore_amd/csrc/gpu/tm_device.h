@@ -32,6 +32,7 @@ struct DMesh {
   const int *voe;         // [2][nE] verticesOnEdge
   const double *dv, *dc, *aE;  // [nE] dvEdge, dcEdge, areaEdge = dv*dc (radius-scaled)
   const int8_t *nEoE;
+  const int8_t *kapE;     // [nE] +1 if the normal cellsOnEdge(1)->(2) points east, else -1
   const int *eoe;         // [10][nE] edgesOnEdge
   const double *woe;      // [10][nE] weightsOnEdge
   const double *d2v;      // [2][12][nE] deriv2OnVertex slots 1..12

@@ -14,9 +14,11 @@
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <random>
 #include <string>
 #include <vector>
 
+#include "closure_kernels.h"
 #include "swm_kernels.h"
 #include "tm_mesh_dev.h"
 #include "tmcore_gpu.h"
@@ -111,6 +113,17 @@ struct tm_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_state = false;
   bool timed = false;
+  // learned closure: attached MLP, its per-step buffers and the training set
+  tm_mlp *cl = nullptr;
+  float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
+  void *cl_sq = nullptr;
+  double *cl_loss = nullptr;
+  int cl_loss_cap = 0;
+  struct ClSample {
+    float *X, *tu, *tg;
+    float tgs;
+  };
+  std::vector<ClSample> ds;
 
   int fail(int code, const std::string &m) {
     err = m;
@@ -277,14 +290,48 @@ FinalBufs<R> final_bufs(tm_ctx *c) {
   return f;
 }
 
+// Closure inputs of the stage-1 fields (the start-of-step state) into X,
+// and sqrt(gd_edge) into sq when sq != nullptr.
+template <typename R>
+int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
+  ClFeatArgs<R> a{};
+  a.u = (const R *)c->u;
+  a.gd_e = (const R *)c->gd_e;
+  a.pv_e = (const R *)c->pv_e;
+  a.tend_gd = (const R *)c->tend_gd;
+  a.norm = mlp_state(c->cl).norm;
+  a.X = X;
+  a.sq = (R *)sq;
+  hipLaunchKernelGGL(k_cl_features<R>, dim3(c->dm.nbE), dim3(kBlock), 0, c->stream, c->dm, a);
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
 template <typename R>
 int enqueue_step(tm_ctx *c) {
   for (int k = 1; k <= 4; ++k) {
     int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
     if (rc) return rc;
+    if (k == 1 && c->cl) {
+      // MLP of the start-of-step state, between stage 1 and stage 2 (which
+      // overwrites u, gd_edge, pv_edge and tend_gd)
+      if ((rc = enqueue_cl_features<R>(c, c->cl_X, c->cl_sq))) return rc;
+      MlpState &ms = mlp_state(c->cl);
+      if ((rc = mlp_forward(ms, c->cl_X, c->nE, c->cl_Y, false, c->stream))) return c->fail(rc, ms.err);
+    }
   }
   hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 1, true));
   hipLaunchKernelGGL(k_finalize<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, final_bufs<R>(c));
+  if (c->cl) {
+    ClApplyArgs<R> a{};
+    a.Y = c->cl_Y;
+    a.sq = (const R *)c->cl_sq;
+    a.norm = mlp_state(c->cl).norm;
+    a.dt = c->o.dt;
+    a.iap = (R *)c->iap_old;
+    a.gd = (R *)c->gd_old;
+    hipLaunchKernelGGL(k_cl_apply<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, a);
+  }
   HIPCHK(c, hipGetLastError());
   return TM_OK;
 }
@@ -361,6 +408,11 @@ extern "C" int tm_ctx_destroy(tm_ctx *c) {
   if (!c) return TM_EINVAL;
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (auto &smp : c->ds) {
+    (void)hipFree(smp.X);
+    (void)hipFree(smp.tu);
+    (void)hipFree(smp.tg);
+  }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (void *p : c->allocs) (void)hipFree(p);
@@ -478,12 +530,20 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   }
   // ---- edges
   std::vector<int> coe(2 * (size_t)nE), voe(2 * (size_t)nE), eoe((size_t)kMaxEoE * nE, 0);
-  std::vector<int8_t> nEoE(nE);
+  std::vector<int8_t> nEoE(nE), kapE(nE);
   std::vector<double> dv(nE), dc(nE), aE(nE), woe((size_t)kMaxEoE * nE, 0.0), d2v((size_t)2 * kFit2 * nE, 0.0);
   for (int ne = 0; ne < nE; ++ne) {
     const int oe = md.pE[ne];
     coe[ne] = md.iC[m->cellsOnEdge[2 * (size_t)oe] - 1];
     coe[(size_t)nE + ne] = md.iC[m->cellsOnEdge[2 * (size_t)oe + 1] - 1];
+    {
+      // +1 when the normal cellsOnEdge(1) -> (2) points east (north if it is
+      // exactly meridional), -1 otherwise
+      const int c1 = m->cellsOnEdge[2 * (size_t)oe] - 1, c2 = m->cellsOnEdge[2 * (size_t)oe + 1] - 1;
+      const double nx = m->xCell[c2] - m->xCell[c1], ny = m->yCell[c2] - m->yCell[c1], nz = m->zCell[c2] - m->zCell[c1];
+      const double de = -nx * m->yEdge[oe] + ny * m->xEdge[oe];
+      kapE[ne] = de > 0.0 ? 1 : (de < 0.0 ? -1 : (nz >= 0.0 ? 1 : -1));
+    }
     voe[ne] = md.iV[m->verticesOnEdge[2 * (size_t)oe] - 1];
     voe[(size_t)nE + ne] = md.iV[m->verticesOnEdge[2 * (size_t)oe + 1] - 1];
     dv[ne] = m->dvEdge[oe] * a;
@@ -566,6 +626,7 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   UP(dc, dc, double);
   UP(aE, aE, double);
   UP(nEoE, nEoE, int8_t);
+  UP(kapE, kapE, int8_t);
   UP(eoe, eoe, int);
   UP(woe, woe, double);
   UP(d2v, d2v, double);
@@ -925,6 +986,199 @@ extern "C" int tm_stage(tm_ctx *c, tm_stage_out *out) {
   if (!c || !out) return TM_EINVAL;
   if (!c->have_state) return c->fail(TM_ESTATE, "tm_stage before tm_state_upload");
   return c->rsz == 8 ? stage_impl<double>(c, out) : stage_impl<float>(c, out);
+}
+
+// ------------------------------------------------------------ learned closure
+namespace {
+
+int cl_check(tm_ctx *c, tm_mlp *mlp) {
+  if (!c) return TM_EINVAL;
+  if (!mlp) return c->fail(TM_EINVAL, "null MLP");
+  if (mlp_state(mlp).device != c->o.device) return c->fail(TM_EINVAL, "MLP and context are on different devices");
+  return TM_OK;
+}
+
+int cl_reserve(tm_ctx *c) {
+  if (c->cl_X) return TM_OK;
+  if (!(c->cl_X = c->alloc<float>((size_t)c->nE * kMlpIn)) || !(c->cl_Y = c->alloc<float>((size_t)c->nE * kMlpOut)) ||
+      !(c->cl_dY = c->alloc<float>((size_t)c->nE * kMlpOut)) || !(c->cl_r = c->alloc<float>(c->nC)) ||
+      !(c->cl_sq = c->alloc_r(c->nE)))
+    return c->fail(TM_ENOMEM, "hipMalloc closure buffers");
+  return TM_OK;
+}
+
+int cl_loss_reserve(tm_ctx *c, int n) {
+  if (n > c->cl_loss_cap) {
+    if (!(c->cl_loss = c->alloc<double>(n))) return c->fail(TM_ENOMEM, "hipMalloc closure loss");
+    c->cl_loss_cap = n;
+  }
+  HIPCHK(c, hipMemsetAsync(c->cl_loss, 0, (size_t)n * sizeof(double), c->stream));
+  return TM_OK;
+}
+
+// Inputs of the current state into X (new order): a diagnostic stage of the
+// state followed by the input kernel, with mlp's normalisation.
+int cl_features_now(tm_ctx *c, tm_mlp *mlp, float *X) {
+  tm_mlp *saved = c->cl;
+  c->cl = mlp;  // enqueue_cl_features reads the normalisation of c->cl
+  int rc = c->rsz == 8 ? launch_stage<double>(c, stage_bufs<double>(c, 0, false), true)
+                       : launch_stage<float>(c, stage_bufs<float>(c, 0, false), true);
+  if (!rc) rc = c->rsz == 8 ? enqueue_cl_features<double>(c, X, nullptr) : enqueue_cl_features<float>(c, X, nullptr);
+  c->cl = saved;
+  return rc;
+}
+
+ClLossArgs cl_loss_args(tm_ctx *c, tm_mlp *mlp, const tm_ctx::ClSample &smp, bool want_dy, double *loss) {
+  ClLossArgs a{};
+  a.Y = c->cl_Y;
+  a.tu = smp.tu;
+  a.tg = smp.tg;
+  a.norm = mlp_state(mlp).norm;
+  a.tgs = smp.tgs;
+  a.r = c->cl_r;
+  a.dY = want_dy ? c->cl_dY : nullptr;
+  a.loss = loss;
+  return a;
+}
+
+int cl_enqueue_loss(tm_ctx *c, const ClLossArgs &a) {
+  hipLaunchKernelGGL(k_cl_resid, dim3(c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, a);
+  hipLaunchKernelGGL(k_cl_dy, dim3(c->dm.nbE), dim3(kBlock), 0, c->stream, c->dm, a);
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
+template <typename T>
+int upload_perm_f32(tm_ctx *c, float *dst, const T *src, const std::vector<int> &perm) {
+  std::vector<float> h(perm.size());
+  for (size_t i = 0; i < perm.size(); ++i) h[i] = (float)src[perm[i]];
+  return upload(c, dst, h);
+}
+
+}  // namespace
+
+extern "C" int tm_closure_attach(tm_ctx *c, tm_mlp *mlp) {
+  if (!c) return TM_EINVAL;
+  if (mlp) {
+    int rc = cl_check(c, mlp);
+    if (rc || (rc = cl_reserve(c))) return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  // the captured step changes shape: capture again at the next run
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->cl = mlp;
+  return TM_OK;
+}
+
+extern "C" int tm_closure_features(tm_ctx *c, tm_mlp *mlp, float *x) {
+  int rc = cl_check(c, mlp);
+  if (rc) return rc;
+  if (!x) return c->fail(TM_EINVAL, "null output");
+  if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
+  if ((rc = cl_reserve(c)) || (rc = cl_features_now(c, mlp, c->cl_X))) return rc;
+  std::vector<float> h((size_t)c->nE * kMlpIn);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->cl_X, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (int i = 0; i < c->nE; ++i)
+    std::memcpy(x + (size_t)c->pE[i] * kMlpIn, h.data() + (size_t)i * kMlpIn, kMlpIn * 4);
+  return TM_OK;
+}
+
+extern "C" int tm_closure_dataset_add(tm_ctx *c, tm_mlp *mlp, const double *t_u, const double *t_gd, double tg_scale) {
+  int rc = cl_check(c, mlp);
+  if (rc) return rc;
+  if (!t_u || !t_gd) return c->fail(TM_EINVAL, "null target");
+  if (!(tg_scale > 0.0) || !std::isfinite(tg_scale)) return c->fail(TM_EINVAL, "tg_scale must be positive");
+  if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
+  if (!finite_all(t_u, c->nE) || !finite_all(t_gd, c->nC)) return c->fail(TM_ENONFINITE, "non-finite target");
+  if ((rc = cl_reserve(c))) return rc;
+  tm_ctx::ClSample smp{nullptr, nullptr, nullptr, (float)tg_scale};
+  if (hipMalloc((void **)&smp.X, (size_t)c->nE * kMlpIn * 4) != hipSuccess ||
+      hipMalloc((void **)&smp.tu, (size_t)c->nE * 4) != hipSuccess ||
+      hipMalloc((void **)&smp.tg, (size_t)c->nC * 4) != hipSuccess) {
+    (void)hipFree(smp.X);
+    (void)hipFree(smp.tu);
+    (void)hipFree(smp.tg);
+    return c->fail(TM_ENOMEM, "hipMalloc closure sample");
+  }
+  c->ds.push_back(smp);  // owned by the context from here on
+  if ((rc = cl_features_now(c, mlp, smp.X)) || (rc = upload_perm_f32(c, smp.tu, t_u, c->pE)) ||
+      (rc = upload_perm_f32(c, smp.tg, t_gd, c->pC)))
+    return rc;
+  return TM_OK;
+}
+
+extern "C" int tm_closure_dataset_size(tm_ctx *c, int *n) {
+  if (!c || !n) return TM_EINVAL;
+  *n = (int)c->ds.size();
+  return TM_OK;
+}
+
+extern "C" int tm_closure_dataset_clear(tm_ctx *c) {
+  if (!c) return TM_EINVAL;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &smp : c->ds) {
+    (void)hipFree(smp.X);
+    (void)hipFree(smp.tu);
+    (void)hipFree(smp.tg);
+  }
+  c->ds.clear();
+  return TM_OK;
+}
+
+extern "C" int tm_closure_eval(tm_ctx *c, tm_mlp *mlp, int first, int count, double *loss) {
+  int rc = cl_check(c, mlp);
+  if (rc) return rc;
+  if (!loss || first < 0 || count < 1 || (size_t)first + count > c->ds.size())
+    return c->fail(TM_EINVAL, "sample range outside the dataset");
+  MlpState &ms = mlp_state(mlp);
+  HIPCHK(c, hipStreamSynchronize(ms.stream));
+  if ((rc = cl_reserve(c)) || (rc = cl_loss_reserve(c, 2))) return rc;
+  for (int i = first; i < first + count; ++i) {
+    if ((rc = mlp_forward(ms, c->ds[i].X, c->nE, c->cl_Y, false, c->stream))) return c->fail(rc, ms.err);
+    if ((rc = cl_enqueue_loss(c, cl_loss_args(c, mlp, c->ds[i], false, c->cl_loss)))) return rc;
+  }
+  HIPCHK(c, hipMemcpyAsync(loss, c->cl_loss, 2 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  loss[0] /= count;
+  loss[1] /= count;
+  return TM_OK;
+}
+
+extern "C" int tm_closure_train(tm_ctx *c, tm_mlp *mlp, int ntrain, int epochs, float lr, uint64_t seed,
+                                double *loss_hist) {
+  int rc = cl_check(c, mlp);
+  if (rc) return rc;
+  if (ntrain < 1 || (size_t)ntrain > c->ds.size() || epochs < 1) return c->fail(TM_EINVAL, "bad ntrain or epochs");
+  if (!(lr > 0.f) || !std::isfinite(lr)) return c->fail(TM_EINVAL, "lr must be positive");
+  MlpState &ms = mlp_state(mlp);
+  HIPCHK(c, hipStreamSynchronize(ms.stream));
+  if ((rc = cl_reserve(c)) || (rc = cl_loss_reserve(c, 2 * epochs))) return rc;
+  if ((rc = mlp_reserve(ms, c->nE)) || (rc = mlp_zero_grad(ms, c->stream))) return c->fail(rc, ms.err);
+  std::vector<int> order(ntrain);
+  std::iota(order.begin(), order.end(), 0);
+  std::mt19937_64 rng(seed);
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    for (int i : order) {
+      const tm_ctx::ClSample &smp = c->ds[i];
+      if ((rc = mlp_forward(ms, smp.X, c->nE, c->cl_Y, true, c->stream))) return c->fail(rc, ms.err);
+      if ((rc = cl_enqueue_loss(c, cl_loss_args(c, mlp, smp, true, c->cl_loss + 2 * ep)))) return rc;
+      if ((rc = mlp_backward(ms, smp.X, c->cl_dY, c->nE, nullptr, c->stream)) ||
+          (rc = mlp_adam(ms, lr, 0.9f, 0.999f, 1e-8f, c->stream)))
+        return c->fail(rc, ms.err);
+    }
+  }
+  std::vector<double> h(2 * (size_t)epochs);
+  HIPCHK(c, hipMemcpyAsync(h.data(), c->cl_loss, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  for (auto &v : h) v /= ntrain;
+  if (loss_hist) std::memcpy(loss_hist, h.data(), h.size() * sizeof(double));
+  if (!finite_all(h.data(), h.size())) return c->fail(TM_ENONFINITE, "non-finite training loss");
+  return TM_OK;
 }
 
 #include "ops_kernels.h"
