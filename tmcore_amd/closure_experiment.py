@@ -1,0 +1,182 @@
+"""Closure experiment driver: train on coarse-grained fine-mesh truth, then
+evaluate multi-day forecasts against it.
+
+    python -m tmcore_amd.closure_experiment train --seeds 5000:5100 --out W.npz [--init W0.npz]
+    python -m tmcore_amd.closure_experiment eval --weights W.npz --seeds 6000:6004 --days 10 --out R.json
+
+Training samples are pairs at coarse steps --samples of every initial state
+(seeded_state(fine, seed)); the last --holdout seeds' samples are kept for
+validation.  Progress lines go to stderr as each initial state or epoch ends.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import sys
+import time
+
+import numpy as np
+
+from .closure import Closure, Mlp
+from .closure_train import STEPS_PER_DAY, Pipeline, l2_thickness, load, save
+from .coarse import CoarseGrainer
+from .mesh import Mesh, seeded_state
+from .swm import SwmGpu
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def seed_range(s):
+    a, b = s.split(":")
+    return list(range(int(a), int(b)))
+
+
+def cmd_train(args):
+    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log)
+    seeds = seed_range(args.seeds)
+    samples = [int(k) for k in args.samples.split(",")]
+    mlp = Mlp(seed=args.init_seed)
+    clo = Closure(p.coarse, mlp)
+    t0 = time.perf_counter()
+    if args.init:
+        meta0 = load(args.init, mlp)
+        p.tg_scale = meta0["scales"]["rms_t_gd"]
+        scales = meta0["scales"]
+        pending = []
+    else:
+        pending = [(a, b, g) for s in seeds[:2] for _, a, b, g in p.truth_pairs(s, samples)]
+        scales = p.fit_scales(mlp, pending)
+    # zero-output network with the same scales: the loss of the plain method
+    zero = Mlp(seed=0)
+    zp = zero.params
+    zp[-(128 * 2 + 2):] = 0.0
+    zero.params = zp
+    zero.set_scales(*mlp.get_scales())
+    for a, b, g in pending:
+        p.add(clo, a, b, g)
+    used = seeds[:2] if not args.init else []
+    for s in seeds[2:] if not args.init else seeds:
+        if time.perf_counter() - t0 > args.data_budget:
+            log(f"data budget of {args.data_budget:g} s reached: stopping before seed {s}")
+            break
+        for _, a, b, g in p.truth_pairs(s, samples):
+            p.add(clo, a, b, g)
+        used.append(s)
+        log(f"seed {s}: {clo.size()} samples, {time.perf_counter() - t0:.0f} s")
+    seeds = used
+    n = clo.size()
+    nval = args.holdout * len(samples)
+    ntrain = n - nval
+    data_s = time.perf_counter() - t0
+    clo0 = Closure(p.coarse, zero)
+    plain_val = clo0.eval(ntrain, nval).tolist()
+    plain_train = clo0.eval(0, ntrain).tolist()
+    log(f"{ntrain} training / {nval} validation samples; plain-method loss train {plain_train} val {plain_val}")
+    hist = []
+    t1 = time.perf_counter()
+    for ep in range(args.epochs):
+        lr = args.lr * 0.5 * (1 + math.cos(math.pi * ep / args.epochs))
+        h = clo.train(ntrain, 1, lr, seed=args.shuffle_seed + ep)[0].tolist()
+        v = clo.eval(ntrain, nval).tolist()
+        hist.append({"epoch": ep, "lr": lr, "train": h, "val": v})
+        log(f"epoch {ep}: lr {lr:.2e} train {h} val {v}")
+    train_s = time.perf_counter() - t1
+    meta = {"fine_level": args.fine_level, "coarse_level": args.coarse_level, "dt": args.dt, "nsub": args.nsub,
+            "seeds": [seeds[0], seeds[-1] + 1], "samples": samples, "ntrain": ntrain, "nval": nval,
+            "scales": scales, "plain_loss": {"train": plain_train, "val": plain_val}, "history": hist,
+            "data_seconds": data_s, "train_seconds": train_s, "init": args.init,
+            "previous": (meta0 if args.init else None)}
+    save(args.out, mlp, meta)
+    log(f"saved {args.out}")
+
+
+def cmd_eval(args):
+    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log)
+    mlp = Mlp(seed=0)
+    meta = load(args.weights, mlp)
+    cm = p.cm
+    plain = SwmGpu(cm, cm.fit(), dt=args.dt)
+    corr = SwmGpu(cm, cm.fit(), dt=args.dt)
+    Closure(corr, mlp).attach()
+    # finer plain runs: compared with the truth filtered to their own level
+    # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so the
+    # partition noise of the Lr -> coarse map applies to run and truth alike
+    others = {}
+    for spec in filter(None, args.compare.split(",")):
+        L, dt = spec.split(":")
+        m = Mesh(int(L))
+        others[f"L{L}"] = (m, SwmGpu(m, m.fit(), dt=float(dt)), CoarseGrainer(m, cm, velocity=False),
+                           CoarseGrainer(p.fm, m, velocity=False))
+    res = {"weights": args.weights, "days": args.days, "seeds": {}, "speed_sim_days_per_hour": {}}
+    area = p.area
+    walls = {}
+    for s in seed_range(args.seeds):
+        truth, ghs_c, mapped = p.truth_daily(s, args.days, [o[3] for o in others.values()])
+        gd_t = [t[0] for t in truth]
+        r = {}
+        for name, model in (("plain", plain), ("closure", corr)):
+            t0 = time.perf_counter()
+            gds = p.coarse_run(model, truth[0], ghs_c, args.days)
+            walls.setdefault(f"L{args.coarse_level}_{name}", []).append(time.perf_counter() - t0)
+            r[f"L{args.coarse_level}_{name}"] = [l2_thickness(g, t, area) for g, t in zip(gds, gd_t)]
+        for k, (name, (m, model, cg, _)) in enumerate(others.items()):
+            gd, u, ghs, _ = seeded_state(m, s)
+            model.upload(gd, ghs, u=u)
+            per_day = int(round(STEPS_PER_DAY / model.opts.dt))
+
+            def err(gd_r, d):
+                return float(np.sqrt(np.sum(area * cg.cell_mean(gd_r - mapped[k][d]) ** 2)
+                                     / np.sum(area * gd_t[d] ** 2)))
+            errs = [err(gd, 0)]
+            t0 = time.perf_counter()
+            for d in range(args.days):
+                model.step(per_day)
+                errs.append(err(model.download()["gd"], d + 1))
+            walls.setdefault(f"{name}_plain", []).append(time.perf_counter() - t0)
+            r[f"{name}_plain"] = errs
+        res["seeds"][s] = r
+        log(f"seed {s}: final-day errors " + ", ".join(f"{k} {v[-1]:.3e}" for k, v in r.items()))
+    keys = list(next(iter(res["seeds"].values())).keys())
+    res["mean"] = {k: np.nanmean([res["seeds"][s][k] for s in res["seeds"]], axis=0).tolist() for k in keys}
+    res["speed_sim_days_per_hour_wall"] = {k: args.days / (np.mean(v) / 3600.0) for k, v in walls.items()}
+    res["train_meta"] = {k: meta[k] for k in ("seeds", "samples", "ntrain", "nval", "plain_loss")}
+    res["train_meta"]["final_epoch"] = meta["history"][-1] if meta["history"] else None
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+    log("mean errors by day: " + json.dumps({k: [round(x, 6) for x in v] for k, v in res["mean"].items()}))
+    log(f"wrote {args.out}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    for name in ("train", "eval"):
+        a = sub.add_parser(name)
+        a.add_argument("--fine-level", type=int, default=9)
+        a.add_argument("--coarse-level", type=int, default=6)
+        a.add_argument("--dt", type=float, default=300.0)
+        a.add_argument("--nsub", type=int, default=8)
+        a.add_argument("--seeds", required=True)
+        a.add_argument("--out", required=True)
+    t = sub.choices["train"]
+    t.add_argument("--samples", default="0,24,48,72,96,120,144")
+    t.add_argument("--epochs", type=int, default=30)
+    t.add_argument("--lr", type=float, default=1e-3)
+    t.add_argument("--holdout", type=int, default=5)
+    t.add_argument("--init", default=None)
+    t.add_argument("--init-seed", type=int, default=0)
+    t.add_argument("--shuffle-seed", type=int, default=0)
+    t.add_argument("--data-budget", type=float, default=1e9, help="seconds of truth generation before training")
+    e = sub.choices["eval"]
+    e.add_argument("--weights", required=True)
+    e.add_argument("--days", type=int, default=10)
+    e.add_argument("--compare", default="7:150,8:90")
+    args = ap.parse_args()
+    cmd_train(args) if args.cmd == "train" else cmd_eval(args)
+
+
+if __name__ == "__main__":
+    main()

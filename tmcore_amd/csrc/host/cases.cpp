@@ -74,11 +74,14 @@ extern "C" int tm_case_seeded(const tm_mesh_desc *m, double radius, uint64_t see
 
   const int32_t nC = m->nCells, nE = m->nEdges, nV = m->nVertices;
   std::vector<double> psi(nV);
+#pragma omp parallel for schedule(static)
   for (int32_t v = 0; v < nV; ++v) psi[v] = -radius * u0 * std::sin(m->latVertex[v]);
+#pragma omp parallel for schedule(static)
   for (int32_t e = 0; e < nE; ++e) {
     int32_t v1 = m->verticesOnEdge[2 * (size_t)e] - 1, v2 = m->verticesOnEdge[2 * (size_t)e + 1] - 1;
     u[e] = -(psi[v2] - psi[v1]) / (m->dvEdge[e] * radius);
   }
+#pragma omp parallel for schedule(static)
   for (int32_t c = 0; c < nC; ++c) {
     double la = m->latCell[c], lo = m->lonCell[c];
     double dm = radius * angle_to(la, lo, mc) / mount_half;
@@ -97,6 +100,7 @@ extern "C" int tm_case_seeded(const tm_mesh_desc *m, double radius, uint64_t see
       SplitMix64 rt{tracer_seeds[t]};
       for (int32_t c = 0; c < nC; ++c) a[c] = rt.uniform();
       for (int pass = 0; pass < 4; ++pass) {
+#pragma omp parallel for schedule(static)
         for (int32_t c = 0; c < nC; ++c) {
           int n = m->nEdgesOnCell[c];
           double s = 0.0;
