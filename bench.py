@@ -84,6 +84,41 @@ def ensemble_bench(days=2.0):
     return out
 
 
+CLOSURE_WEIGHTS = os.path.join(ROOT, "tmcore_amd", "data", "closure_L6.npz")
+
+
+def closure_bench(days=1, seed=6000):
+    """Closure form: level 6 (40,962 cells, dt 300 s) with the trained closure
+    attached.  Speed: one simulated day of the plain and the corrected
+    level-6 model and of plain level 7 and 8, timed with HIP events.  Accuracy: held-out seed
+    6000 over `days` days against the level-9 FP32 truth coarse-grained to
+    level 6, beside plain level-7 (dt 150 s) and level-8 (dt 90 s) runs of
+    the same initial state, each compared with the truth at its own level and
+    both mapped to level 6 (relative L2 of the thickness)."""
+    from tmcore_amd.closure_experiment import Evaluator
+    from tmcore_amd.closure_train import Pipeline
+    from tmcore_amd.mesh import seeded_state
+    if not os.path.exists(CLOSURE_WEIGHTS):
+        return {"skipped": f"no trained weights at {os.path.relpath(CLOSURE_WEIGHTS, ROOT)}"}
+    p = Pipeline(9, 6, 300.0, 8, log=log)
+    ev = Evaluator(p, CLOSURE_WEIGHTS, "7:150,8:90")
+    speed = {}
+    for name, g in (("L6_plain", ev.plain), ("L6_closure", ev.corr), ("L7_plain", ev.others["L7"][1]),
+                    ("L8_plain", ev.others["L8"][1])):
+        gd, u, ghs, _ = seeded_state(g.mesh, seed)
+        g.upload(gd, ghs, u=u)
+        n = int(round(86400.0 / g.opts.dt))
+        g.step_async(20)
+        g.sync()
+        g.step_async(n)
+        g.sync()
+        ms = g.last_elapsed_ms()
+        speed[name] = {"ms_per_step": round(ms / n, 5), "sim_days_per_hour": round(1.0 / (ms / 3.6e6), 1)}
+    errs, _ = ev.run_seed(seed, days)
+    return {"speed": speed, f"l2_h_rel_day{days}_seed{seed}": {k: v[-1] for k, v in errs.items()},
+            "trained_on": ev.meta["seeds"], "weights": os.path.relpath(CLOSURE_WEIGHTS, ROOT)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -97,6 +132,8 @@ def main():
     ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=2.0)
+    ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--closure-days", type=int, default=1)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -167,6 +204,10 @@ def main():
     if not args.no_ensemble:
         forms["ensemble_fp32"] = ensemble_bench(args.ensemble_days)
         log(f"ensemble: {forms['ensemble_fp32']}")
+
+    if not args.no_closure:
+        forms["closure_L6"] = closure_bench(args.closure_days)
+        log(f"closure: {forms['closure_L6']}")
 
     cpu = None
     if not args.no_cpu:

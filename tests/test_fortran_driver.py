@@ -36,7 +36,7 @@ ntracers     = 1
 seed         = 7
 tracer_seed0 = 77
 print_every  = 1
-/
+{extra}/
 """
 
 LINE = re.compile(r"step=\s*(\d+) total_mass=\s*(\S+) total_energy=\s*(\S+) total_potential_enstropy=\s*(\S+) "
@@ -58,7 +58,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
         pytest.skip("Fortran driver not built (amdflang absent when building)")
     level, dt, hours = 4, 600.0, 2
     nml = tmp_path / "namelist.tmcore_sw"
-    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours))
+    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=""))
     r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = parse(r.stdout)
@@ -82,3 +82,34 @@ def test_fortran_driver_rejects_missing_namelist(tmp_path):
     r = subprocess.run([EXE, str(tmp_path / "nope.nml")], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "cannot open namelist" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_driver_attaches_closure_file(tmp_path):
+    """closure_file: a zero-output closure leaves every diagnostic line of the
+    plain run unchanged; a non-zero one changes them and keeps the mass."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang absent when building)")
+    from tmcore_amd.closure import NPARAMS
+    level, dt, hours = 3, 600.0, 1
+    rng = np.random.default_rng(3)
+    params = (rng.standard_normal(NPARAMS) * 0.05).astype("<f4")
+    scales = np.concatenate([np.zeros(40), np.ones(40), [1e-6, 1e6]]).astype("<f4")
+    zero = params.copy()
+    zero[-(128 * 2 + 2):] = 0.0
+    outs = {}
+    for name, p in (("plain", None), ("zero", zero), ("random", params)):
+        extra = ""
+        if p is not None:
+            f = tmp_path / f"{name}.bin"
+            np.concatenate([p, scales]).astype("<f4").tofile(f)
+            extra = f"closure_file = '{f}'\n"
+        nml = tmp_path / f"nml_{name}"
+        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=extra))
+        r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[name] = parse(r.stdout)
+    assert outs["zero"] == outs["plain"]
+    last = max(outs["plain"])
+    assert outs["random"][last][1] != outs["plain"][last][1]  # energy changed
+    assert abs(outs["random"][last][0] / outs["plain"][last][0] - 1) < 1e-13  # mass kept

@@ -93,37 +93,42 @@ def cmd_train(args):
     log(f"saved {args.out}")
 
 
-def cmd_eval(args):
-    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log)
-    mlp = Mlp(seed=0)
-    meta = load(args.weights, mlp)
-    cm = p.cm
-    plain = SwmGpu(cm, cm.fit(), dt=args.dt)
-    corr = SwmGpu(cm, cm.fit(), dt=args.dt)
-    Closure(corr, mlp).attach()
-    # finer plain runs: compared with the truth filtered to their own level
-    # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so the
-    # partition noise of the Lr -> coarse map applies to run and truth alike
-    others = {}
-    for spec in filter(None, args.compare.split(",")):
-        L, dt = spec.split(":")
-        m = Mesh(int(L))
-        others[f"L{L}"] = (m, SwmGpu(m, m.fit(), dt=float(dt)), CoarseGrainer(m, cm, velocity=False),
-                           CoarseGrainer(p.fm, m, velocity=False))
-    res = {"weights": args.weights, "days": args.days, "seeds": {}, "speed_sim_days_per_hour": {}}
-    area = p.area
-    walls = {}
-    for s in seed_range(args.seeds):
-        truth, ghs_c, mapped = p.truth_daily(s, args.days, [o[3] for o in others.values()])
+class Evaluator:
+    """Plain and corrected coarse runs, and plain finer runs, against the
+    coarse-grained truth of held-out initial states."""
+
+    def __init__(self, p: Pipeline, weights: str, compare: str = "7:150,8:90"):
+        self.p = p
+        self.mlp = Mlp(seed=0)
+        self.meta = load(weights, self.mlp)
+        cm = p.cm
+        self.plain = SwmGpu(cm, cm.fit(), dt=p.dt)
+        self.corr = SwmGpu(cm, cm.fit(), dt=p.dt)
+        Closure(self.corr, self.mlp).attach()
+        # finer plain runs: compared with the truth filtered to their own level
+        # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so
+        # the partition noise of the Lr -> coarse map applies to run and truth alike
+        self.others = {}
+        for spec in filter(None, compare.split(",")):
+            L, dt = spec.split(":")
+            m = Mesh(int(L))
+            self.others[f"L{L}"] = (m, SwmGpu(m, m.fit(), dt=float(dt)), CoarseGrainer(m, cm, velocity=False),
+                                    CoarseGrainer(p.fm, m, velocity=False))
+
+    def run_seed(self, seed, days):
+        """{run: [error at day 0..days]}, {run: wall seconds of its days}."""
+        p, area = self.p, self.p.area
+        truth, ghs_c, mapped = p.truth_daily(seed, days, [o[3] for o in self.others.values()])
         gd_t = [t[0] for t in truth]
-        r = {}
-        for name, model in (("plain", plain), ("closure", corr)):
+        r, walls = {}, {}
+        L = p.cm_level
+        for name, model in (("plain", self.plain), ("closure", self.corr)):
             t0 = time.perf_counter()
-            gds = p.coarse_run(model, truth[0], ghs_c, args.days)
-            walls.setdefault(f"L{args.coarse_level}_{name}", []).append(time.perf_counter() - t0)
-            r[f"L{args.coarse_level}_{name}"] = [l2_thickness(g, t, area) for g, t in zip(gds, gd_t)]
-        for k, (name, (m, model, cg, _)) in enumerate(others.items()):
-            gd, u, ghs, _ = seeded_state(m, s)
+            gds = p.coarse_run(model, truth[0], ghs_c, days)
+            walls[f"L{L}_{name}"] = time.perf_counter() - t0
+            r[f"L{L}_{name}"] = [l2_thickness(g, t, area) for g, t in zip(gds, gd_t)]
+        for k, (name, (m, model, cg, _)) in enumerate(self.others.items()):
+            gd, u, ghs, _ = seeded_state(m, seed)
             model.upload(gd, ghs, u=u)
             per_day = int(round(STEPS_PER_DAY / model.opts.dt))
 
@@ -132,16 +137,29 @@ def cmd_eval(args):
                                      / np.sum(area * gd_t[d] ** 2)))
             errs = [err(gd, 0)]
             t0 = time.perf_counter()
-            for d in range(args.days):
+            for d in range(days):
                 model.step(per_day)
                 errs.append(err(model.download()["gd"], d + 1))
-            walls.setdefault(f"{name}_plain", []).append(time.perf_counter() - t0)
+            walls[f"{name}_plain"] = time.perf_counter() - t0
             r[f"{name}_plain"] = errs
+        return r, walls
+
+
+def cmd_eval(args):
+    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log)
+    ev = Evaluator(p, args.weights, args.compare)
+    res = {"weights": args.weights, "days": args.days, "seeds": {}}
+    walls = {}
+    for s in seed_range(args.seeds):
+        r, w = ev.run_seed(s, args.days)
+        for k, v in w.items():
+            walls.setdefault(k, []).append(v)
         res["seeds"][s] = r
         log(f"seed {s}: final-day errors " + ", ".join(f"{k} {v[-1]:.3e}" for k, v in r.items()))
     keys = list(next(iter(res["seeds"].values())).keys())
     res["mean"] = {k: np.nanmean([res["seeds"][s][k] for s in res["seeds"]], axis=0).tolist() for k in keys}
     res["speed_sim_days_per_hour_wall"] = {k: args.days / (np.mean(v) / 3600.0) for k, v in walls.items()}
+    meta = ev.meta
     res["train_meta"] = {k: meta[k] for k in ("seeds", "samples", "ntrain", "nval", "plain_loss")}
     res["train_meta"]["final_epoch"] = meta["history"][-1] if meta["history"] else None
     with open(args.out, "w") as f:

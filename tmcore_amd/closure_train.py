@@ -34,6 +34,13 @@ def save(path, mlp: Mlp, meta: dict):
     np.savez(path, params=mlp.params, in_shift=sh, in_scale=sc, out_scale=out, meta=np.array(json.dumps(meta)))
 
 
+def export_raw(path, npz_path):
+    """The raw weights file the Fortran driver reads (closure_file): little-endian
+    float32 [parameters, 40 input shifts, 40 input scales, 2 output scales]."""
+    z = np.load(npz_path, allow_pickle=False)
+    np.concatenate([z["params"], z["in_shift"], z["in_scale"], z["out_scale"]]).astype("<f4").tofile(path)
+
+
 def load(path, mlp: Mlp) -> dict:
     z = np.load(path, allow_pickle=False)
     mlp.params = z["params"]
@@ -52,6 +59,7 @@ class Pipeline:
         self.fm, self.cm = Mesh(fine_level), Mesh(coarse_level)
         self.cg = CoarseGrainer(self.fm, self.cm)
         self.dt, self.nsub = dt, nsub
+        self.cm_level = coarse_level
         self.fine = SwmGpu(self.fm, self.fm.fit(), dt=dt / nsub, precision=fine_precision)
         self.coarse = SwmGpu(self.cm, self.cm.fit(), dt=dt)
         coe = self.cm.cellsOnEdge - 1

@@ -184,6 +184,36 @@ module tm_c_binding_mod
       real(c_double) :: u(*), gd(*), pv(*), flux(*)
     end function tm_op_tangent_vor_flux
 
+    ! learned closure (include/tmcore_gpu.h): the per-edge MLP and its attachment
+    integer(c_int) function tm_mlp_create(mlp, device, seed) bind(c, name='tm_mlp_create')
+      import :: c_int, c_ptr, c_int64_t
+      type(c_ptr) :: mlp
+      integer(c_int), value :: device
+      integer(c_int64_t), value :: seed
+    end function tm_mlp_create
+
+    integer(c_int) function tm_mlp_destroy(mlp) bind(c, name='tm_mlp_destroy')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: mlp
+    end function tm_mlp_destroy
+
+    integer(c_int) function tm_mlp_set_params(mlp, params) bind(c, name='tm_mlp_set_params')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: mlp
+      real(c_float) :: params(*)
+    end function tm_mlp_set_params
+
+    integer(c_int) function tm_mlp_set_scales(mlp, in_shift, in_scale, out_scale) bind(c, name='tm_mlp_set_scales')
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: mlp
+      real(c_float) :: in_shift(40), in_scale(40), out_scale(2)
+    end function tm_mlp_set_scales
+
+    integer(c_int) function tm_closure_attach(ctx, mlp) bind(c, name='tm_closure_attach')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, mlp
+    end function tm_closure_attach
+
     integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

@@ -64,8 +64,11 @@ module tm_params_mod
   integer :: device = 0
   integer :: renumber = 1
   integer :: print_every = 1         ! steps between diagnostic lines
+  character(max_file_path_len) :: closure_file = 'N/A'  ! trained closure: raw little-endian float32
+                                     ! [38530 parameters, 40 input shifts, 40 input scales, 2 output scales]
 
-  namelist /gpu_control/ mesh_level, precision, ntracers, seed, tracer_seed0, device, renumber, print_every
+  namelist /gpu_control/ mesh_level, precision, ntracers, seed, tracer_seed0, device, renumber, print_every, &
+                         closure_file
 
 contains
 

@@ -55,6 +55,7 @@ module tm_swm_gpu_mod
   type(tm_fit_desc), target :: fit
   type(tm_opts) :: opts
   type(c_ptr) :: ctx = c_null_ptr
+  type(c_ptr) :: mlp = c_null_ptr
 
 contains
 
@@ -127,6 +128,7 @@ contains
     end if
     call check(tm_ctx_create(ctx, mesh, fit, opts), 'tm_ctx_create')
     call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
+    if (trim(closure_file) /= 'N/A') call attach_closure(trim(closure_file))
 
     write(*, '(A,I0,A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh level ', mesh_level, ': ', nCells, ' cells, ', &
       nEdges, ' edges, ', nVertices, ' vertices; dt = ', dt, ' s; precision ', precision
@@ -204,11 +206,39 @@ contains
 
   end subroutine tm_swm_run
 
+  ! The learned closure of a raw weights file (see tm_params_mod closure_file),
+  ! applied in every following step.
+  subroutine attach_closure(path)
+
+    character(*), intent(in) :: path
+    real(c_float), allocatable :: w(:)
+    integer :: un, ios
+    integer, parameter :: nparams = 38530
+
+    allocate(w(nparams + 82))
+    open(newunit=un, file=path, access='stream', form='unformatted', status='old', iostat=ios)
+    if (ios == 0) read(un, iostat=ios) w
+    if (ios /= 0) then
+      write(*, '(A)') '[Error]: cannot read closure_file ' // path
+      stop 1
+    end if
+    close(un)
+    call check(tm_mlp_create(mlp, device, 0_c_int64_t), 'tm_mlp_create')
+    call check(tm_mlp_set_params(mlp, w(1:nparams)), 'tm_mlp_set_params')
+    call check(tm_mlp_set_scales(mlp, w(nparams + 1:nparams + 40), w(nparams + 41:nparams + 80), &
+                                 w(nparams + 81:nparams + 82)), 'tm_mlp_set_scales')
+    call check(tm_closure_attach(ctx, mlp), 'tm_closure_attach')
+    write(*, '(A)') '[Notice]: learned closure attached from ' // path
+
+  end subroutine attach_closure
+
   subroutine tm_swm_final()
 
     integer(c_int) :: rc
     if (c_associated(ctx)) rc = tm_ctx_destroy(ctx)
     ctx = c_null_ptr
+    if (c_associated(mlp)) rc = tm_mlp_destroy(mlp)
+    mlp = c_null_ptr
 
   end subroutine tm_swm_final
 
