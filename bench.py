@@ -18,7 +18,6 @@ import os
 import sys
 import time
 
-import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -50,38 +49,8 @@ def cpu_baseline(mesh, fit, state, dt, budget_s=15.0):
 
 
 def ensemble_bench(days=2.0):
-    """Batched ensemble: 128 members on level 6 (dt 300 s), 32 on level 7
-    (dt 150 s), 8 on level 8 (dt 90 s), FP32 state with FP64 invariants, each
-    member seeded 1000+m, all advanced in one persistent launch."""
-    from tmcore_amd.ensemble import Ensemble
-    from tmcore_amd.mesh import Mesh, seeded_state
-    spec = [(6, 128, 300.0), (7, 32, 150.0), (8, 8, 90.0)]
-    meshes = [Mesh(L) for L, _, _ in spec]
-    ens = Ensemble([(m, m.fit(), M, dt) for m, (L, M, dt) in zip(meshes, spec)], precision=32)
-    seed = 1000
-    for li, (m, (L, M, dt)) in enumerate(zip(meshes, spec)):
-        for k in range(M):
-            gd, u, ghs, _ = seeded_state(m, seed)
-            seed += 1
-            ens.upload(li, k, gd, u, ghs)
-    (chunk, bpc), tick_ms = ens.autotune(ticks=2)
-    steps = [int(round(days * 86400 / dt)) for _, _, dt in spec]
-    ms = ens.run(steps)
-    member_days = sum(M for _, M, _ in spec) * days
-    inv = np.array([ens.invariants(li, k) for li, (_, M, _) in enumerate(spec) for k in range(M)])
-    out = {"member_days_per_hour": member_days / (ms / 3.6e6), "run_s": ms / 1e3, "member_days": member_days,
-           "members": {f"L{L}": M for L, M, _ in spec}, "steps": {f"L{L}": s for (L, _, _), s in zip(spec, steps)},
-           "autotune": {"chunk": chunk, "blocks_per_cu": bpc, "ms_per_tick": round(tick_ms, 4)},
-           "finite_invariants": bool(np.isfinite(inv).all())}
-    # per-size step latency: each level group alone, 20 steps
-    lat = {}
-    for li, (L, M, dt) in enumerate(spec):
-        ns = [0] * len(spec)
-        ns[li] = 20
-        lat[f"L{L}x{M}"] = round(ens.run(ns) / 20, 4)
-    out["step_latency_ms_alone"] = lat
-    ens.close()
-    return out
+    from tmcore_amd.ensemble_bench import config4
+    return config4(days)
 
 
 CLOSURE_WEIGHTS = os.path.join(ROOT, "tmcore_amd", "data", "closure_L6.npz")

@@ -167,11 +167,13 @@ int tm_ens_download(tm_ens *e, int level, int member, double *gd, double *iap_u,
  * tm_ens_invariants. */
 int tm_ens_run(tm_ens *e, const int *nsteps, double *ms);
 int tm_ens_invariants(tm_ens *e, int level, int member, double out[4]);
-/* Tuning: elements per work chunk and workgroups per CU of the persistent
- * grid.  tm_ens_autotune times candidates over `ticks` ticks on a saved copy
- * of the state (restored afterwards) and keeps the fastest. */
-int tm_ens_set_tuning(tm_ens *e, int chunk, int blocks_per_cu);
-int tm_ens_autotune(tm_ens *e, int ticks, int best[2], double *best_ms_per_tick);
+/* Tuning: elements per work chunk, workgroups per CU of the persistent grid
+ * and members per thread (1, 2 or 4; it must divide every level's member
+ * count and span at most 16 bytes).  tm_ens_autotune times candidates over
+ * `ticks` ticks on a saved copy of the state (restored afterwards), keeps the
+ * fastest and returns it in best = {chunk, blocks_per_cu, members_per_thread}. */
+int tm_ens_set_tuning(tm_ens *e, int chunk, int blocks_per_cu, int members_per_thread);
+int tm_ens_autotune(tm_ens *e, int ticks, int best[3], double *best_ms_per_tick);
 
 /* ------------------------------------------------------------------------
  * Learned closure (no reference counterpart): a per-edge MLP
@@ -229,6 +231,11 @@ int tm_closure_features(tm_ctx *ctx, tm_mlp *mlp, float *x);
  * The MLP must outlive the attachment; its parameters and scales may change
  * between steps (training) without re-attaching. */
 int tm_closure_attach(tm_ctx *ctx, tm_mlp *mlp);
+/* The MLP runs on every nsteps-th step (counted from the attach or the last
+ * tm_state_upload) and its correction, with the sqrt(gd_edge) of that step,
+ * is added unchanged in the steps between: the form of a closure trained on
+ * the mean tendency error over nsteps steps.  Default 1. */
+int tm_closure_set_interval(tm_ctx *ctx, int nsteps);
 /* Adds one training sample held on the device: the inputs of the context's
  * current state (normalised with mlp's scales at this call) and its one-step
  * targets in reference order, along the mesh normal,

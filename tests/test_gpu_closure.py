@@ -187,6 +187,36 @@ def test_closure_step_matches_oracle_and_keeps_mass():
     mlp.close()
 
 
+def test_closure_interval_holds_the_correction():
+    """set_interval(3): the MLP runs on step 0 and its correction (with the
+    sqrt(gd_edge) of step 0) is added after each of the 3 steps."""
+    m, g, mlp, (gd, u, ghs), (sh, sc, out) = setup_closure(3, s_out=(2e-5, 1e-1))
+    p = mlp.params.astype(np.float64)
+    dt = 600.0
+    o = Oracle(m, m.fit(), dt=dt)
+    iap = o.initial_iap(gd, u)
+    tg, _, d = o.stage(gd, iap, ghs)
+    xr = mo.features(m, d["u"], d["gd_edge"], d["pv_edge"], tg, sh, sc)
+    Yr, _ = mo.forward(p, xr, bf=True)
+    gd_c, iap_c = gd, iap
+    for _ in range(3):
+        gp, ip, _, _, _ = o.run(1, gd_c, iap_c, ghs)
+        gd_c, iap_c = mo.apply(m, gp, ip, d["gd_edge"], Yr, out, dt, RADIUS)
+    gd_p, iap_p, _, _, _ = o.run(3, gd, iap, ghs)
+    clo = Closure(g, mlp)
+    clo.attach()
+    clo.set_interval(3)
+    print("closure interval 3, L3", flush=True)
+    g.step(3)
+    st = g.download()
+    assert rel_l2(st["gd"] - gd_p, gd_c - gd_p) <= 5e-3
+    assert rel_l2(st["iap_u"] - iap_p, iap_c - iap_p) <= 5e-3
+    with pytest.raises(Exception):
+        clo.set_interval(0)
+    g.close()
+    mlp.close()
+
+
 def test_closure_training_reduces_loss_and_eval_matches_oracle():
     """Targets from a fixed 'teacher' network on the same inputs; the student
     trained by tm_closure_train on the device must fit them."""

@@ -18,8 +18,8 @@ def rel_l2(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
 
 
-def _run_case(precision, tol, ntracers=0, tuning=None):
-    specs = [(3, 5, 600.0, 6), (4, 3, 300.0, 4)]  # (level, members, dt, steps): uneven sizes and counts
+def _run_case(precision, tol, ntracers=0, tuning=None, specs=((3, 5, 600.0, 6), (4, 3, 300.0, 4))):
+    # specs: (level, members, dt, steps): uneven sizes and counts
     meshes = {L: Mesh(L) for L, *_ in specs}
     ens = Ensemble([(meshes[L], meshes[L].fit(), M, dt) for L, M, dt, _ in specs], precision=precision,
                    ntracers=ntracers)
@@ -59,7 +59,24 @@ def test_ensemble_fp32_members_match_single_runs():
 
 
 def test_ensemble_tuning_does_not_change_results():
-    _run_case(64, 1e-12, tuning=(64, 1))
+    _run_case(64, 1e-12, tuning=(64, 1, 1))
+
+
+@pytest.mark.parametrize("precision,width", [(32, 1), (32, 2), (32, 4), (64, 2)])
+def test_ensemble_members_per_thread(precision, width):
+    """W members per thread (vector loads of member rows) gives the same members."""
+    print(f"ensemble precision={precision} members_per_thread={width}", flush=True)
+    _run_case(precision, 1e-12 if precision == 64 else 1e-5, ntracers=1, tuning=(128, 1, width),
+              specs=((3, 8, 600.0, 6), (4, 4, 300.0, 4)))
+
+
+def test_ensemble_rejects_bad_member_width():
+    m = Mesh(2)
+    ens = Ensemble([(m, m.fit(), 6, 600.0)], precision=32)
+    with pytest.raises(Exception):
+        ens.set_tuning(128, 1, 4)  # 4 does not divide 6
+    ens.set_tuning(128, 1, 2)
+    ens.close()
 
 
 def test_ensemble_autotune_restores_state():
@@ -69,8 +86,8 @@ def test_ensemble_autotune_restores_state():
     for k in range(4):
         ens.upload(0, k, gd, u, ghs)
     before = ens.download(0, 2)
-    (chunk, bpc), ms = ens.autotune(ticks=1)
+    (chunk, bpc, width), ms = ens.autotune(ticks=1)
     after = ens.download(0, 2)
     np.testing.assert_array_equal(before["gd"], after["gd"])
-    assert chunk in (64, 128, 256, 512, 1024) and bpc >= 1 and ms > 0
+    assert chunk in (64, 128, 256, 512, 1024) and bpc >= 1 and width in (1, 2, 4) and ms > 0
     ens.close()

@@ -21,7 +21,7 @@ import numpy as np
 from .closure import Closure, Mlp
 from .closure_train import STEPS_PER_DAY, Pipeline, l2_thickness, load, save
 from .coarse import CoarseGrainer
-from .mesh import Mesh, seeded_state
+from .mesh import RADIUS, Mesh, seeded_state
 from .swm import SwmGpu
 
 
@@ -34,8 +34,52 @@ def seed_range(s):
     return list(range(int(a), int(b)))
 
 
+def online_check(p: Pipeline, mlp: Mlp, seed: int, hours: int = 24, every_h: int = 3, alphas=(1.0, 0.5),
+                 intervals=None):
+    """Coarse runs from the coarse-grained truth of `seed`, plain (alpha 0) and
+    with the closure outputs scaled by alpha, the MLP run every `interval`
+    steps (default: the training horizon) and held in between; thickness error
+    against the truth every `every_h` hours (NaN after a run stops)."""
+    steps = int(round(every_h * 3600.0 / p.dt))
+    n = hours // every_h
+    gd, u, ghs, _ = seeded_state(p.fm, seed)
+    p.fine.upload(gd, ghs, u=u)
+    truth = [p._cg_now()]
+    for _ in range(n):
+        p._advance_fine(steps * p.nsub)
+        truth.append(p._cg_now())
+    ghs_c = p.cg.cell_mean(ghs)
+    g = SwmGpu(p.cm, p.cm.fit(), dt=p.dt)
+    clo = Closure(g, mlp)
+    sh, sc, out = mlp.get_scales()
+    res = {}
+    runs = [(0.0, 1)] + [(a, iv) for a in alphas for iv in (intervals or [p.horizon])]
+    for alpha, iv in runs:
+        if alpha == 0.0:
+            clo.detach()
+        else:
+            mlp.set_scales(sh, sc, out * alpha)
+            clo.attach()
+            clo.set_interval(iv)
+        g.upload(truth[0][0], ghs_c, u=truth[0][1])
+        errs = [0.0]
+        for k in range(n):
+            try:
+                g.step(steps)
+                errs.append(l2_thickness(g.download()["gd"], truth[k + 1][0], p.area))
+            except Exception:
+                errs += [float("nan")] * (n - k)
+                break
+        key = "plain" if alpha == 0.0 else f"alpha {alpha:g} interval {iv}"
+        res[key] = errs
+        log(f"online seed {seed} {key}: " + " ".join(f"{x:.3e}" for x in errs))
+    mlp.set_scales(sh, sc, out)
+    g.close()
+    return res
+
+
 def cmd_train(args):
-    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log)
+    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log, horizon=args.horizon)
     seeds = seed_range(args.seeds)
     samples = [int(k) for k in args.samples.split(",")]
     mlp = Mlp(seed=args.init_seed)
@@ -84,10 +128,12 @@ def cmd_train(args):
         hist.append({"epoch": ep, "lr": lr, "train": h, "val": v})
         log(f"epoch {ep}: lr {lr:.2e} train {h} val {v}")
     train_s = time.perf_counter() - t1
+    online = online_check(p, mlp, seeds[-1], args.online_hours) if args.online_hours > 0 else None
     meta = {"fine_level": args.fine_level, "coarse_level": args.coarse_level, "dt": args.dt, "nsub": args.nsub,
+            "horizon": args.horizon,
             "seeds": [seeds[0], seeds[-1] + 1], "samples": samples, "ntrain": ntrain, "nval": nval,
             "scales": scales, "plain_loss": {"train": plain_train, "val": plain_val}, "history": hist,
-            "data_seconds": data_s, "train_seconds": train_s, "init": args.init,
+            "data_seconds": data_s, "train_seconds": train_s, "init": args.init, "online_check": online,
             "previous": (meta0 if args.init else None)}
     save(args.out, mlp, meta)
     log(f"saved {args.out}")
@@ -104,7 +150,9 @@ class Evaluator:
         cm = p.cm
         self.plain = SwmGpu(cm, cm.fit(), dt=p.dt)
         self.corr = SwmGpu(cm, cm.fit(), dt=p.dt)
-        Closure(self.corr, self.mlp).attach()
+        self.closure = Closure(self.corr, self.mlp)
+        self.closure.attach()
+        self.closure.set_interval(int(self.meta.get("horizon", 1)))
         # finer plain runs: compared with the truth filtered to their own level
         # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so
         # the partition noise of the Lr -> coarse map applies to run and truth alike
@@ -168,10 +216,52 @@ def cmd_eval(args):
     log(f"wrote {args.out}")
 
 
+def cmd_check(args):
+    """online_check of trained weights on one held-out seed."""
+    mlp = Mlp(seed=0)
+    meta = load(args.weights, mlp)
+    p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log, horizon=int(meta.get("horizon", 1)))
+    alphas = tuple(float(a) for a in args.alphas.split(","))
+    intervals = [int(i) for i in args.intervals.split(",")]
+    res = online_check(p, mlp, int(args.seeds.split(":")[0]), args.hours, 3, alphas, intervals)
+    with open(args.out, "w") as f:
+        json.dump(res, f, indent=1)
+
+
+def cmd_profile(args):
+    """A short closure workload for kernel profiling: level-6 steps with the
+    closure attached (features, MFMA forward, correction) and two training
+    epochs over synthetic samples (forward, loss, backward, Adam)."""
+    m = Mesh(args.coarse_level)
+    g = SwmGpu(m, m.fit(), dt=args.dt)
+    gd, u, ghs, _ = seeded_state(m, 6000)
+    g.upload(gd, ghs, u=u)
+    mlp = Mlp(seed=1)
+    sh, sc = np.zeros(40), np.ones(40)
+    sh[10:20] = np.mean(gd)
+    sc[:10], sc[10:20], sc[20:30], sc[30:40] = 1 / 11.4, 1 / 3900.0, 1 / 2e-9, 1 / 1.6e-2
+    mlp.set_scales(sh, sc, np.array([1.2e-5, 2.7e-3 * np.mean(m.areaCell) * RADIUS / np.mean(m.dvEdge)]))
+    clo = Closure(g, mlp)
+    rng = np.random.default_rng(0)
+    for k in range(8):
+        clo.add_sample(rng.standard_normal(m.nEdges) * 1.2e-5, rng.standard_normal(m.nCells) * 2.7e-3, 2.7e-3)
+    clo.attach()
+    g.step_async(args.steps)
+    g.sync()
+    log(f"closure steps: {g.last_elapsed_ms() / args.steps:.4f} ms/step")
+    t0 = time.perf_counter()
+    clo.train(8, 2, 1e-4)
+    log(f"training: {(time.perf_counter() - t0) / 16 * 1e3:.3f} ms per sample step")
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
-    for name in ("train", "eval"):
+    pr = sub.add_parser("profile")
+    pr.add_argument("--coarse-level", type=int, default=6)
+    pr.add_argument("--dt", type=float, default=300.0)
+    pr.add_argument("--steps", type=int, default=50)
+    for name in ("train", "eval", "check"):
         a = sub.add_parser(name)
         a.add_argument("--fine-level", type=int, default=9)
         a.add_argument("--coarse-level", type=int, default=6)
@@ -180,7 +270,8 @@ def main():
         a.add_argument("--seeds", required=True)
         a.add_argument("--out", required=True)
     t = sub.choices["train"]
-    t.add_argument("--samples", default="0,24,48,72,96,120,144")
+    t.add_argument("--samples", default="0,15,30,45,60,75,90,105")
+    t.add_argument("--horizon", type=int, default=12, help="coarse steps of the target tendency")
     t.add_argument("--epochs", type=int, default=30)
     t.add_argument("--lr", type=float, default=1e-3)
     t.add_argument("--holdout", type=int, default=5)
@@ -188,12 +279,18 @@ def main():
     t.add_argument("--init-seed", type=int, default=0)
     t.add_argument("--shuffle-seed", type=int, default=0)
     t.add_argument("--data-budget", type=float, default=1e9, help="seconds of truth generation before training")
+    t.add_argument("--online-hours", type=int, default=24, help="online check on the last (held-out) seed")
+    c = sub.choices["check"]
+    c.add_argument("--weights", required=True)
+    c.add_argument("--hours", type=int, default=24)
+    c.add_argument("--alphas", default="1,0.5")
+    c.add_argument("--intervals", default="1,12")
     e = sub.choices["eval"]
     e.add_argument("--weights", required=True)
     e.add_argument("--days", type=int, default=10)
     e.add_argument("--compare", default="7:150,8:90")
     args = ap.parse_args()
-    cmd_train(args) if args.cmd == "train" else cmd_eval(args)
+    {"train": cmd_train, "eval": cmd_eval, "check": cmd_check, "profile": cmd_profile}[args.cmd](args)
 
 
 if __name__ == "__main__":

@@ -3,10 +3,13 @@
 Truth: a fine icosahedral model (level 9 by default, dt = dt_coarse / nsub,
 FP32 form) started from seeded initial states, coarse-grained onto the coarse
 mesh (level 6) by ``CoarseGrainer``.  A training sample is a pair of
-coarse-grained truth states a = CG(X(t)), b = CG(X(t + dt)) and the one-step
-error of the plain coarse model from a:
-    t_u  = (iap(b) - iap(Plain(a))) / (dt * sqrt(gd_edge(a)))
-    t_gd = (gd(b) - gd(Plain(a))) / dt
+coarse-grained truth states a = CG(X(t)), b = CG(X(t + n dt)) and the mean
+tendency error of n plain coarse steps from a (n = horizon):
+    t_u  = (iap(b) - iap(Plain^n(a))) / (n dt sqrt(gd_edge(a)))
+    t_gd = (gd(b) - gd(Plain^n(a))) / (n dt)
+With n = 1 this is the one-step error; a horizon of an hour averages out the
+fast gravity-wave part of the one-step error, which does not accumulate in a
+forecast and which a closure fitted to it feeds back on.
 held on the device (``Closure.add_sample``) and fitted by the library's
 trainer.  Evaluation runs plain and corrected coarse models for several days
 from held-out initial states beside the coarse-grained truth and, for the
@@ -53,12 +56,13 @@ def l2_thickness(gd, gd_ref, area):
 
 
 class Pipeline:
-    def __init__(self, fine_level=9, coarse_level=6, dt=300.0, nsub=8, fine_precision=32, log=print):
+    def __init__(self, fine_level=9, coarse_level=6, dt=300.0, nsub=8, fine_precision=32, log=print, horizon=1):
         self.log = log
         t0 = time.perf_counter()
         self.fm, self.cm = Mesh(fine_level), Mesh(coarse_level)
         self.cg = CoarseGrainer(self.fm, self.cm)
         self.dt, self.nsub = dt, nsub
+        self.horizon = horizon
         self.cm_level = coarse_level
         self.fine = SwmGpu(self.fm, self.fm.fit(), dt=dt / nsub, precision=fine_precision)
         self.coarse = SwmGpu(self.cm, self.cm.fit(), dt=dt)
@@ -82,29 +86,32 @@ class Pipeline:
         return self.cg.cell_mean(st["gd"]), self.cg.edge_velocity(st["u"])
 
     def truth_pairs(self, seed, sample_steps):
-        """Coarse-grained truth pairs (k, a, b, ghs_c) at coarse steps k -> k+1."""
+        """Coarse-grained truth pairs (k, a, b, ghs_c) at coarse steps k -> k + horizon."""
         gd, u, ghs, _ = seeded_state(self.fm, seed)
         self.fine.upload(gd, ghs, u=u)
         ghs_c = self.cg.cell_mean(ghs)
         cur = 0
         for k in sorted(sample_steps):
+            if k < cur:
+                raise ValueError("sample steps closer than the horizon")
             self._advance_fine(self.nsub * (k - cur))
             a = self._cg_now()
-            self._advance_fine(self.nsub)
+            self._advance_fine(self.nsub * self.horizon)
             b = self._cg_now()
-            cur = k + 1
+            cur = k + self.horizon
             yield k, a, b, ghs_c
 
     def targets(self, a, b, ghs_c):
         g = self.coarse
         g.upload(a[0], ghs_c, u=a[1])
-        g.step(1)
+        g.step(self.horizon)
         p = g.download()
         g.upload(b[0], ghs_c, u=b[1])
         iap_b = g.download()["iap_u"]
         gde = 0.5 * (a[0][self.c1] + a[0][self.c2])
-        t_u = (iap_b - p["iap_u"]) / (self.dt * np.sqrt(gde))
-        t_gd = (b[0] - p["gd"]) / self.dt
+        T = self.horizon * self.dt
+        t_u = (iap_b - p["iap_u"]) / (T * np.sqrt(gde))
+        t_gd = (b[0] - p["gd"]) / T
         return t_u, t_gd
 
     # ------------------------------------------------------------ training

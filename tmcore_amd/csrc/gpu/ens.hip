@@ -2,13 +2,13 @@
 // advanced by one persistent kernel per run.
 //
 // Layout: every state and intermediate array of a level group is
-// [element][member] (member fastest).  A workgroup takes a chunk of elements
-// and spreads its 256 lanes over (element, member) pairs with the member
-// fastest, so the lanes that share an element read its constants (indices,
-// TRiSK and fit weights, geometry) once and the neighbour gathers of a
-// stencil slot are contiguous rows of M values.  The arithmetic per element
-// is the single-model code (edge_prep_e, cell_e, vertex_e, edge_pv_e,
-// edge_tend_e) with the Member layout.
+// [element][member] (member fastest).  A workgroup takes a chunk of elements;
+// each thread carries W consecutive members of one element as one vector
+// value (W = 4 for FP32 and 2 for FP64 when M allows), so an element's
+// constants (indices, TRiSK and fit weights, geometry) are loaded once per W
+// members and each neighbour gather is one W-wide load of a contiguous row.
+// The arithmetic per element is the single-model code (edge_prep_e, cell_e,
+// vertex_e, edge_pv_e, edge_tend_e) with the Members<W> layout.
 //
 // Schedule: the run is T = max(nsteps) ticks.  Level l takes a step in tick t
 // when floor((t+1) S_l / T) > floor(t S_l / T), so its S_l steps are spread
@@ -152,52 +152,58 @@ __device__ __forceinline__ int n_elems(const EnsLevelDev<R> &L, int kind) {
   return kind == kEdge ? L.m.nE : (kind == kCell ? L.m.nC : L.m.nV);
 }
 
-// Folds the per-lane partials of the members of one chunk and stores them at
-// part[id][member][chunk].
-template <int N>
-__device__ __forceinline__ void chunk_sums(const double (&v)[N], const int (&ids)[N], double2 *part, int M,
-                                           int stride, int chunk, int lanes) {
-  __shared__ double sh[N][kEnsBlock];
+// Folds the per-thread partials (W members each) of one chunk and stores
+// them at part[id][member][chunk].  lds holds 2 * kEnsBlock * W doubles.
+template <int N, int W>
+__device__ __forceinline__ void chunk_sums(const val_t<double, W> (&v)[N], const int (&ids)[N], double2 *part, int M,
+                                           int stride, int chunk, int lanes, double *lds) {
+  const int G = M / W;
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < N; ++i) sh[i][threadIdx.x] = v[i];
+  for (int i = 0; i < N; ++i)
+#pragma unroll
+    for (int w = 0; w < W; ++w) lds[(i * kEnsBlock + threadIdx.x) * W + w] = vget<double, W>(v[i], w);
   __syncthreads();
   if ((int)threadIdx.x < M) {
+    const int g = threadIdx.x / W, w = threadIdx.x % W;
 #pragma unroll
     for (int i = 0; i < N; ++i) {
       double2 a = make_double2(0.0, 0.0);
-      for (int l = 0; l < lanes; ++l) a = dd_add(a, make_double2(sh[i][l * M + threadIdx.x], 0.0));
+      for (int l = 0; l < lanes; ++l) a = dd_add(a, make_double2(lds[(i * kEnsBlock + l * G + g) * W + w], 0.0));
       part[((size_t)ids[i] * M + threadIdx.x) * stride + chunk] = a;
     }
   }
 }
 
-// One chunk of one phase of level L.
-template <typename R>
-__device__ void ens_chunk(const EnsLevelDev<R> &L, int phase, int k, bool inv, int ch, int chunk) {
+// One chunk of one phase of level L, W members per thread.
+template <typename R, int W>
+__device__ void ens_chunk(const EnsLevelDev<R> &L, int phase, int k, bool inv, int ch, int chunk, double *lds) {
+  typedef val_t<double, W> D;
+  typedef Members<W> Lay;
   const int M = L.M;
-  const int lanes = kEnsBlock / M;
-  const int lane = threadIdx.x / M;
-  const Member at{M, (int)(threadIdx.x % M)};
+  const int G = M / W;  // threads per element
+  const int lanes = kEnsBlock / G;
+  const int lane = threadIdx.x / G;
+  const Lay at{M, (int)(threadIdx.x % G) * W};
   const bool live = lane < lanes;
   const int kind = phase == 1 ? kCell : (phase == 4 ? kVert : kEdge);
   const int n = phase == 5 ? L.m.nE + L.m.nC : n_elems(L, kind);
   const int e0 = ch * chunk, e1 = min(e0 + chunk, n);
   const StageBufs<R> b = ens_bufs(L, k, inv);
   if (phase == 0) {  // edge prep
-    double en = 0.0;
+    D en = D(0.0);
     if (live)
-      for (int e = e0 + lane; e < e1; e += lanes) en += edge_prep_e<R>(L.m, b, at, e);
+      for (int e = e0 + lane; e < e1; e += lanes) en += edge_prep_e<R, Lay>(L.m, b, at, e);
     if (inv) {
-      const double v[1] = {en};
+      const D v[1] = {en};
       const int ids[1] = {kEnergyE};
-      chunk_sums<1>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<1, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     }
   } else if (phase == 1) {  // cells
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    D s[4] = {D(0.0), D(0.0), D(0.0), D(0.0)};
     if (live)
       for (int c = e0 + lane; c < e1; c += lanes) {
-        const CellSums r = cell_e<R>(L.m, b, at, c);
+        const CellSums<D> r = cell_e<R, Lay>(L.m, b, at, c);
         s[0] += r.mass;
         s[1] += r.enc;
         s[2] += r.rp;
@@ -205,87 +211,94 @@ __device__ void ens_chunk(const EnsLevelDev<R> &L, int phase, int k, bool inv, i
       }
     const int rid = k == 2 ? kR12c : (k == 3 ? kR23c : kR34c);
     if (inv) {
-      const double v[2] = {s[0], s[1]};
+      const D v[2] = {s[0], s[1]};
       const int ids[2] = {kMass, kEnergyC};
-      chunk_sums<2>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<2, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     } else if (k == 4) {
-      const double v[2] = {s[2], s[3]};
+      const D v[2] = {s[2], s[3]};
       const int ids[2] = {rid, kR44c};
-      chunk_sums<2>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<2, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     } else if (k >= 2) {
-      const double v[1] = {s[2]};
+      const D v[1] = {s[2]};
       const int ids[1] = {rid};
-      chunk_sums<1>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<1, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     }
   } else if (phase == 4) {  // vertices
-    double s[2] = {0.0, 0.0};
+    D s[2] = {D(0.0), D(0.0)};
     if (live)
       for (int v = e0 + lane; v < e1; v += lanes) {
-        const VertSums r = vertex_e<R>(L.m, b, at, v);
+        const VertSums<D> r = vertex_e<R, Lay>(L.m, b, at, v);
         s[0] += r.ens;
         s[1] += r.av;
       }
     if (inv) {
-      const double v[2] = {s[0], s[1]};
+      const D v[2] = {s[0], s[1]};
       const int ids[2] = {kEnstrophy, kAbsVor};
-      chunk_sums<2>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<2, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     }
   } else if (phase == 2) {  // PV on edges
     if (live)
-      for (int e = e0 + lane; e < e1; e += lanes) edge_pv_e<R>(L.m, b, at, e);
+      for (int e = e0 + lane; e < e1; e += lanes) edge_pv_e<R, Lay>(L.m, b, at, e);
   } else if (phase == 3) {  // PV flux, tendency, RK update
-    double s[2] = {0.0, 0.0};
+    D s[2] = {D(0.0), D(0.0)};
     if (live)
       for (int e = e0 + lane; e < e1; e += lanes) {
-        const EdgeSums r = edge_tend_e<R>(L.m, b, at, e);
+        const EdgeSums<D> r = edge_tend_e<R, Lay>(L.m, b, at, e);
         s[0] += r.rp;
         s[1] += r.r44;
       }
     const int rid = k == 2 ? kR12e : (k == 3 ? kR23e : kR34e);
     if (k == 4) {
-      const double v[2] = {s[0], s[1]};
+      const D v[2] = {s[0], s[1]};
       const int ids[2] = {rid, kR44e};
-      chunk_sums<2>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<2, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     } else if (k >= 2) {
-      const double v[1] = {s[0]};
+      const D v[1] = {s[0]};
       const int ids[1] = {rid};
-      chunk_sums<1>(v, ids, L.part, M, L.part_stride, ch, lanes);
+      chunk_sums<1, W>(v, ids, L.part, M, L.part_stride, ch, lanes, lds);
     }
   } else if (phase == 5) {  // finalize: update_state(beta*dt, ...) per member
     if (live) {
-      const int mem = threadIdx.x % M;
-      const double *r = L.sums + (size_t)mem * kNumSums;
-      double beta = 1.0;
-      if (L.conserve_energy) {
-        const double R44 = (r[kR44e] + r[kR44c]) * L.inv_total_area;
-        const double R12 = (r[kR12e] + r[kR12c]) * L.inv_total_area;
-        const double R23 = (r[kR23e] + r[kR23c]) * L.inv_total_area;
-        const double R34 = (r[kR34e] + r[kR34c]) * L.inv_total_area;
-        beta = (R12 + R23 + R34) / (3.0 * R44);
+      typedef val_t<R, W> V;
+      double beta[W];
+      V bdt;
+#pragma unroll
+      for (int w = 0; w < W; ++w) {
+        const double *r = L.sums + (size_t)(at.m0 + w) * kNumSums;
+        beta[w] = 1.0;
+        if (L.conserve_energy) {
+          const double R44 = (r[kR44e] + r[kR44c]) * L.inv_total_area;
+          const double R12 = (r[kR12e] + r[kR12c]) * L.inv_total_area;
+          const double R23 = (r[kR23e] + r[kR23c]) * L.inv_total_area;
+          const double R34 = (r[kR34e] + r[kR34c]) * L.inv_total_area;
+          beta[w] = (R12 + R23 + R34) / (3.0 * R44);
+        }
+        vset<R, W>(bdt, w, (R)(beta[w] * L.dt));
       }
-      const R bdt = (R)(beta * L.dt);
       // edges first, then cells (chunk covers [e0,e1) of a combined range)
       for (int i = e0 + lane; i < e1; i += lanes) {
         if (i < L.m.nE) {
-          const size_t o = at(i);
-          L.iap_old[o] = L.iap_old[o] + bdt * L.acc_iap[o];
+          at.st(L.iap_old, i, at.ld(L.iap_old, i) + bdt * at.ld(L.acc_iap, i));
         } else {
           const int c = i - L.m.nE;
-          const size_t o = at(c);
-          L.gd_old[o] = L.gd_old[o] + bdt * L.acc_gd[o];
+          at.st(L.gd_old, c, at.ld(L.gd_old, c) + bdt * at.ld(L.acc_gd, c));
           for (int t = 0; t < L.nT; ++t) {
-            const size_t oq = at((size_t)t * L.m.nC + c);
-            L.q_old[oq] = L.q_old[oq] + (R)L.dt * L.acc_q[oq];
+            const size_t q = (size_t)t * L.m.nC + c;
+            at.st(L.q_old, q, at.ld(L.q_old, q) + (R)L.dt * at.ld(L.acc_q, q));
           }
         }
       }
-      if (ch == 0 && lane == 0) {
-        L.beta[mem] = beta;
-        L.inv[(size_t)mem * 4 + 1] = r[kEnergyE] + r[kEnergyC];
-      }
+      if (ch == 0 && lane == 0)
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+          const double *r = L.sums + (size_t)(at.m0 + w) * kNumSums;
+          L.beta[at.m0 + w] = beta[w];
+          L.inv[(size_t)(at.m0 + w) * 4 + 1] = r[kEnergyE] + r[kEnergyC];
+        }
     }
   }
 }
+
 
 // Number of chunks of a phase for one level.
 template <typename R>
@@ -379,35 +392,36 @@ __device__ void ens_energy_out(const EnsLevelDev<R> *lv, const EnsCtl &c) {
     }
 }
 
-template <typename R>
+template <typename R, int W>
 __global__ __launch_bounds__(kEnsBlock) void k_ens_run(const EnsLevelDev<R> *__restrict__ lv, EnsCtl c) {
+  __shared__ double lds[2 * kEnsBlock * W];
   bool act[kMaxLevels];
   for (int t = 0; t < c.T; ++t) {
     for (int l = 0; l < c.nlev; ++l) act[l] = level_active(lv[l].nsteps, c.T, t);
     for (int k = 1; k <= 4; ++k) {
       // phases: 0 edge prep, 1 cells (+ 4 vertices in the same pass), 2 PV, 3 tendency
-      for_chunks<R>(lv, c, act, 0, [&](int l, int ch) { ens_chunk<R>(lv[l], 0, k, k == 1, ch, c.chunk); });
+      for_chunks<R>(lv, c, act, 0, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 0, k, k == 1, ch, c.chunk, lds); });
       if (!grid_sync(c)) return;
-      for_chunks<R>(lv, c, act, 1, [&](int l, int ch) { ens_chunk<R>(lv[l], 1, k, k == 1, ch, c.chunk); });
-      for_chunks<R>(lv, c, act, 4, [&](int l, int ch) { ens_chunk<R>(lv[l], 4, k, k == 1, ch, c.chunk); });
+      for_chunks<R>(lv, c, act, 1, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 1, k, k == 1, ch, c.chunk, lds); });
+      for_chunks<R>(lv, c, act, 4, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 4, k, k == 1, ch, c.chunk, lds); });
       if (!grid_sync(c)) return;
-      for_chunks<R>(lv, c, act, 2, [&](int l, int ch) { ens_chunk<R>(lv[l], 2, k, k == 1, ch, c.chunk); });
+      for_chunks<R>(lv, c, act, 2, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 2, k, k == 1, ch, c.chunk, lds); });
       if (!grid_sync(c)) return;
-      for_chunks<R>(lv, c, act, 3, [&](int l, int ch) { ens_chunk<R>(lv[l], 3, k, k == 1, ch, c.chunk); });
+      for_chunks<R>(lv, c, act, 3, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 3, k, k == 1, ch, c.chunk, lds); });
       if (!grid_sync(c)) return;
     }
     ens_reduce<R>(lv, c, act, false);
     if (!grid_sync(c)) return;
-    for_chunks<R>(lv, c, act, 5, [&](int l, int ch) { ens_chunk<R>(lv[l], 5, 0, false, ch, c.chunk); });
+    for_chunks<R>(lv, c, act, 5, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 5, 0, false, ch, c.chunk, lds); });
     if (!grid_sync(c)) return;
   }
   // refresh: invariants of the final state of every member
   bool all[kMaxLevels];
   for (int l = 0; l < c.nlev; ++l) all[l] = true;
-  for_chunks<R>(lv, c, all, 0, [&](int l, int ch) { ens_chunk<R>(lv[l], 0, 0, true, ch, c.chunk); });
+  for_chunks<R>(lv, c, all, 0, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 0, 0, true, ch, c.chunk, lds); });
   if (!grid_sync(c)) return;
-  for_chunks<R>(lv, c, all, 1, [&](int l, int ch) { ens_chunk<R>(lv[l], 1, 0, true, ch, c.chunk); });
-  for_chunks<R>(lv, c, all, 4, [&](int l, int ch) { ens_chunk<R>(lv[l], 4, 0, true, ch, c.chunk); });
+  for_chunks<R>(lv, c, all, 1, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 1, 0, true, ch, c.chunk, lds); });
+  for_chunks<R>(lv, c, all, 4, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 4, 0, true, ch, c.chunk, lds); });
   if (!grid_sync(c)) return;
   ens_reduce<R>(lv, c, all, true);
   if (!grid_sync(c)) return;
@@ -472,7 +486,9 @@ struct tm_ens {
   std::vector<void *> backup;  // autotune copies of gd_old, iap_old, q_old per level
   unsigned *bar = nullptr;     // [count, gen]
   int *abort_flag = nullptr;
-  int chunk = 256, bpc = 2, max_bpc = 1, cus = 1;
+  int chunk = 256, bpc = 2, cus = 1;
+  int W = 1;                 // members per thread, divides every level's M
+  int max_bpc[3] = {1, 1, 1};  // occupancy of the kernel for W = 1, 2, 4
   unsigned long long wait_limit = 0;
 
   int fail(int code, const std::string &m) {
@@ -633,7 +649,13 @@ int ens_launch(tm_ens *E, const int *nsteps, double *ms) {
   hipGraph_t g = nullptr;
   hipGraphExec_t ge = nullptr;
   ENSCHK(E, hipStreamBeginCapture(E->stream, hipStreamCaptureModeThreadLocal));
-  hipLaunchKernelGGL(k_ens_run<R>, dim3(G), dim3(kEnsBlock), 0, E->stream, (const EnsLevelDev<R> *)E->d_lv, c);
+  const EnsLevelDev<R> *d_lv = (const EnsLevelDev<R> *)E->d_lv;
+  if (E->W == 4)
+    hipLaunchKernelGGL((k_ens_run<R, 4>), dim3(G), dim3(kEnsBlock), 0, E->stream, d_lv, c);
+  else if (E->W == 2)
+    hipLaunchKernelGGL((k_ens_run<R, 2>), dim3(G), dim3(kEnsBlock), 0, E->stream, d_lv, c);
+  else
+    hipLaunchKernelGGL((k_ens_run<R, 1>), dim3(G), dim3(kEnsBlock), 0, E->stream, d_lv, c);
   hipError_t ce = hipStreamEndCapture(E->stream, &g);
   if (ce != hipSuccess) return E->fail(TM_EDEVICE, std::string("capture: ") + hipGetErrorString(ce));
   ce = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
@@ -671,10 +693,21 @@ int ens_create_levels(tm_ens *E) {
     if (rc) return rc;
   }
   if (!(E->d_lv = E->alloc(sizeof(EnsLevelDev<R>) * E->nlev))) return E->fail(TM_ENOMEM, "hipMalloc levels");
-  int occ = 0;
-  ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_ens_run<R>, kEnsBlock, 0));
-  E->max_bpc = std::max(1, occ);
-  E->bpc = std::min(E->bpc, E->max_bpc);
+  int occ[3] = {0, 0, 0};
+  ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], k_ens_run<R, 1>, kEnsBlock, 0));
+  ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], k_ens_run<R, 2>, kEnsBlock, 0));
+  ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], k_ens_run<R, 4>, kEnsBlock, 0));
+  for (int i = 0; i < 3; ++i) E->max_bpc[i] = std::max(1, occ[i]);
+  // widest vector of members that divides every level's member count:
+  // 16-byte rows (4 x FP32, 2 x FP64)
+  E->W = 1;
+  for (int w : {2, 4}) {
+    if (w * sizeof(R) > 16) continue;
+    bool ok = true;
+    for (int l = 0; l < E->nlev; ++l) ok = ok && (E->M[l] % w == 0);
+    if (ok) E->W = w;
+  }
+  E->bpc = std::min(E->bpc, E->max_bpc[E->W == 4 ? 2 : E->W - 1]);
   return TM_OK;
 }
 
@@ -781,12 +814,27 @@ extern "C" int tm_ens_invariants(tm_ens *E, int l, int m, double out[4]) {
   return TM_OK;
 }
 
-extern "C" int tm_ens_set_tuning(tm_ens *E, int chunk, int bpc) {
+namespace {
+int wslot(int w) { return w == 4 ? 2 : w - 1; }
+bool w_valid(const tm_ens *E, int w) {
+  if (w != 1 && w != 2 && w != 4) return false;
+  if ((size_t)w * E->rsz > 16) return false;
+  for (int l = 0; l < E->nlev; ++l)
+    if (E->M[l] % w) return false;
+  return true;
+}
+}  // namespace
+
+extern "C" int tm_ens_set_tuning(tm_ens *E, int chunk, int bpc, int members_per_thread) {
   if (!E) return TM_EINVAL;
   if (chunk < 32 || chunk > 4096 || bpc < 1) return E->fail(TM_EINVAL, "chunk must be 32..4096, blocks_per_cu >= 1");
-  if (bpc > E->max_bpc) return E->fail(TM_EINVAL, "blocks_per_cu exceeds the occupancy of the persistent kernel");
+  if (!w_valid(E, members_per_thread))
+    return E->fail(TM_EINVAL, "members_per_thread must be 1, 2 or 4, divide every level's members and span <= 16 bytes");
+  if (bpc > E->max_bpc[wslot(members_per_thread)])
+    return E->fail(TM_EINVAL, "blocks_per_cu exceeds the occupancy of the persistent kernel");
   E->chunk = chunk;
   E->bpc = bpc;
+  E->W = members_per_thread;
   return TM_OK;
 }
 
@@ -819,34 +867,41 @@ int ens_state_copy(tm_ens *E, bool save) {
 }
 }  // namespace
 
-extern "C" int tm_ens_autotune(tm_ens *E, int ticks, int best[2], double *best_ms) {
+extern "C" int tm_ens_autotune(tm_ens *E, int ticks, int best[3], double *best_ms) {
   if (!E || ticks < 1) return TM_EINVAL;
   int rc = E->rsz == 8 ? ens_state_copy<double>(E, true) : ens_state_copy<float>(E, true);
   if (rc) return rc;
   std::vector<int> steps(E->nlev, ticks);
   double bms = 1e300;
-  int bc = E->chunk, bb = E->bpc;
-  for (int chunk : {64, 128, 256, 512, 1024})
-    for (int bpc : {1, 2, 4, 8}) {
-      if (bpc > E->max_bpc) continue;
-      E->chunk = chunk;
-      E->bpc = bpc;
-      double ms = 0.0;
-      rc = tm_ens_run(E, steps.data(), &ms);
-      int rc2 = E->rsz == 8 ? ens_state_copy<double>(E, false) : ens_state_copy<float>(E, false);
-      if (rc) return rc;
-      if (rc2) return rc2;
-      if (ms < bms) {
-        bms = ms;
-        bc = chunk;
-        bb = bpc;
+  int bc = E->chunk, bb = E->bpc, bw = E->W;
+  for (int w : {1, 2, 4}) {
+    if (!w_valid(E, w)) continue;
+    for (int chunk : {64, 128, 256, 512, 1024})
+      for (int bpc : {1, 2, 4, 8}) {
+        if (bpc > E->max_bpc[wslot(w)]) continue;
+        E->chunk = chunk;
+        E->bpc = bpc;
+        E->W = w;
+        double ms = 0.0;
+        rc = tm_ens_run(E, steps.data(), &ms);
+        int rc2 = E->rsz == 8 ? ens_state_copy<double>(E, false) : ens_state_copy<float>(E, false);
+        if (rc) return rc;
+        if (rc2) return rc2;
+        if (ms < bms) {
+          bms = ms;
+          bc = chunk;
+          bb = bpc;
+          bw = w;
+        }
       }
-    }
+  }
   E->chunk = bc;
   E->bpc = bb;
+  E->W = bw;
   if (best) {
     best[0] = bc;
     best[1] = bb;
+    best[2] = bw;
   }
   if (best_ms) *best_ms = bms / ticks;
   return TM_OK;

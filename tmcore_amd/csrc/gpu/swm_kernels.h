@@ -94,57 +94,149 @@ __device__ __forceinline__ void block_sums(const double (&v)[N], const int (&ids
 }
 
 // ----------------------------------------------------------- element code
-// Layouts: where element i of a state or intermediate array lives.  Solo is
-// one model; Member puts the M ensemble members of element i side by side
-// (member fastest), so the lanes of a wavefront that share an element read
-// its mesh constants once and gather neighbour rows contiguously.
+// Values: one model's element is a scalar R.  Ensemble members of an element
+// are stored side by side (member fastest) and a thread may carry W of them
+// as one vector value (val_t<R, W>), so the element's mesh constants and
+// stencil indices are loaded once for W members and every state gather is
+// one W-wide load.  With W = 1 the value type is R itself and the code below
+// is the plain scalar code.
+template <typename R, int W>
+struct val_s {
+  typedef R type __attribute__((ext_vector_type(W)));
+};
+template <typename R>
+struct val_s<R, 1> {
+  typedef R type;
+};
+template <typename R, int W>
+using val_t = typename val_s<R, W>::type;
+
+template <typename T, int W>
+__device__ __forceinline__ val_t<T, W> tm_sqrt(val_t<T, W> x) {
+  if constexpr (W == 1) {
+    return tm_sqrt(x);
+  } else {
+    val_t<T, W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r[w] = tm_sqrt(T(x[w]));
+    return r;
+  }
+}
+// Component w of a value (the value itself when W = 1).
+template <typename T, int W>
+__device__ __forceinline__ T vget(const val_t<T, W> &v, int w) {
+  if constexpr (W == 1) {
+    (void)w;
+    return v;
+  } else {
+    return v[w];
+  }
+}
+template <typename T, int W>
+__device__ __forceinline__ void vset(val_t<T, W> &v, int w, T x) {
+  if constexpr (W == 1) {
+    (void)w;
+    v = x;
+  } else {
+    v[w] = x;
+  }
+}
+template <typename R, int W>
+__device__ __forceinline__ val_t<R, W> sign1v(val_t<R, W> x) {
+  if constexpr (W == 1) {
+    return sign1(x);
+  } else {
+    val_t<R, W> r;
+#pragma unroll
+    for (int w = 0; w < W; ++w) r[w] = sign1(R(x[w]));
+    return r;
+  }
+}
+template <typename R, int W>
+__device__ __forceinline__ val_t<double, W> to_d(val_t<R, W> x) {
+  if constexpr (W == 1)
+    return (double)x;
+  else
+    return __builtin_convertvector(x, val_t<double, W>);
+}
+
+// Layouts: where element i of a state or intermediate array lives, and how
+// a thread's W values of it are loaded and stored.  Solo is one model;
+// Members<W> holds the M members of element i at [i*M, i*M + M) and a thread
+// takes members [m0, m0 + W).
 struct Solo {
+  static constexpr int W = 1;
   __device__ __forceinline__ size_t operator()(size_t i) const { return i; }
+  template <typename R>
+  __device__ __forceinline__ R ld(const R *p, size_t i) const {
+    return p[i];
+  }
+  template <typename R>
+  __device__ __forceinline__ void st(R *p, size_t i, R x) const {
+    p[i] = x;
+  }
 };
-struct Member {
-  int M, m;
-  __device__ __forceinline__ size_t operator()(size_t i) const { return i * (size_t)M + (size_t)m; }
+template <int NW>
+struct Members {
+  static constexpr int W = NW;
+  int M, m0;
+  __device__ __forceinline__ size_t operator()(size_t i) const { return i * (size_t)M + (size_t)m0; }
+  template <typename R>
+  __device__ __forceinline__ val_t<R, NW> ld(const R *p, size_t i) const {
+    return *reinterpret_cast<const val_t<R, NW> *>(p + (*this)(i));
+  }
+  template <typename R>
+  __device__ __forceinline__ void st(R *p, size_t i, val_t<R, NW> x) const {
+    *reinterpret_cast<val_t<R, NW> *>(p + (*this)(i)) = x;
+  }
 };
+using Member = Members<1>;
 
 // gd_edge, u and the per-edge products the cell/vertex sums gather; returns
 // the edge part of the energy invariant when b.want_inv.
 template <typename R, typename L>
-__device__ __forceinline__ double edge_prep_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+__device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+  typedef val_t<R, L::W> V;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const R gde = R(0.5) * (b.gd[at(c1)] + b.gd[at(c2)]);
-  const R iap = b.iap[at(e)];
-  const R u = iap / tm_sqrt(gde);
+  const V gde = R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2));
+  const V iap = at.ld(b.iap, e);
+  const V u = iap / tm_sqrt<R, L::W>(gde);
   const R dv = (R)m.dv[e];
-  b.gd_e[at(e)] = gde;
-  b.u[at(e)] = u;
-  b.Fdv[at(e)] = (u * gde) * dv;
-  b.kee[at(e)] = (u * u) * (R)m.aE[e];
-  b.udc[at(e)] = u * (R)m.dc[e];
-  return b.want_inv ? ((double)iap * (double)iap) * m.aE[e] : 0.0;
+  at.st(b.gd_e, e, gde);
+  at.st(b.u, e, u);
+  at.st(b.Fdv, e, (u * gde) * dv);
+  at.st(b.kee, e, (u * u) * (R)m.aE[e]);
+  at.st(b.udc, e, u * (R)m.dc[e]);
+  const val_t<double, L::W> id = to_d<R, L::W>(iap);
+  return b.want_inv ? (id * id) * m.aE[e] : val_t<double, L::W>(0.0);
 }
 
 // CRK4 bookkeeping of one prognostic value: next stage state, accumulator
 // t1 + 2 t2 + 2 t3 (+ t4)/6, and the previous/current tendency for beta.
-template <typename R>
-__device__ __forceinline__ R rk_accumulate(const StageBufs<R> &b, R *acc_arr, size_t o, R t) {
-  R acc;
+template <typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> rk_accumulate(const StageBufs<R> &b, R *acc_arr, L at, size_t i,
+                                                        val_t<R, L::W> t) {
+  val_t<R, L::W> acc;
   if (b.k == 1)
     acc = t;
   else if (b.k < 4)
-    acc = acc_arr[o] + R(2) * t;
+    acc = at.ld(acc_arr, i) + R(2) * t;
   else
-    acc = (acc_arr[o] + t) / R(6);
-  acc_arr[o] = acc;
+    acc = (at.ld(acc_arr, i) + t) / R(6);
+  at.st(acc_arr, i, acc);
   return acc;
 }
 
+template <typename D>
 struct CellSums {
-  double mass, enc, rp, r44;
+  D mass, enc, rp, r44;
 };
 
 template <typename R, typename L>
-__device__ __forceinline__ CellSums cell_e(const DMesh &m, const StageBufs<R> &b, L at, int c) {
-  CellSums r{0.0, 0.0, 0.0, 0.0};
+__device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, const StageBufs<R> &b, L at, int c) {
+  typedef val_t<R, L::W> V;
+  typedef val_t<double, L::W> D;
+  CellSums<D> r{D(0.0), D(0.0), D(0.0), D(0.0)};
   // Unused slots of pentagons hold index 0 (a valid edge): every gather is
   // issued unconditionally so they overlap, and the unused terms are
   // replaced by an exact +0 afterwards.
@@ -152,104 +244,109 @@ __device__ __forceinline__ CellSums cell_e(const DMesh &m, const StageBufs<R> &b
   int idx[kMaxEoC];
 #pragma unroll
   for (int s = 0; s < kMaxEoC; ++s) idx[s] = m.eocS[s * m.nC + c];
-  R fv[kMaxEoC], kv[kMaxEoC];
+  V fv[kMaxEoC], kv[kMaxEoC];
 #pragma unroll
   for (int s = 0; s < kMaxEoC; ++s) {
     const int e = unsign(idx[s]);
-    fv[s] = b.Fdv[at(e)];
-    kv[s] = b.kee[at(e)];
+    fv[s] = at.ld(b.Fdv, e);
+    kv[s] = at.ld(b.kee, e);
   }
-  R sdiv = R(0), ske = R(0);
+  V sdiv = V(R(0)), ske = V(R(0));
 #pragma unroll
   for (int s = 0; s < kMaxEoC; ++s) {
-    const R f = idx[s] < 0 ? -fv[s] : fv[s];
-    sdiv += s < n ? f : R(0);
-    ske += s < n ? kv[s] : R(0);
+    const V f = idx[s] < 0 ? -fv[s] : fv[s];
+    sdiv += s < n ? f : V(R(0));
+    ske += s < n ? kv[s] : V(R(0));
   }
   const double A = m.areaCell[c];
-  const R tend = (-sdiv) / (R)A;
-  const R ke = (ske * R(0.25)) / (R)A;
-  const R gd = b.gd[at(c)];
-  const R ghs = b.ghs[at(c)];
-  b.E[at(c)] = (ke + gd) + ghs;
-  b.tend_gd[at(c)] = tend;
-  if (b.d_ke) b.d_ke[at(c)] = ke;
+  const V tend = (-sdiv) / (R)A;
+  const V ke = (ske * R(0.25)) / (R)A;
+  const V gd = at.ld(b.gd, c);
+  const V ghs = at.ld(b.ghs, c);
+  at.st(b.E, c, (ke + gd) + ghs);
+  at.st(b.tend_gd, c, tend);
+  if (b.d_ke) at.st(b.d_ke, c, ke);
   if (b.want_inv) {
-    const double g = gd;
+    const D g = to_d<R, L::W>(gd);
     r.mass = g * A;
-    r.enc = ((g * g) + (2.0 * g) * (double)ghs) * A;
+    r.enc = ((g * g) + (2.0 * g) * to_d<R, L::W>(ghs)) * A;
   }
   if (b.k >= 1) {
-    if (b.k < 4) b.gd_next[at(c)] = b.gd_base[at(c)] + (R)b.cdt * tend;
-    const R acc = rk_accumulate(b, b.acc_gd, at(c), tend);
-    if (b.k >= 2) r.rp = ((double)b.tprev_gd[at(c)] * (double)tend) * A;
-    if (b.k == 4) r.r44 = ((double)acc * (double)acc) * A;
-    if (b.k < 4) b.tcur_gd[at(c)] = tend;
+    if (b.k < 4) at.st(b.gd_next, c, at.ld(b.gd_base, c) + (R)b.cdt * tend);
+    const V acc = rk_accumulate<R, L>(b, b.acc_gd, at, c, tend);
+    if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_gd, c)) * to_d<R, L::W>(tend)) * A;
+    if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * A;
+    if (b.k < 4) at.st(b.tcur_gd, c, tend);
   }
   if (b.nT > 0) {
     int nb[kMaxEoC];
-    R uv[kMaxEoC], dvv[kMaxEoC];
+    V uv[kMaxEoC];
+    R dvv[kMaxEoC];
 #pragma unroll
     for (int s = 0; s < kMaxEoC; ++s) {
       const int e = unsign(idx[s]);
       nb[s] = m.cocN[s * m.nC + c];
-      uv[s] = b.u[at(e)];
+      uv[s] = at.ld(b.u, e);
       dvv[s] = (R)m.dv[e];
     }
     for (int t = 0; t < b.nT; ++t) {
       // adv_scheme_sg11_run: f_edge = c2e(q), flux = u*f_edge, div_operator
       const size_t tb = (size_t)t * m.nC;
-      const R q0 = b.q[at(tb + c)];
-      R qn[kMaxEoC];
+      const V q0 = at.ld(b.q, tb + c);
+      V qn[kMaxEoC];
 #pragma unroll
-      for (int s = 0; s < kMaxEoC; ++s) qn[s] = b.q[at(tb + nb[s])];
-      R sq = R(0);
+      for (int s = 0; s < kMaxEoC; ++s) qn[s] = at.ld(b.q, tb + nb[s]);
+      V sq = V(R(0));
 #pragma unroll
       for (int s = 0; s < kMaxEoC; ++s) {
-        const R qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
-        const R fl = (uv[s] * qe) * dvv[s];
-        const R f = idx[s] < 0 ? -fl : fl;
-        sq += s < n ? f : R(0);
+        const V qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
+        const V fl = (uv[s] * qe) * dvv[s];
+        const V f = idx[s] < 0 ? -fl : fl;
+        sq += s < n ? f : V(R(0));
       }
-      const R tq = (-sq) / (R)A;
-      const size_t o = at(tb + c);
-      if (b.d_tq) b.d_tq[o] = tq;
+      const V tq = (-sq) / (R)A;
+      if (b.d_tq) at.st(b.d_tq, tb + c, tq);
       if (b.k >= 1) {
-        if (b.k < 4) b.q_next[o] = b.q_base[o] + (R)b.cdt * tq;
-        (void)rk_accumulate(b, b.acc_q, o, tq);
+        if (b.k < 4) at.st(b.q_next, tb + c, at.ld(b.q_base, tb + c) + (R)b.cdt * tq);
+        (void)rk_accumulate<R, L>(b, b.acc_q, at, tb + c, tq);
       }
     }
   }
   return r;
 }
 
+template <typename D>
 struct VertSums {
-  double ens, av;
+  D ens, av;
 };
 
 template <typename R, typename L>
-__device__ __forceinline__ VertSums vertex_e(const DMesh &m, const StageBufs<R> &b, L at, int v) {
-  R s = R(0);
+__device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m, const StageBufs<R> &b, L at,
+                                                                  int v) {
+  typedef val_t<R, L::W> V;
+  typedef val_t<double, L::W> D;
+  V s = V(R(0));
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const int idx = m.eovS[j * m.nV + v];
-    const R f = b.udc[at(unsign(idx))];
+    const V f = at.ld(b.udc, unsign(idx));
     s += idx < 0 ? -f : f;
   }
   const double At = m.areaTri[v];
-  const R vor = (-s) / (R)At;
-  R g = R(0);
+  const V vor = (-s) / (R)At;
+  V g = V(R(0));
 #pragma unroll
-  for (int j = 0; j < 3; ++j) g = g + b.gd[at(m.cov[j * m.nV + v])] * (R)m.kite[j * m.nV + v];
-  const R gdv = g / (R)At;
-  const R pv = ((R)m.fV[v] + vor) / gdv;
-  b.pv_v[at(v)] = pv;
-  if (b.d_vor) b.d_vor[at(v)] = vor;
-  if (b.d_gdv) b.d_gdv[at(v)] = gdv;
-  VertSums r{0.0, 0.0};
+  for (int j = 0; j < 3; ++j) g = g + at.ld(b.gd, m.cov[j * m.nV + v]) * (R)m.kite[j * m.nV + v];
+  const V gdv = g / (R)At;
+  const V pv = ((R)m.fV[v] + vor) / gdv;
+  at.st(b.pv_v, v, pv);
+  if (b.d_vor) at.st(b.d_vor, v, vor);
+  if (b.d_gdv) at.st(b.d_gdv, v, gdv);
+  VertSums<D> r{D(0.0), D(0.0)};
   if (b.want_inv) {
-    r.ens = ((double)gdv * ((double)pv * (double)pv)) * At;
-    r.av = ((double)vor + m.fV[v]) * At;
+    const D p = to_d<R, L::W>(pv);
+    r.ens = (to_d<R, L::W>(gdv) * (p * p)) * At;
+    r.av = (to_d<R, L::W>(vor) + m.fV[v]) * At;
   }
   return r;
 }
@@ -259,20 +356,21 @@ __device__ __forceinline__ VertSums vertex_e(const DMesh &m, const StageBufs<R> 
 // a padded term adds an exact +0.
 template <typename R, typename L>
 __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
-  R v = R(0);
+  typedef val_t<R, L::W> V;
+  V v = V(R(0));
   {
     int ep[kMaxEoE];
-    R uv[kMaxEoE];
+    V uv[kMaxEoE];
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
 #pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) uv[s] = b.u[at(ep[s])];
+    for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
   }
   const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
-  const R p1 = b.pv_v[at(v1)], p2 = b.pv_v[at(v2)];
-  R pve = R(0.5) * (p1 + p2);
+  const V p1 = at.ld(b.pv_v, v1), p2 = at.ld(b.pv_v, v2);
+  V pve = R(0.5) * (p1 + p2);
   if (b.pv_order == 3 || b.pv_order == 4) {
     const R coef2 = b.pv_order == 3 ? R(1) : R(0);
     int f1[kFit2], f2[kFit2];
@@ -281,64 +379,68 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
       f1[j] = m.fit2V[j * m.nV + v1];
       f2[j] = m.fit2V[j * m.nV + v2];
     }
-    R q1[kFit2], q2[kFit2];
+    V q1[kFit2], q2[kFit2];
 #pragma unroll
     for (int j = 0; j < kFit2; ++j) {
-      q1[j] = b.pv_v[at(f1[j])];
-      q2[j] = b.pv_v[at(f2[j])];
+      q1[j] = at.ld(b.pv_v, f1[j]);
+      q2[j] = at.ld(b.pv_v, f2[j]);
     }
-    R d1 = R(0), d2 = R(0);
+    V d1 = V(R(0)), d2 = V(R(0));
 #pragma unroll
     for (int j = 0; j < kFit2; ++j) {
       d1 += (R)m.d2v[(size_t)j * m.nE + e] * (q1[j] - p1);
       d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (q2[j] - p2);
     }
     const R dv = (R)m.dv[e];
-    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1(v) * coef2) * (d2 - d1))) / R(12);
+    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
   }
-  b.pv_e[at(e)] = pve;
-  if (b.d_v) b.d_v[at(e)] = v;
+  at.st(b.pv_e, e, pve);
+  if (b.d_v) at.st(b.d_v, e, v);
 }
 
+template <typename D>
 struct EdgeSums {
-  double rp, r44;
+  D rp, r44;
 };
 
 // TRiSK PV flux, edge gradient, IAP tendency and the RK update of iap_u.
 template <typename R, typename L>
-__device__ __forceinline__ EdgeSums edge_tend_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
-  const R pve = b.pv_e[at(e)];
+__device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh &m, const StageBufs<R> &b, L at,
+                                                                     int e) {
+  typedef val_t<R, L::W> V;
+  typedef val_t<double, L::W> D;
+  const V pve = at.ld(b.pv_e, e);
   int ep[kMaxEoE];
-  R uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
+  V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) ep[q] = m.eoe[q * m.nE + e];
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
-    uv[q] = b.u[at(ep[q])];
-    gv[q] = b.gd_e[at(ep[q])];
-    pv[q] = b.pv_e[at(ep[q])];
+    uv[q] = at.ld(b.u, ep[q]);
+    gv[q] = at.ld(b.gd_e, ep[q]);
+    pv[q] = at.ld(b.pv_e, ep[q]);
   }
-  R s = R(0);
+  V s = V(R(0));
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q)
     s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const R dEdx = (b.E[at(c2)] - b.E[at(c1)]) / (R)m.dc[e];
-  const R ut = s - dEdx;
-  const R sg = tm_sqrt(b.gd_e[at(e)]);
-  const R tge = R(0.5) * (b.tend_gd[at(c1)] + b.tend_gd[at(c2)]);
-  const R it = sg * ut + ((R(0.5) * b.u[at(e)]) / sg) * tge;
-  if (b.d_pvflx) b.d_pvflx[at(e)] = s;
-  if (b.d_utend) b.d_utend[at(e)] = ut;
-  if (b.d_tiap) b.d_tiap[at(e)] = it;
-  EdgeSums r{0.0, 0.0};
+  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / (R)m.dc[e];
+  const V ut = s - dEdx;
+  const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
+  const V tge = R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2));
+  const V it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
+  if (b.d_pvflx) at.st(b.d_pvflx, e, s);
+  if (b.d_utend) at.st(b.d_utend, e, ut);
+  if (b.d_tiap) at.st(b.d_tiap, e, it);
+  EdgeSums<D> r{D(0.0), D(0.0)};
   if (b.k >= 1) {
     const double aE = m.aE[e];
-    if (b.k < 4) b.iap_next[at(e)] = b.iap_base[at(e)] + (R)b.cdt * it;
-    const R acc = rk_accumulate(b, b.acc_iap, at(e), it);
-    if (b.k >= 2) r.rp = ((double)b.tprev_iap[at(e)] * (double)it) * aE;
-    if (b.k == 4) r.r44 = ((double)acc * (double)acc) * aE;
-    if (b.k < 4) b.tcur_iap[at(e)] = it;
+    if (b.k < 4) at.st(b.iap_next, e, at.ld(b.iap_base, e) + (R)b.cdt * it);
+    const V acc = rk_accumulate<R, L>(b, b.acc_iap, at, e, it);
+    if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
+    if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * aE;
+    if (b.k < 4) at.st(b.tcur_iap, e, it);
   }
   return r;
 }
@@ -349,7 +451,7 @@ __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE);
   const int e = tile * kBlock + threadIdx.x;
   double en = 0.0;
-  if (e < m.nE) en = edge_prep_e<R>(m, b, Solo{}, e);
+  if (e < m.nE) en = edge_prep_e<R, Solo>(m, b, Solo{}, e);
   if (b.want_inv) {
     const double v[1] = {en};
     const int ids[1] = {kEnergyE};
@@ -360,8 +462,8 @@ __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
 template <typename R>
 __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b, int tile) {
   const int c = tile * kBlock + threadIdx.x;
-  CellSums r{0.0, 0.0, 0.0, 0.0};
-  if (c < m.nC) r = cell_e<R>(m, b, Solo{}, c);
+  CellSums<double> r{0.0, 0.0, 0.0, 0.0};
+  if (c < m.nC) r = cell_e<R, Solo>(m, b, Solo{}, c);
   if (b.want_inv) {
     const double v[2] = {r.mass, r.enc};
     const int ids[2] = {kMass, kEnergyC};
@@ -384,8 +486,8 @@ __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b,
 template <typename R>
 __device__ __forceinline__ void vertex_part(const DMesh &m, const StageBufs<R> &b, int tile) {
   const int v = tile * kBlock + threadIdx.x;
-  VertSums r{0.0, 0.0};
-  if (v < m.nV) r = vertex_e<R>(m, b, Solo{}, v);
+  VertSums<double> r{0.0, 0.0};
+  if (v < m.nV) r = vertex_e<R, Solo>(m, b, Solo{}, v);
   if (b.want_inv) {
     const double vv[2] = {r.ens, r.av};
     const int ids[2] = {kEnstrophy, kAbsVor};
@@ -404,15 +506,15 @@ __global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b)
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
   const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
-  if (e < m.nE) edge_pv_e<R>(m, b, Solo{}, e);
+  if (e < m.nE) edge_pv_e<R, Solo>(m, b, Solo{}, e);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE);
   const int e = tile * kBlock + threadIdx.x;
-  EdgeSums r{0.0, 0.0};
-  if (e < m.nE) r = edge_tend_e<R>(m, b, Solo{}, e);
+  EdgeSums<double> r{0.0, 0.0};
+  if (e < m.nE) r = edge_tend_e<R, Solo>(m, b, Solo{}, e);
   if (b.k >= 2) {
     const int rid = b.k == 2 ? kR12e : (b.k == 3 ? kR23e : kR34e);
     if (b.k == 4) {

@@ -110,11 +110,16 @@ struct tm_ctx {
   int hist_cap = 4096;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
+  // with a closure held over several steps: the step graph without the MLP
+  hipGraph_t graph_hold = nullptr;
+  hipGraphExec_t gexec_hold = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_state = false;
   bool timed = false;
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
+  int cl_interval = 1;  // MLP every cl_interval steps, its correction held in between
+  long long cl_step = 0;  // steps since the state or the closure changed
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
   void *cl_sq = nullptr;
   double *cl_loss = nullptr;
@@ -308,11 +313,11 @@ int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
 }
 
 template <typename R>
-int enqueue_step(tm_ctx *c) {
+int enqueue_step(tm_ctx *c, bool infer) {
   for (int k = 1; k <= 4; ++k) {
     int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
     if (rc) return rc;
-    if (k == 1 && c->cl) {
+    if (k == 1 && c->cl && infer) {
       // MLP of the start-of-step state, between stage 1 and stage 2 (which
       // overwrites u, gd_edge, pv_edge and tend_gd)
       if ((rc = enqueue_cl_features<R>(c, c->cl_X, c->cl_sq))) return rc;
@@ -347,16 +352,32 @@ int enqueue_refresh(tm_ctx *c) {
   return TM_OK;
 }
 
-int build_graph(tm_ctx *c) {
-  if (c->gexec) return TM_OK;
+int capture_step(tm_ctx *c, bool infer, hipGraph_t *graph, hipGraphExec_t *exec) {
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = c->rsz == 8 ? enqueue_step<double>(c) : enqueue_step<float>(c);
+  int rc = c->rsz == 8 ? enqueue_step<double>(c, infer) : enqueue_step<float>(c, infer);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) return rc;
   if (e != hipSuccess) return c->fail(TM_EDEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
-  c->graph = g;
-  HIPCHK(c, hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+  *graph = g;
+  HIPCHK(c, hipGraphInstantiate(exec, g, nullptr, nullptr, 0));
+  return TM_OK;
+}
+
+void drop_graphs(tm_ctx *c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->gexec_hold) (void)hipGraphExecDestroy(c->gexec_hold);
+  if (c->graph_hold) (void)hipGraphDestroy(c->graph_hold);
+  c->gexec = c->gexec_hold = nullptr;
+  c->graph = c->graph_hold = nullptr;
+}
+
+int build_graph(tm_ctx *c) {
+  int rc = TM_OK;
+  if (!c->gexec && (rc = capture_step(c, true, &c->graph, &c->gexec))) return rc;
+  if (c->cl && c->cl_interval > 1 && !c->gexec_hold && (rc = capture_step(c, false, &c->graph_hold, &c->gexec_hold)))
+    return rc;
   return TM_OK;
 }
 
@@ -365,7 +386,12 @@ int run_steps(tm_ctx *c, int nsteps, bool refresh) {
   if (rc) return rc;
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  for (int s = 0; s < nsteps; ++s) HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  for (int s = 0; s < nsteps; ++s) {
+    // a held closure step replays the last MLP outputs (cl_Y, cl_sq)
+    const bool hold = c->cl && c->cl_interval > 1 && (c->cl_step % c->cl_interval) != 0;
+    HIPCHK(c, hipGraphLaunch(hold ? c->gexec_hold : c->gexec, c->stream));
+    ++c->cl_step;
+  }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   c->timed = true;
   if (refresh) {
@@ -406,8 +432,7 @@ extern "C" const char *tm_last_error(const tm_ctx *ctx) { return ctx ? ctx->err.
 
 extern "C" int tm_ctx_destroy(tm_ctx *c) {
   if (!c) return TM_EINVAL;
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  drop_graphs(c);
   for (auto &smp : c->ds) {
     (void)hipFree(smp.X);
     (void)hipFree(smp.tu);
@@ -776,6 +801,7 @@ extern "C" int tm_state_upload(tm_ctx *c, const double *gd, const double *u, con
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }
   c->have_state = true;
+  c->cl_step = 0;  // a held closure correction belongs to the previous state
   return TM_OK;
 }
 
@@ -1065,11 +1091,24 @@ extern "C" int tm_closure_attach(tm_ctx *c, tm_mlp *mlp) {
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
   // the captured step changes shape: capture again at the next run
-  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
-  c->gexec = nullptr;
-  c->graph = nullptr;
+  drop_graphs(c);
   c->cl = mlp;
+  c->cl_step = 0;
+  return TM_OK;
+}
+
+extern "C" int tm_closure_set_interval(tm_ctx *c, int nsteps) {
+  if (!c) return TM_EINVAL;
+  if (nsteps < 1) return c->fail(TM_EINVAL, "closure interval must be >= 1");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (nsteps != c->cl_interval) {
+    if (c->gexec_hold) (void)hipGraphExecDestroy(c->gexec_hold);
+    if (c->graph_hold) (void)hipGraphDestroy(c->graph_hold);
+    c->gexec_hold = nullptr;
+    c->graph_hold = nullptr;
+  }
+  c->cl_interval = nsteps;
+  c->cl_step = 0;
   return TM_OK;
 }
 

@@ -33,7 +33,7 @@ def _bind(L):
     L.tm_ens_download.argtypes = [_vp, C.c_int, C.c_int, _dp, _dp, _dp]
     L.tm_ens_run.argtypes = [_vp, C.POINTER(C.c_int), _dp]
     L.tm_ens_invariants.argtypes = [_vp, C.c_int, C.c_int, _dp]
-    L.tm_ens_set_tuning.argtypes = [_vp, C.c_int, C.c_int]
+    L.tm_ens_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.tm_ens_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), _dp]
     for f in ("tm_ens_create", "tm_ens_destroy", "tm_ens_upload", "tm_ens_download", "tm_ens_run",
               "tm_ens_invariants", "tm_ens_set_tuning", "tm_ens_autotune"):
@@ -105,11 +105,13 @@ class Ensemble:
         self._check(gpu_lib().tm_ens_invariants(self.h, level, member, _p(out)), "tm_ens_invariants")
         return out
 
-    def set_tuning(self, chunk, blocks_per_cu):
-        self._check(gpu_lib().tm_ens_set_tuning(self.h, chunk, blocks_per_cu), "tm_ens_set_tuning")
+    def set_tuning(self, chunk, blocks_per_cu, members_per_thread=1):
+        self._check(gpu_lib().tm_ens_set_tuning(self.h, chunk, blocks_per_cu, members_per_thread),
+                    "tm_ens_set_tuning")
 
     def autotune(self, ticks=2):
-        best = (C.c_int * 2)()
+        """Returns ((chunk, blocks_per_cu, members_per_thread), ms per tick)."""
+        best = (C.c_int * 3)()
         ms = C.c_double()
         self._check(gpu_lib().tm_ens_autotune(self.h, ticks, best, C.byref(ms)), "tm_ens_autotune")
-        return (best[0], best[1]), ms.value
+        return (best[0], best[1], best[2]), ms.value
