@@ -232,10 +232,15 @@ int tm_closure_features(tm_ctx *ctx, tm_mlp *mlp, float *x);
  * between steps (training) without re-attaching. */
 int tm_closure_attach(tm_ctx *ctx, tm_mlp *mlp);
 /* The MLP runs on every nsteps-th step (counted from the attach or the last
- * tm_state_upload) and its correction, with the sqrt(gd_edge) of that step,
- * is added unchanged in the steps between: the form of a closure trained on
- * the mean tendency error over nsteps steps.  Default 1. */
-int tm_closure_set_interval(tm_ctx *ctx, int nsteps);
+ * tm_state_upload), on the state at the start of that step: the form of a
+ * closure trained on the mean tendency error over nsteps steps.  Its
+ * correction (with the sqrt(gd_edge) of that state) is added
+ *   lumped = 0: after every step of the interval, unchanged;
+ *   lumped = 1: once, after the last step of the interval, times nsteps
+ *               (the interval's whole correction, as the training target
+ *               b - Plain^n(a) defines it).
+ * Default 1 step (MLP and correction every step). */
+int tm_closure_set_interval(tm_ctx *ctx, int nsteps, int lumped);
 /* Adds one training sample held on the device: the inputs of the context's
  * current state (normalised with mlp's scales at this call) and its one-step
  * targets in reference order, along the mesh normal,

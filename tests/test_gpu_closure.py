@@ -189,7 +189,8 @@ def test_closure_step_matches_oracle_and_keeps_mass():
 
 def test_closure_interval_holds_the_correction():
     """set_interval(3): the MLP runs on step 0 and its correction (with the
-    sqrt(gd_edge) of step 0) is added after each of the 3 steps."""
+    sqrt(gd_edge) of step 0) is added after each of the 3 steps, or (lumped)
+    once after the third step with 3 dt."""
     m, g, mlp, (gd, u, ghs), (sh, sc, out) = setup_closure(3, s_out=(2e-5, 1e-1))
     p = mlp.params.astype(np.float64)
     dt = 600.0
@@ -211,6 +212,18 @@ def test_closure_interval_holds_the_correction():
     st = g.download()
     assert rel_l2(st["gd"] - gd_p, gd_c - gd_p) <= 5e-3
     assert rel_l2(st["iap_u"] - iap_p, iap_c - iap_p) <= 5e-3
+    # lumped: 3 plain steps, then the correction once with 3 dt
+    gd_l, iap_l = mo.apply(m, gd_p, iap_p, d["gd_edge"], Yr, out, 3 * dt, RADIUS)
+    g.upload(gd, ghs, u=u)
+    clo.set_interval(3, lumped=True)
+    g.step(2)
+    st2 = g.download()
+    gd_p2, iap_p2, _, _, _ = o.run(2, gd, iap, ghs)
+    assert rel_l2(st2["gd"], gd_p2) <= 1e-13  # no correction before the interval ends
+    g.step(1)
+    st = g.download()
+    assert rel_l2(st["gd"] - gd_p, gd_l - gd_p) <= 5e-3
+    assert rel_l2(st["iap_u"] - iap_p, iap_l - iap_p) <= 5e-3
     with pytest.raises(Exception):
         clo.set_interval(0)
     g.close()

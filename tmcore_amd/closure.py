@@ -45,7 +45,7 @@ def _lib():
         L.tm_mlp_adam_step.argtypes = [_vp, _fp, C.c_float]
         L.tm_closure_features.argtypes = [_vp, _vp, _fp]
         L.tm_closure_attach.argtypes = [_vp, _vp]
-        L.tm_closure_set_interval.argtypes = [_vp, C.c_int]
+        L.tm_closure_set_interval.argtypes = [_vp, C.c_int, C.c_int]
         L.tm_closure_dataset_add.argtypes = [_vp, _vp, _dp, _dp, C.c_double]
         L.tm_closure_dataset_size.argtypes = [_vp, C.POINTER(C.c_int)]
         L.tm_closure_dataset_clear.argtypes = [_vp]
@@ -169,9 +169,10 @@ class Closure:
     def detach(self):
         self.g._check(_lib().tm_closure_attach(self.g.h, None), "tm_closure_attach")
 
-    def set_interval(self, nsteps: int):
-        """Run the MLP every nsteps steps and hold its correction in between."""
-        self.g._check(_lib().tm_closure_set_interval(self.g.h, nsteps), "tm_closure_set_interval")
+    def set_interval(self, nsteps: int, lumped: bool = False):
+        """Run the MLP every nsteps steps; add its correction after every step
+        (lumped False) or once per interval, times nsteps (lumped True)."""
+        self.g._check(_lib().tm_closure_set_interval(self.g.h, nsteps, int(lumped)), "tm_closure_set_interval")
 
     def add_sample(self, t_u, t_gd, tg_scale: float):
         """Training sample: inputs of the current state plus targets

@@ -35,7 +35,7 @@ def seed_range(s):
 
 
 def online_check(p: Pipeline, mlp: Mlp, seed: int, hours: int = 24, every_h: int = 3, alphas=(1.0, 0.5),
-                 intervals=None):
+                 intervals=None, lumped=(True,)):
     """Coarse runs from the coarse-grained truth of `seed`, plain (alpha 0) and
     with the closure outputs scaled by alpha, the MLP run every `interval`
     steps (default: the training horizon) and held in between; thickness error
@@ -53,14 +53,15 @@ def online_check(p: Pipeline, mlp: Mlp, seed: int, hours: int = 24, every_h: int
     clo = Closure(g, mlp)
     sh, sc, out = mlp.get_scales()
     res = {}
-    runs = [(0.0, 1)] + [(a, iv) for a in alphas for iv in (intervals or [p.horizon])]
-    for alpha, iv in runs:
+    runs = [(0.0, 1, False)] + [(a, iv, lm) for a in alphas for iv in (intervals or [p.horizon])
+                                for lm in (lumped if iv > 1 else (False,))]
+    for alpha, iv, lm in runs:
         if alpha == 0.0:
             clo.detach()
         else:
             mlp.set_scales(sh, sc, out * alpha)
             clo.attach()
-            clo.set_interval(iv)
+            clo.set_interval(iv, lm)
         g.upload(truth[0][0], ghs_c, u=truth[0][1])
         errs = [0.0]
         for k in range(n):
@@ -70,7 +71,7 @@ def online_check(p: Pipeline, mlp: Mlp, seed: int, hours: int = 24, every_h: int
             except Exception:
                 errs += [float("nan")] * (n - k)
                 break
-        key = "plain" if alpha == 0.0 else f"alpha {alpha:g} interval {iv}"
+        key = "plain" if alpha == 0.0 else f"alpha {alpha:g} interval {iv}" + (" lumped" if lm else "")
         res[key] = errs
         log(f"online seed {seed} {key}: " + " ".join(f"{x:.3e}" for x in errs))
     mlp.set_scales(sh, sc, out)
@@ -92,7 +93,8 @@ def cmd_train(args):
         pending = []
     else:
         pending = [(a, b, g) for s in seeds[:2] for _, a, b, g in p.truth_pairs(s, samples)]
-        scales = p.fit_scales(mlp, pending)
+        masked = tuple(int(f) for f in args.mask_fields.split(",") if f != "")
+        scales = p.fit_scales(mlp, pending, masked)
     # zero-output network with the same scales: the loss of the plain method
     zero = Mlp(seed=0)
     zp = zero.params
@@ -128,7 +130,8 @@ def cmd_train(args):
         hist.append({"epoch": ep, "lr": lr, "train": h, "val": v})
         log(f"epoch {ep}: lr {lr:.2e} train {h} val {v}")
     train_s = time.perf_counter() - t1
-    online = online_check(p, mlp, seeds[-1], args.online_hours) if args.online_hours > 0 else None
+    online = (online_check(p, mlp, seeds[-1], args.online_hours, 6, (1.0, 0.5), [p.horizon], (True,))
+              if args.online_hours > 0 else None)
     meta = {"fine_level": args.fine_level, "coarse_level": args.coarse_level, "dt": args.dt, "nsub": args.nsub,
             "horizon": args.horizon,
             "seeds": [seeds[0], seeds[-1] + 1], "samples": samples, "ntrain": ntrain, "nval": nval,
@@ -152,7 +155,7 @@ class Evaluator:
         self.corr = SwmGpu(cm, cm.fit(), dt=p.dt)
         self.closure = Closure(self.corr, self.mlp)
         self.closure.attach()
-        self.closure.set_interval(int(self.meta.get("horizon", 1)))
+        self.closure.set_interval(int(self.meta.get("horizon", 1)), lumped=True)
         # finer plain runs: compared with the truth filtered to their own level
         # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so
         # the partition noise of the Lr -> coarse map applies to run and truth alike
@@ -223,7 +226,7 @@ def cmd_check(args):
     p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log, horizon=int(meta.get("horizon", 1)))
     alphas = tuple(float(a) for a in args.alphas.split(","))
     intervals = [int(i) for i in args.intervals.split(",")]
-    res = online_check(p, mlp, int(args.seeds.split(":")[0]), args.hours, 3, alphas, intervals)
+    res = online_check(p, mlp, int(args.seeds.split(":")[0]), args.hours, 3, alphas, intervals, (False, True))
     with open(args.out, "w") as f:
         json.dump(res, f, indent=1)
 
@@ -280,6 +283,7 @@ def main():
     t.add_argument("--shuffle-seed", type=int, default=0)
     t.add_argument("--data-budget", type=float, default=1e9, help="seconds of truth generation before training")
     t.add_argument("--online-hours", type=int, default=24, help="online check on the last (held-out) seed")
+    t.add_argument("--mask-fields", default="", help="input fields (0 u, 1 gd_edge, 2 pv_edge, 3 tend_gd) not shown")
     c = sub.choices["check"]
     c.add_argument("--weights", required=True)
     c.add_argument("--hours", type=int, default=24)

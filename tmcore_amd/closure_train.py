@@ -115,8 +115,10 @@ class Pipeline:
         return t_u, t_gd
 
     # ------------------------------------------------------------ training
-    def fit_scales(self, mlp: Mlp, pairs):
-        """Input normalisation per field and output scales from the first pairs."""
+    def fit_scales(self, mlp: Mlp, pairs, masked_fields=()):
+        """Input normalisation per field and output scales from the first pairs;
+        fields in masked_fields get a zero input scale (the network does not
+        see them)."""
         clo = Closure(self.coarse, mlp)
         mlp.set_scales(np.zeros(NIN), np.ones(NIN), np.ones(NOUT))
         xs, tu, tg = [], [], []
@@ -131,13 +133,13 @@ class Pipeline:
         for f in range(4):
             blk = X[:, 10 * f:10 * f + 10]
             sh[10 * f:10 * f + 10] = blk.mean()
-            sc[10 * f:10 * f + 10] = 1.0 / max(blk.std(), 1e-300)
+            sc[10 * f:10 * f + 10] = 0.0 if f in masked_fields else 1.0 / max(blk.std(), 1e-300)
         rms_u = float(np.sqrt(np.mean(np.concatenate(tu) ** 2)))
         rms_g = float(np.sqrt(np.mean(np.concatenate(tg) ** 2)))
         s1 = rms_g * np.mean(self.area) / np.mean(self.dv)
         mlp.set_scales(sh, sc, np.array([rms_u, s1]))
         self.tg_scale = rms_g
-        self.log(f"scales: field means {sh[::10].tolist()} stds {(1 / sc[::10]).tolist()} "
+        self.log(f"scales: field means {sh[::10].tolist()} inverse stds {sc[::10].tolist()} "
                  f"rms t_u {rms_u:.3e} m/s^2 rms t_gd {rms_g:.3e} m^2/s^3")
         return {"in_shift": sh.tolist(), "in_scale": sc.tolist(), "rms_t_u": rms_u, "rms_t_gd": rms_g}
 

@@ -110,15 +110,17 @@ struct tm_ctx {
   int hist_cap = 4096;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
-  // with a closure held over several steps: the step graph without the MLP
-  hipGraph_t graph_hold = nullptr;
-  hipGraphExec_t gexec_hold = nullptr;
+  // closure over several steps: step graphs without the MLP (kind 1: apply
+  // the held correction; 2: MLP only; 3: nothing; 4: apply interval x dt)
+  hipGraph_t graph_k[5] = {};
+  hipGraphExec_t gexec_k[5] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_state = false;
   bool timed = false;
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
-  int cl_interval = 1;  // MLP every cl_interval steps, its correction held in between
+  int cl_interval = 1;  // MLP every cl_interval steps
+  int cl_lumped = 0;    // 0: correction added every step; 1: once per interval, times the interval
   long long cl_step = 0;  // steps since the state or the closure changed
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
   void *cl_sq = nullptr;
@@ -313,7 +315,7 @@ int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
 }
 
 template <typename R>
-int enqueue_step(tm_ctx *c, bool infer) {
+int enqueue_step(tm_ctx *c, bool infer, int apply_factor) {
   for (int k = 1; k <= 4; ++k) {
     int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
     if (rc) return rc;
@@ -327,12 +329,12 @@ int enqueue_step(tm_ctx *c, bool infer) {
   }
   hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 1, true));
   hipLaunchKernelGGL(k_finalize<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, final_bufs<R>(c));
-  if (c->cl) {
+  if (c->cl && apply_factor > 0) {
     ClApplyArgs<R> a{};
     a.Y = c->cl_Y;
     a.sq = (const R *)c->cl_sq;
     a.norm = mlp_state(c->cl).norm;
-    a.dt = c->o.dt;
+    a.dt = c->o.dt * apply_factor;
     a.iap = (R *)c->iap_old;
     a.gd = (R *)c->gd_old;
     hipLaunchKernelGGL(k_cl_apply<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, a);
@@ -352,9 +354,9 @@ int enqueue_refresh(tm_ctx *c) {
   return TM_OK;
 }
 
-int capture_step(tm_ctx *c, bool infer, hipGraph_t *graph, hipGraphExec_t *exec) {
+int capture_step(tm_ctx *c, bool infer, int apply_factor, hipGraph_t *graph, hipGraphExec_t *exec) {
   HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  int rc = c->rsz == 8 ? enqueue_step<double>(c, infer) : enqueue_step<float>(c, infer);
+  int rc = c->rsz == 8 ? enqueue_step<double>(c, infer, apply_factor) : enqueue_step<float>(c, infer, apply_factor);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(c->stream, &g);
   if (rc) return rc;
@@ -367,18 +369,47 @@ int capture_step(tm_ctx *c, bool infer, hipGraph_t *graph, hipGraphExec_t *exec)
 void drop_graphs(tm_ctx *c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
-  if (c->gexec_hold) (void)hipGraphExecDestroy(c->gexec_hold);
-  if (c->graph_hold) (void)hipGraphDestroy(c->graph_hold);
-  c->gexec = c->gexec_hold = nullptr;
-  c->graph = c->graph_hold = nullptr;
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  for (int k = 1; k < 5; ++k) {
+    if (c->gexec_k[k]) (void)hipGraphExecDestroy(c->gexec_k[k]);
+    if (c->graph_k[k]) (void)hipGraphDestroy(c->graph_k[k]);
+    c->gexec_k[k] = nullptr;
+    c->graph_k[k] = nullptr;
+  }
+}
+
+// Step graph of kind k: 0 = MLP + correction (every step of a closure with
+// interval 1, or the plain step without closure), 1 = held correction only,
+// 2 = MLP only, 3 = plain step, 4 = correction times the interval.
+int step_graph(tm_ctx *c, int k, hipGraphExec_t *out) {
+  int rc = TM_OK;
+  if (k == 0) {
+    if (!c->gexec && (rc = capture_step(c, true, 1, &c->graph, &c->gexec))) return rc;
+    *out = c->gexec;
+    return TM_OK;
+  }
+  if (!c->gexec_k[k]) {
+    const bool infer = k == 2;
+    const int factor = k == 1 ? 1 : (k == 4 ? c->cl_interval : 0);
+    if ((rc = capture_step(c, infer, factor, &c->graph_k[k], &c->gexec_k[k]))) return rc;
+  }
+  *out = c->gexec_k[k];
+  return TM_OK;
+}
+
+// Which step graph the closure schedule runs at step s of the interval.
+int schedule_kind(const tm_ctx *c) {
+  if (!c->cl || c->cl_interval == 1) return 0;
+  const int r = (int)(c->cl_step % c->cl_interval);
+  if (!c->cl_lumped) return r == 0 ? 0 : 1;
+  if (r == 0) return 2;
+  return r == c->cl_interval - 1 ? 4 : 3;
 }
 
 int build_graph(tm_ctx *c) {
-  int rc = TM_OK;
-  if (!c->gexec && (rc = capture_step(c, true, &c->graph, &c->gexec))) return rc;
-  if (c->cl && c->cl_interval > 1 && !c->gexec_hold && (rc = capture_step(c, false, &c->graph_hold, &c->gexec_hold)))
-    return rc;
-  return TM_OK;
+  hipGraphExec_t g = nullptr;
+  return step_graph(c, 0, &g);
 }
 
 int run_steps(tm_ctx *c, int nsteps, bool refresh) {
@@ -387,9 +418,9 @@ int run_steps(tm_ctx *c, int nsteps, bool refresh) {
   hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, c->stream, c->hist_idx);
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   for (int s = 0; s < nsteps; ++s) {
-    // a held closure step replays the last MLP outputs (cl_Y, cl_sq)
-    const bool hold = c->cl && c->cl_interval > 1 && (c->cl_step % c->cl_interval) != 0;
-    HIPCHK(c, hipGraphLaunch(hold ? c->gexec_hold : c->gexec, c->stream));
+    hipGraphExec_t g = nullptr;
+    if ((rc = step_graph(c, schedule_kind(c), &g))) return rc;
+    HIPCHK(c, hipGraphLaunch(g, c->stream));
     ++c->cl_step;
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
@@ -1097,17 +1128,14 @@ extern "C" int tm_closure_attach(tm_ctx *c, tm_mlp *mlp) {
   return TM_OK;
 }
 
-extern "C" int tm_closure_set_interval(tm_ctx *c, int nsteps) {
+extern "C" int tm_closure_set_interval(tm_ctx *c, int nsteps, int lumped) {
   if (!c) return TM_EINVAL;
   if (nsteps < 1) return c->fail(TM_EINVAL, "closure interval must be >= 1");
+  if (lumped != 0 && lumped != 1) return c->fail(TM_EINVAL, "lumped must be 0 or 1");
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  if (nsteps != c->cl_interval) {
-    if (c->gexec_hold) (void)hipGraphExecDestroy(c->gexec_hold);
-    if (c->graph_hold) (void)hipGraphDestroy(c->graph_hold);
-    c->gexec_hold = nullptr;
-    c->graph_hold = nullptr;
-  }
+  drop_graphs(c);  // kind 4 bakes the interval into its time step
   c->cl_interval = nsteps;
+  c->cl_lumped = lumped;
   c->cl_step = 0;
   return TM_OK;
 }
@@ -1223,10 +1251,13 @@ extern "C" int tm_closure_train(tm_ctx *c, tm_mlp *mlp, int ntrain, int epochs, 
 #include "ops_kernels.h"
 
 // ------------------------------------------------------------- copy probe
+// One 16-byte element per thread over a grid that covers the array (the
+// fastest of the copy forms measured on MI355X: grid-stride loops with 2-8
+// elements per thread, with and without non-temporal hints, were slower).
 __global__ __launch_bounds__(256) void tm_copy_probe_kernel(const double2 *__restrict__ a, double2 *__restrict__ b,
                                                             size_t n) {
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 extern "C" int tm_hbm_copy_probe(int device, double bytes, int iters, double *gbps) {
@@ -1243,7 +1274,7 @@ extern "C" int tm_hbm_copy_probe(int device, double bytes, int iters, double *gb
     rc = TM_ENOMEM;
   if (!rc) {
     (void)hipMemsetAsync(a, 0, n * sizeof(double2), s);
-    const int grid = 8 * 1024;
+    const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(tm_copy_probe_kernel, dim3(grid), dim3(256), 0, s, (const double2 *)a, (double2 *)b, n);
     (void)hipEventRecord(e0, s);
     for (int i = 0; i < iters; ++i)

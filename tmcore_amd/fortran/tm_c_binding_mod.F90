@@ -214,10 +214,10 @@ module tm_c_binding_mod
       type(c_ptr), value :: ctx, mlp
     end function tm_closure_attach
 
-    integer(c_int) function tm_closure_set_interval(ctx, nsteps) bind(c, name='tm_closure_set_interval')
+    integer(c_int) function tm_closure_set_interval(ctx, nsteps, lumped) bind(c, name='tm_closure_set_interval')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), value :: nsteps
+      integer(c_int), value :: nsteps, lumped
     end function tm_closure_set_interval
 
     integer(c_size_t) function c_strlen(s) bind(c, name='strlen')

@@ -67,10 +67,11 @@ module tm_params_mod
   character(max_file_path_len) :: closure_file = 'N/A'  ! trained closure: raw little-endian float32
                                      ! [38530 parameters, 40 input shifts, 40 input scales, 2 output scales]
 
-  integer :: closure_interval = 1    ! steps between MLP evaluations (the correction is held between)
+  integer :: closure_interval = 1    ! steps between MLP evaluations
+  integer :: closure_lumped = 0      ! 0: correction after every step; 1: once per interval
 
   namelist /gpu_control/ mesh_level, precision, ntracers, seed, tracer_seed0, device, renumber, print_every, &
-                         closure_file, closure_interval
+                         closure_file, closure_interval, closure_lumped
 
 contains
 

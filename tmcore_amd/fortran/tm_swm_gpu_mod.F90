@@ -228,7 +228,7 @@ contains
     call check(tm_mlp_set_scales(mlp, w(nparams + 1:nparams + 40), w(nparams + 41:nparams + 80), &
                                  w(nparams + 81:nparams + 82)), 'tm_mlp_set_scales')
     call check(tm_closure_attach(ctx, mlp), 'tm_closure_attach')
-    call check(tm_closure_set_interval(ctx, closure_interval), 'tm_closure_set_interval')
+    call check(tm_closure_set_interval(ctx, closure_interval, closure_lumped), 'tm_closure_set_interval')
     write(*, '(A)') '[Notice]: learned closure attached from ' // path
 
   end subroutine attach_closure
