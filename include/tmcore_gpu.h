@@ -186,7 +186,10 @@ int tm_ens_autotune(tm_ens *e, int ticks, int best[3], double *best_ms_per_tick)
  * 9 edgesOnEdge neighbours (slots 1..9: the west cell's other edges
  * counter-clockwise from e, then the east cell's; the 10th is not used):
  *   0: kappa(e) * nSign(e,c) * nSign(e',c) * u(e'), c the shared cell
- *   1: gd_edge(e')   2: pv_edge(e')   3: (tend_gd(c1(e')) + tend_gd(c2(e')))/2
+ *   1: gd_edge(e')
+ *   2: relative vorticity at the edge, (vor(v1(e')) + vor(v2(e')))/2
+ *   3: velocity divergence at the edge, (div(c1(e')) + div(c2(e')))/2,
+ *      div_c = (1/A_c) sum_i nSign(i,c) u(e_i) dv(e_i)
  * so the inputs do not depend on the mesh's edge orientation; each is
  * normalised as (value - in_shift[j]) * in_scale[j].
  * Outputs: y0, a normal-velocity tendency kappa * out_scale[0] * y0 (m s^-2),

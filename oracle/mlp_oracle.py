@@ -132,22 +132,37 @@ def slot_edges(mesh):
     return nb
 
 
-def features(mesh, u, gd_edge, pv_edge, tend_gd, in_shift, in_scale):
-    """The 40 normalised inputs of every edge, [nEdges, 40] float32."""
+def divergence(mesh, u, radius):
+    """Velocity divergence of the cells, (1/A_c) sum_i nSign(i,c) u(e_i) dv(e_i)."""
+    dv = mesh.dvEdge * radius
+    A = mesh.areaCell * radius ** 2
+    coe = mesh.cellsOnEdge - 1
+    s = np.bincount(coe[:, 0], weights=u * dv, minlength=mesh.nCells) \
+        - np.bincount(coe[:, 1], weights=u * dv, minlength=mesh.nCells)
+    return s / A
+
+
+def features(mesh, u, gd_edge, vor_vertex, div_cell, in_shift, in_scale):
+    """The 40 normalised inputs of every edge, [nEdges, 40] float32: normal
+    velocity (canonical frame), thickness, vorticity and divergence averaged
+    to the edge, on the edge and its 9 neighbours."""
     nE = mesh.nEdges
     e = np.arange(nE)
     nb = slot_edges(mesh)
     kap = kappa(mesh)
     c1 = mesh.cellsOnEdge[:, 0] - 1
     c2 = mesh.cellsOnEdge[:, 1] - 1
-    tge = 0.5 * (tend_gd[c1] + tend_gd[c2])
+    v1 = mesh.verticesOnEdge[:, 0] - 1
+    v2 = mesh.verticesOnEdge[:, 1] - 1
+    vor_e = 0.5 * (vor_vertex[v1] + vor_vertex[v2])
+    div_e = 0.5 * (div_cell[c1] + div_cell[c2])
     f = np.empty((nE, 40))
     for s in range(10):
         j = nb[:, s]
         f[:, s] = kap * orient(mesh, e, j) * u[j]
         f[:, 10 + s] = gd_edge[j]
-        f[:, 20 + s] = pv_edge[j]
-        f[:, 30 + s] = tge[j]
+        f[:, 20 + s] = vor_e[j]
+        f[:, 30 + s] = div_e[j]
     return ((f - np.asarray(in_shift, np.float64)) * np.asarray(in_scale, np.float64)).astype(np.float32)
 
 

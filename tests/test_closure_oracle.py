@@ -125,9 +125,13 @@ def flipped_mesh(m, flip):
         n = m.nEdgesOnEdge[e]
         n1 = m.nEdgesOnCell[m.cellsOnEdge[e, 0] - 1] - 1
         eoe[e, :n] = np.concatenate([m.edgesOnEdge[e, n1:n], m.edgesOnEdge[e, :n1]])
+    # the vertices of a reversed edge swap too (verticesOnEdge follows the normal)
+    voe = m.verticesOnEdge.copy()
+    voe[flip] = voe[flip][:, ::-1]
     keys = ("nEdges", "nCells", "nEdgesOnEdge", "nEdgesOnCell", "xCell", "yCell", "zCell", "xEdge", "yEdge",
             "dvEdge", "areaCell")
-    return types.SimpleNamespace(cellsOnEdge=coe, edgesOnEdge=eoe, **{k: getattr(m, k) for k in keys})
+    return types.SimpleNamespace(cellsOnEdge=coe, edgesOnEdge=eoe, verticesOnEdge=voe,
+                                 **{k: getattr(m, k) for k in keys})
 
 
 def test_inputs_and_correction_do_not_depend_on_edge_orientation(m2):
@@ -135,14 +139,17 @@ def test_inputs_and_correction_do_not_depend_on_edge_orientation(m2):
     every edge's inputs, and the thickness correction, unchanged."""
     rng = np.random.default_rng(4)
     u = rng.standard_normal(m2.nEdges)
-    gde, pve = rng.standard_normal(m2.nEdges), rng.standard_normal(m2.nEdges)
-    tg = rng.standard_normal(m2.nCells)
+    gde = rng.standard_normal(m2.nEdges)
+    vor, div = rng.standard_normal(m2.nVertices), rng.standard_normal(m2.nCells)
     sh, sc = rng.standard_normal(40), 1 + rng.random(40)
-    x0 = mo.features(m2, u, gde, pve, tg, sh, sc)
+    x0 = mo.features(m2, u, gde, vor, div, sh, sc)
     flip = rng.random(m2.nEdges) < 0.4
     mf = flipped_mesh(m2, flip)
-    x1 = mo.features(mf, np.where(flip, -u, u), gde, pve, tg, sh, sc)
+    x1 = mo.features(mf, np.where(flip, -u, u), gde, vor, div, sh, sc)
     np.testing.assert_array_equal(x0, x1)
+    # the divergence itself is orientation independent
+    np.testing.assert_allclose(mo.divergence(mf, np.where(flip, -u, u), RADIUS), mo.divergence(m2, u, RADIUS),
+                               rtol=1e-12, atol=1e-12 * np.abs(mo.divergence(m2, u, RADIUS)).max())
     # same outputs -> same thickness tendency; the velocity correction flips with the normal
     y1 = rng.standard_normal(m2.nEdges)
     np.testing.assert_allclose(mo.flux_div(mf, y1, 5.0, RADIUS), mo.flux_div(m2, y1, 5.0, RADIUS), rtol=1e-13,
@@ -154,13 +161,13 @@ def test_inputs_and_correction_do_not_depend_on_edge_orientation(m2):
     k0 = mo.kappa
     try:
         mo.kappa = lambda m: np.ones(m.nEdges)
-        assert not np.array_equal(mo.features(m2, u, gde, pve, tg, sh, sc),
-                                  mo.features(mf, np.where(flip, -u, u), gde, pve, tg, sh, sc))
+        assert not np.array_equal(mo.features(m2, u, gde, vor, div, sh, sc),
+                                  mo.features(mf, np.where(flip, -u, u), gde, vor, div, sh, sc))
     finally:
         mo.kappa = k0
     mf_noswap = flipped_mesh(m2, flip)
     mf_noswap.edgesOnEdge = m2.edgesOnEdge
-    assert not np.array_equal(x0, mo.features(mf_noswap, np.where(flip, -u, u), gde, pve, tg, sh, sc))
+    assert not np.array_equal(x0, mo.features(mf_noswap, np.where(flip, -u, u), gde, vor, div, sh, sc))
 
 
 def test_canonical_slots_are_a_permutation_of_edges_on_edge(m2):
@@ -169,3 +176,14 @@ def test_canonical_slots_are_a_permutation_of_edges_on_edge(m2):
         n = m2.nEdgesOnEdge[e]
         assert set(nb[e, 1:]) <= set(m2.edgesOnEdge[e, :n] - 1)
         assert len(set(nb[e, 1:])) == 9
+
+
+def test_divergence_matches_the_c_oracle_div_operator(m2):
+    """div = -div_operator(u) (the reference's div_operator returns -div F,
+    operators_mod.F90:40-55), computed by the independent C oracle."""
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(6)
+    u = rng.standard_normal(m2.nEdges)
+    o = Oracle(m2, m2.fit(), dt=600.0)
+    ref = -o.op("div", u, out_where="c")
+    np.testing.assert_allclose(mo.divergence(m2, u, RADIUS), ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())

@@ -104,8 +104,8 @@ def scales_for(m, gd, u):
     sh[10:20] = np.mean(gd)
     sc[0:10] = 1.0 / 20.0
     sc[10:20] = 1.0 / (9.80616 * 500.0)
-    sc[20:30] = 1.0 / 2e-9
-    sc[30:40] = 1.0 / 1e-1
+    sc[20:30] = 1.0 / 1e-5
+    sc[30:40] = 1.0 / 1e-6
     return sh, sc
 
 
@@ -131,7 +131,7 @@ def test_closure_inputs_match_oracle(precision):
     o = Oracle(m, m.fit(), dt=600.0)
     iap = o.initial_iap(gd, u)
     tg, _, d = o.stage(gd, iap, ghs)
-    xr = mo.features(m, d["u"], d["gd_edge"], d["pv_edge"], tg, sh, sc)
+    xr = mo.features(m, d["u"], d["gd_edge"], d["vor"], mo.divergence(m, d["u"], RADIUS), sh, sc)
     tol = 1e-6 if precision == 64 else 1e-3
     for f in range(4):
         blk = slice(10 * f, 10 * f + 10)
@@ -168,7 +168,7 @@ def test_closure_step_matches_oracle_and_keeps_mass():
     iap = o.initial_iap(gd, u)
     tg, _, d = o.stage(gd, iap, ghs)
     gd_p, iap_p, _, inv_p, _ = o.run(1, gd, iap, ghs)
-    xr = mo.features(m, d["u"], d["gd_edge"], d["pv_edge"], tg, sh, sc)
+    xr = mo.features(m, d["u"], d["gd_edge"], d["vor"], mo.divergence(m, d["u"], RADIUS), sh, sc)
     Yr, _ = mo.forward(p, xr, bf=True)
     gd_c, iap_c = mo.apply(m, gd_p, iap_p, d["gd_edge"], Yr, out, dt, RADIUS)
     Closure(g, mlp).attach()
@@ -197,7 +197,7 @@ def test_closure_interval_holds_the_correction():
     o = Oracle(m, m.fit(), dt=dt)
     iap = o.initial_iap(gd, u)
     tg, _, d = o.stage(gd, iap, ghs)
-    xr = mo.features(m, d["u"], d["gd_edge"], d["pv_edge"], tg, sh, sc)
+    xr = mo.features(m, d["u"], d["gd_edge"], d["vor"], mo.divergence(m, d["u"], RADIUS), sh, sc)
     Yr, _ = mo.forward(p, xr, bf=True)
     gd_c, iap_c = gd, iap
     for _ in range(3):

@@ -20,7 +20,7 @@ from .swm import SwmGpu, gpu_lib
 
 NIN, NHID, NOUT = 40, 128, 2
 NPARAMS = NIN * NHID + NHID + 2 * (NHID * NHID + NHID) + NHID * NOUT + NOUT  # 38530
-FIELDS = ("u", "gd_edge", "pv_edge", "tend_gd_edge")  # input j = field * 10 + slot
+FIELDS = ("u", "gd_edge", "vorticity_edge", "divergence_edge")  # input j = field * 10 + slot
 
 _vp = C.c_void_p
 _fp = C.POINTER(C.c_float)

@@ -83,6 +83,16 @@ def cmd_train(args):
     p = Pipeline(args.fine_level, args.coarse_level, args.dt, args.nsub, log=log, horizon=args.horizon)
     seeds = seed_range(args.seeds)
     samples = [int(k) for k in args.samples.split(",")]
+
+    def sample_steps(seed):
+        """The fixed sample steps, or (--window-end) n samples spaced by the
+        horizon in a window whose end is drawn from [n*horizon, window_end]
+        per seed (seeded), so the initial states cover the whole range."""
+        if args.window_end <= 0:
+            return samples
+        n, h = len(samples), args.horizon
+        end = int(np.random.default_rng(seed).integers(n * h, args.window_end + 1))
+        return [end - (n - j) * h for j in range(n)]
     mlp = Mlp(seed=args.init_seed)
     clo = Closure(p.coarse, mlp)
     t0 = time.perf_counter()
@@ -92,7 +102,7 @@ def cmd_train(args):
         scales = meta0["scales"]
         pending = []
     else:
-        pending = [(a, b, g) for s in seeds[:2] for _, a, b, g in p.truth_pairs(s, samples)]
+        pending = [(a, b, g) for s in seeds[:2] for _, a, b, g in p.truth_pairs(s, sample_steps(s))]
         masked = tuple(int(f) for f in args.mask_fields.split(",") if f != "")
         scales = p.fit_scales(mlp, pending, masked)
     # zero-output network with the same scales: the loss of the plain method
@@ -108,7 +118,7 @@ def cmd_train(args):
         if time.perf_counter() - t0 > args.data_budget:
             log(f"data budget of {args.data_budget:g} s reached: stopping before seed {s}")
             break
-        for _, a, b, g in p.truth_pairs(s, samples):
+        for _, a, b, g in p.truth_pairs(s, sample_steps(s)):
             p.add(clo, a, b, g)
         used.append(s)
         log(f"seed {s}: {clo.size()} samples, {time.perf_counter() - t0:.0f} s")
@@ -134,7 +144,8 @@ def cmd_train(args):
               if args.online_hours > 0 else None)
     meta = {"fine_level": args.fine_level, "coarse_level": args.coarse_level, "dt": args.dt, "nsub": args.nsub,
             "horizon": args.horizon,
-            "seeds": [seeds[0], seeds[-1] + 1], "samples": samples, "ntrain": ntrain, "nval": nval,
+            "seeds": [seeds[0], seeds[-1] + 1], "samples": samples, "window_end": args.window_end,
+            "masked_fields": args.mask_fields, "ntrain": ntrain, "nval": nval,
             "scales": scales, "plain_loss": {"train": plain_train, "val": plain_val}, "history": hist,
             "data_seconds": data_s, "train_seconds": train_s, "init": args.init, "online_check": online,
             "previous": (meta0 if args.init else None)}
@@ -242,7 +253,7 @@ def cmd_profile(args):
     mlp = Mlp(seed=1)
     sh, sc = np.zeros(40), np.ones(40)
     sh[10:20] = np.mean(gd)
-    sc[:10], sc[10:20], sc[20:30], sc[30:40] = 1 / 11.4, 1 / 3900.0, 1 / 2e-9, 1 / 1.6e-2
+    sc[:10], sc[10:20], sc[20:30], sc[30:40] = 1 / 11.4, 1 / 3900.0, 1 / 1e-5, 1 / 1e-6
     mlp.set_scales(sh, sc, np.array([1.2e-5, 2.7e-3 * np.mean(m.areaCell) * RADIUS / np.mean(m.dvEdge)]))
     clo = Closure(g, mlp)
     rng = np.random.default_rng(0)
@@ -283,7 +294,9 @@ def main():
     t.add_argument("--shuffle-seed", type=int, default=0)
     t.add_argument("--data-budget", type=float, default=1e9, help="seconds of truth generation before training")
     t.add_argument("--online-hours", type=int, default=24, help="online check on the last (held-out) seed")
-    t.add_argument("--mask-fields", default="", help="input fields (0 u, 1 gd_edge, 2 pv_edge, 3 tend_gd) not shown")
+    t.add_argument("--mask-fields", default="", help="input fields (0 u, 1 gd_edge, 2 vorticity, 3 divergence) not shown")
+    t.add_argument("--window-end", type=int, default=0,
+                   help="random per-seed sample windows ending by this coarse step (0: the fixed --samples)")
     c = sub.choices["check"]
     c.add_argument("--weights", required=True)
     c.add_argument("--hours", type=int, default=24)

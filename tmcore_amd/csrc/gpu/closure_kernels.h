@@ -10,10 +10,12 @@
 // the correction do not depend on the mesh's edge orientation.
 // Inputs of edge e, j = field * 10 + slot, slot 0 = e, slot s = the s-th
 // edge of that list (of up to 10, the last one is not used):
-//   field 0  kappa(e) * sigma(e, e') * u(e')
-//   field 1  gd_edge(e')
-//   field 2  pv_edge(e')
-//   field 3  tend_gd at the edge, (tend_gd(c1(e')) + tend_gd(c2(e'))) / 2
+//   field 0  kappa(e) * sigma(e, e') * u(e')        normal velocity
+//   field 1  gd_edge(e')                             thickness (g h)
+//   field 2  (vor(v1(e')) + vor(v2(e'))) / 2         relative vorticity
+//   field 3  (div(c1(e')) + div(c2(e'))) / 2         velocity divergence
+// vor is the vertex curl of the stage (curl_operator), div the cell
+// divergence of u, (1/A_c) sum_i nSign(i,c) u(e_i) dv(e_i).
 // sigma(e, e') = nSign(e, c) * nSign(e', c) with c the cell e and e' share,
 // nSign(e, c) = +1 when c = cellsOnEdge(1, e).
 #pragma once
@@ -35,7 +37,7 @@ __device__ __forceinline__ float cl_orient(const DMesh &m, int e, int j) {
 
 template <typename R>
 struct ClFeatArgs {
-  const R *u, *gd_e, *pv_e, *tend_gd;  // stage-1 (start of step) fields
+  const R *u, *gd_e, *vor, *div;       // stage-1 (start of step) fields
   const float *norm;                   // [kNormLen] shift, scale, out scales
   float *X;                            // [nE][40]
   R *sq;                               // [nE] sqrt(gd_edge), or null
@@ -66,10 +68,11 @@ __global__ __launch_bounds__(kBlock) void k_cl_features(DMesh m, ClFeatArgs<R> a
   for (int s = 0; s < kClSlots; ++s) {
     const int j = nb[s];
     const int c1 = m.coe[j], c2 = m.coe[m.nE + j];
+    const int v1 = m.voe[j], v2 = m.voe[m.nE + j];
     f[s] = ((float)kap * cl_orient(m, e, j)) * (float)a.u[j];
     f[kClSlots + s] = (float)((double)a.gd_e[j] - (double)a.norm[kNormShift + kClSlots + s]);
-    f[2 * kClSlots + s] = (float)a.pv_e[j];
-    f[3 * kClSlots + s] = (float)(R(0.5) * (a.tend_gd[c1] + a.tend_gd[c2]));
+    f[2 * kClSlots + s] = (float)(R(0.5) * (a.vor[v1] + a.vor[v2]));
+    f[3 * kClSlots + s] = (float)(R(0.5) * (a.div[c1] + a.div[c2]));
   }
   float4 *out = reinterpret_cast<float4 *>(a.X + (size_t)e * kMlpIn);
 #pragma unroll
@@ -85,6 +88,23 @@ __global__ __launch_bounds__(kBlock) void k_cl_features(DMesh m, ClFeatArgs<R> a
     out[q] = make_float4(v[0], v[1], v[2], v[3]);
   }
   if (a.sq) a.sq[e] = tm_sqrt(a.gd_e[e]);
+}
+
+// Velocity divergence of the cells, (1/A_c) sum_i nSign(i,c) u(e_i) dv(e_i).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cl_div(DMesh m, const R *__restrict__ u, R *__restrict__ div) {
+  const int c = xcd_tile(blockIdx.x, m.nbC) * kBlock + threadIdx.x;
+  if (c >= m.nC) return;
+  const int n = m.nEoC[c];
+  R s = R(0);
+#pragma unroll
+  for (int q = 0; q < kMaxEoC; ++q) {
+    const int idx = m.eocS[q * m.nC + c];
+    const int e = unsign(idx);
+    const R f = u[e] * (R)m.dv[e];
+    s += q < n ? (idx < 0 ? -f : f) : R(0);
+  }
+  div[c] = s / (R)m.areaCell[c];
 }
 
 template <typename R>

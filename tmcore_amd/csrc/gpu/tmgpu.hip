@@ -123,7 +123,7 @@ struct tm_ctx {
   int cl_lumped = 0;    // 0: correction added every step; 1: once per interval, times the interval
   long long cl_step = 0;  // steps since the state or the closure changed
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
-  void *cl_sq = nullptr;
+  void *cl_sq = nullptr, *cl_vor = nullptr, *cl_div = nullptr;
   double *cl_loss = nullptr;
   int cl_loss_cap = 0;
   struct ClSample {
@@ -299,13 +299,16 @@ FinalBufs<R> final_bufs(tm_ctx *c) {
 
 // Closure inputs of the stage-1 fields (the start-of-step state) into X,
 // and sqrt(gd_edge) into sq when sq != nullptr.
+// (the stage's vertex pass wrote the vorticity into cl_vor: see cl_stage_bufs)
 template <typename R>
 int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
+  hipLaunchKernelGGL(k_cl_div<R>, dim3(c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, (const R *)c->u,
+                     (R *)c->cl_div);
   ClFeatArgs<R> a{};
   a.u = (const R *)c->u;
   a.gd_e = (const R *)c->gd_e;
-  a.pv_e = (const R *)c->pv_e;
-  a.tend_gd = (const R *)c->tend_gd;
+  a.vor = (const R *)c->cl_vor;
+  a.div = (const R *)c->cl_div;
   a.norm = mlp_state(c->cl).norm;
   a.X = X;
   a.sq = (R *)sq;
@@ -317,7 +320,9 @@ int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
 template <typename R>
 int enqueue_step(tm_ctx *c, bool infer, int apply_factor) {
   for (int k = 1; k <= 4; ++k) {
-    int rc = launch_stage<R>(c, stage_bufs<R>(c, k, k == 1));
+    StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
+    if (k == 1 && c->cl && infer) b.d_vor = (R *)c->cl_vor;  // closure input
+    int rc = launch_stage<R>(c, b);
     if (rc) return rc;
     if (k == 1 && c->cl && infer) {
       // MLP of the start-of-step state, between stage 1 and stage 2 (which
@@ -1059,7 +1064,7 @@ int cl_reserve(tm_ctx *c) {
   if (c->cl_X) return TM_OK;
   if (!(c->cl_X = c->alloc<float>((size_t)c->nE * kMlpIn)) || !(c->cl_Y = c->alloc<float>((size_t)c->nE * kMlpOut)) ||
       !(c->cl_dY = c->alloc<float>((size_t)c->nE * kMlpOut)) || !(c->cl_r = c->alloc<float>(c->nC)) ||
-      !(c->cl_sq = c->alloc_r(c->nE)))
+      !(c->cl_sq = c->alloc_r(c->nE)) || !(c->cl_vor = c->alloc_r(c->nV)) || !(c->cl_div = c->alloc_r(c->nC)))
     return c->fail(TM_ENOMEM, "hipMalloc closure buffers");
   return TM_OK;
 }
@@ -1075,11 +1080,17 @@ int cl_loss_reserve(tm_ctx *c, int n) {
 
 // Inputs of the current state into X (new order): a diagnostic stage of the
 // state followed by the input kernel, with mlp's normalisation.
+template <typename R>
+int cl_diag_stage(tm_ctx *c) {
+  StageBufs<R> b = stage_bufs<R>(c, 0, false);
+  b.d_vor = (R *)c->cl_vor;
+  return launch_stage<R>(c, b, false);
+}
+
 int cl_features_now(tm_ctx *c, tm_mlp *mlp, float *X) {
   tm_mlp *saved = c->cl;
   c->cl = mlp;  // enqueue_cl_features reads the normalisation of c->cl
-  int rc = c->rsz == 8 ? launch_stage<double>(c, stage_bufs<double>(c, 0, false), true)
-                       : launch_stage<float>(c, stage_bufs<float>(c, 0, false), true);
+  int rc = c->rsz == 8 ? cl_diag_stage<double>(c) : cl_diag_stage<float>(c);
   if (!rc) rc = c->rsz == 8 ? enqueue_cl_features<double>(c, X, nullptr) : enqueue_cl_features<float>(c, X, nullptr);
   c->cl = saved;
   return rc;
