@@ -126,6 +126,10 @@ def setup_closure(level, precision=64, seed=2, dt=600.0, mlp_seed=5, s_out=(1e-6
 @pytest.mark.parametrize("precision", [64, 32])
 def test_closure_inputs_match_oracle(precision):
     m, g, mlp, (gd, u, ghs), (sh, sc, out) = setup_closure(3, precision)
+    # the seeded wind is non-divergent by construction: perturb it so that
+    # the divergence input is more than round-off
+    u = u + 0.5 * np.random.default_rng(1).standard_normal(m.nEdges)
+    g.upload(gd, ghs, u=u)
     print(f"closure inputs L3 precision={precision}", flush=True)
     x = Closure(g, mlp).features()
     o = Oracle(m, m.fit(), dt=600.0)

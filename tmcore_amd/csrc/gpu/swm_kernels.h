@@ -351,6 +351,16 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
   return r;
 }
 
+// Fit weight deriv2OnVertex: FP32 copy in the precision form (a stencil
+// coefficient, not geometry or a TRiSK weight), FP64 otherwise.
+template <typename R>
+__device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
+  if constexpr (sizeof(R) == 4) {
+    if (m.d2v32) return m.d2v32[i];
+  }
+  return (R)m.d2v[i];
+}
+
 // Tangent wind + scheme-3 PV on the edge.  Padded stencil slots point at the
 // element itself with weight 0, so every load is issued unconditionally and
 // a padded term adds an exact +0.
@@ -388,8 +398,8 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
     V d1 = V(R(0)), d2 = V(R(0));
 #pragma unroll
     for (int j = 0; j < kFit2; ++j) {
-      d1 += (R)m.d2v[(size_t)j * m.nE + e] * (q1[j] - p1);
-      d2 += (R)m.d2v[(size_t)(kFit2 + j) * m.nE + e] * (q2[j] - p2);
+      d1 += d2v_at<R>(m, (size_t)j * m.nE + e) * (q1[j] - p1);
+      d2 += d2v_at<R>(m, (size_t)(kFit2 + j) * m.nE + e) * (q2[j] - p2);
     }
     const R dv = (R)m.dv[e];
     pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);

@@ -36,6 +36,7 @@ struct DMesh {
   const int *eoe;         // [10][nE] edgesOnEdge
   const double *woe;      // [10][nE] weightsOnEdge
   const double *d2v;      // [2][12][nE] deriv2OnVertex slots 1..12
+  const float *d2v32;     // the same in FP32 (precision-32 contexts), else null
   const int *eovS;        // [3][nV] signed edgesOnVertex
   const int *cov;         // [3][nV] cellsOnVertex
   const double *kite;     // [3][nV] kiteAreasOnVertex

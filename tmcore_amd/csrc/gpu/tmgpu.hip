@@ -691,6 +691,12 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   UP(eoe, eoe, int);
   UP(woe, woe, double);
   UP(d2v, d2v, double);
+  if (o.precision == 32) {
+    std::vector<float> d2v32(d2v.begin(), d2v.end());
+    UP(d2v32, d2v32, float);
+  } else {
+    d.d2v32 = nullptr;
+  }
   UP(eovS, eovS, int);
   UP(cov, cov, int);
   UP(kite, kite, double);
@@ -773,7 +779,8 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   // Algorithmic bytes (compulsory traffic, actual stencil nnz): constants of
   // one stage + 56 B (FP64) per prognostic element, finalize 24 B per element.
   const double sumEoE = c->md.sumEoE, sumFitE = c->md.sumFitE, sumFitV = c->md.sumFitV, sumEoC = c->md.sumEoC;
-  const double cst = nE * (8.0 + 8.0 + 8.0 + 8.0) + sumEoE * 12.0 + sumFitE * 8.0 +
+  const double fsz = c->rsz == 4 ? 4.0 : 8.0;  // deriv2OnVertex bytes (FP32 copy in the precision form)
+  const double cst = nE * (8.0 + 8.0 + 8.0 + 8.0) + sumEoE * 12.0 + sumFitE * fsz +
                      nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0) + sumFitV * 4.0 + sumEoC * 4.0 + nC * (8.0 + 8.0);
   const double rs = (double)c->rsz;
   const double prog = (double)nE + (double)nC * (1 + c->nT);
@@ -786,7 +793,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * 2.0 +
                                 (T > 0 ? 4.0 * sumEoC + nE * (rs + 8.0) : 0.0) + T * nC * 5.0 * rs +
                                 nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
-  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + 8.0 + 2.0 * rs) + 12.0 * sumEoE + 8.0 * sumFitE +
+  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + 8.0 + 2.0 * rs) + 12.0 * sumEoE + fsz * sumFitE +
                             nV * (rs + 1.0) + 4.0 * sumFitV;
   c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + 8.0 + 9.0 * rs) + 12.0 * sumEoE + nC * 2.0 * rs;
   c->kbytes[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
