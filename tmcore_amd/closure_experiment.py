@@ -218,6 +218,8 @@ def cmd_eval(args):
             walls.setdefault(k, []).append(v)
         res["seeds"][s] = r
         log(f"seed {s}: final-day errors " + ", ".join(f"{k} {v[-1]:.3e}" for k, v in r.items()))
+        with open(args.out, "w") as f:  # partial results survive an interrupted run
+            json.dump(res, f, indent=1)
     keys = list(next(iter(res["seeds"].values())).keys())
     res["mean"] = {k: np.nanmean([res["seeds"][s][k] for s in res["seeds"]], axis=0).tolist() for k in keys}
     res["speed_sim_days_per_hour_wall"] = {k: args.days / (np.mean(v) / 3600.0) for k, v in walls.items()}
