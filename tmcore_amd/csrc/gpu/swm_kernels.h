@@ -52,6 +52,7 @@ struct StageBufs {
   int pv_order;   // interp_pv_order (2..4)
   double2 *part;  // [kNumSums][part_stride] block partial sums (hi, lo)
   int part_stride;
+  int rev;        // sweep the tiles from the last one (see xcd_tile)
   // optional diagnostics (test path), NULL in timed runs
   R *d_ke, *d_vor, *d_gdv, *d_v, *d_pvflx, *d_utend, *d_tiap, *d_tq;
 };
@@ -458,7 +459,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
 // ---------------------------------------------------------------- kernels
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
-  const int tile = xcd_tile(blockIdx.x, m.nbE);
+  const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
   const int e = tile * kBlock + threadIdx.x;
   double en = 0.0;
   if (e < m.nE) en = edge_prep_e<R, Solo>(m, b, Solo{}, e);
@@ -508,20 +509,20 @@ __device__ __forceinline__ void vertex_part(const DMesh &m, const StageBufs<R> &
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b) {
   if ((int)blockIdx.x < m.nbC)
-    cell_part<R>(m, b, xcd_tile(blockIdx.x, m.nbC));
+    cell_part<R>(m, b, xcd_tile(blockIdx.x, m.nbC, b.rev));
   else
-    vertex_part<R>(m, b, xcd_tile(blockIdx.x - m.nbC, m.nbV));
+    vertex_part<R>(m, b, xcd_tile(blockIdx.x - m.nbC, m.nbV, b.rev));
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
-  const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
+  const int e = xcd_tile(blockIdx.x, m.nbE, b.rev) * kBlock + threadIdx.x;
   if (e < m.nE) edge_pv_e<R, Solo>(m, b, Solo{}, e);
 }
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
-  const int tile = xcd_tile(blockIdx.x, m.nbE);
+  const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
   const int e = tile * kBlock + threadIdx.x;
   EdgeSums<double> r{0.0, 0.0};
   if (e < m.nE) r = edge_tend_e<R, Solo>(m, b, Solo{}, e);

@@ -56,5 +56,12 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   const int per = nb / kXcd;
   return (b % kXcd) * per + b / kXcd;
 }
+// The same tiles swept from the other end (rev = 1): consecutive kernels
+// alternate direction, so each starts on the data the previous one touched
+// last, which is still in the memory-side (Infinity) cache.
+__device__ __forceinline__ int xcd_tile(int b, int nb, int rev) {
+  const int t = xcd_tile(b, nb);
+  return rev ? nb - 1 - t : t;
+}
 
 }  // namespace tmk

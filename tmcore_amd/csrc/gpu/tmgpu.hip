@@ -117,6 +117,7 @@ struct tm_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool have_state = false;
   bool timed = false;
+  int serpentine = 1;  // alternate the tile sweep direction between kernels
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
   int cl_interval = 1;  // MLP every cl_interval steps
@@ -242,13 +243,20 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   return b;
 }
 
+// The four kernels sweep forward, backward, forward, backward (see
+// xcd_tile): the tiles each starts on were the last the previous one touched.
 template <typename R>
-int launch_stage(tm_ctx *c, const StageBufs<R> &b, bool tail = true) {
+int launch_stage(tm_ctx *c, const StageBufs<R> &b0, bool tail = true) {
   const DMesh &m = c->dm;
+  StageBufs<R> b = b0;
+  b.rev = 0;
   hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+  b.rev = c->serpentine;
   hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
   if (tail) {
+    b.rev = 0;
     hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    b.rev = c->serpentine;
     hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
   }
   HIPCHK(c, hipGetLastError());
@@ -953,12 +961,16 @@ static int profile_one_step(tm_ctx *c, std::vector<hipEvent_t> &ev, double *ms, 
   for (int k = 1; k <= 4; ++k) {
     StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
     if ((rc = mark(TM_K_EDGE_PREP))) return rc;
+    b.rev = 0;
     hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
     if ((rc = done()) || (rc = mark(TM_K_CELL_VERTEX))) return rc;
+    b.rev = c->serpentine;
     hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
     if ((rc = done()) || (rc = mark(TM_K_EDGE_PV))) return rc;
+    b.rev = 0;
     hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
     if ((rc = done()) || (rc = mark(TM_K_EDGE_TEND))) return rc;
+    b.rev = c->serpentine;
     hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
     if ((rc = done())) return rc;
   }
