@@ -106,6 +106,9 @@ int tm_op_ke(tm_ctx *ctx, const double *u_edge, double *ke_cell);              /
 int tm_op_tangent_wind(tm_ctx *ctx, const double *u_edge, double *v_edge);     /* :223-237 */
 int tm_op_tangent_vor_flux(tm_ctx *ctx, const double *u_edge, const double *gd_edge,
                            const double *pv_edge, double *flux);               /* :239-260 */
+/* f_edge += the upwind correction of f_cell at the context's adv_order
+ * (unchanged at adv_order 2), upwinded by sign(u_edge).                        :423-472 */
+int tm_op_upwind_cell(tm_ctx *ctx, const double *f_cell, const double *u_edge, double *f_edge);
 
 /* Device timing of the last tm_step/tm_step_async batch, in milliseconds,
  * measured with HIP events on the context's stream. */

@@ -9,7 +9,7 @@ order, no FMA contraction, IEEE division and square root).
 import numpy as np
 import pytest
 
-from oracle.oracle import Oracle
+from oracle.oracle import Oracle, _p as _op, lib as oracle_lib
 from tmcore_amd.mesh import Mesh, seeded_state, steady_geostrophic_state
 from tmcore_amd.swm import SwmGpu
 
@@ -64,6 +64,47 @@ def test_stage_operators_match_oracle(level, ntracers):
         assert err <= 1e-12, (k, err)
     if ntracers:
         assert rel_l2(r["tend_q"], d["tend_q"]) <= 1e-12
+    g.close()
+
+
+@pytest.mark.parametrize("adv_order,adv_mono", [(3, True), (3, False), (4, True), (5, True), (6, True)])
+def test_upwind_cell_correction_matches_oracle(adv_order, adv_mono):
+    """add_upwind_correction_on_cell at adv_order 3..6 (operators_mod.F90:423-472):
+    on the initial iap_u, the edge thickness and the edge thickness tendency
+    of every stage, and the tracer edge values; level 4, 2 tracers, cell fits
+    from a max_fit 31 table (order-4 fits for orders 5 and 6)."""
+    m = mesh(4)
+    f = m.fit(max_fit=31, order4=adv_order >= 5)
+    gd, u, ghs, q = seeded_state(m, 2, [201, 202])
+    kw = dict(adv_order=adv_order, adv_mono=adv_mono)
+    o = Oracle(m, f, dt=600.0, **kw)
+    iap = o.initial_iap(gd, u)
+    g = SwmGpu(m, f, dt=600.0, ntracers=2, **kw)
+    g.upload(gd, ghs, u=u, tracers=q)
+    assert rel_l2(g.download()["iap_u"], iap) <= 1e-15
+    # the single operator on random fields (inout f_edge, sign from u)
+    rng = np.random.default_rng(adv_order)
+    fc, ue, fe = rng.standard_normal(m.nCells), rng.standard_normal(m.nEdges), rng.standard_normal(m.nEdges)
+    ref = fe.copy()
+    oracle_lib().om_upwind_cell(o.h, _op(fc), _op(ue), _op(ref))
+    assert rel_l2(g.add_upwind_correction_on_cell(fc, ue, fe), ref) <= 1e-14
+    tg, ti, d = o.stage(gd, iap, ghs, q)
+    # the correction is active: the edge thickness is not the plain cell mean
+    plain = 0.5 * (gd[m.cellsOnEdge[:, 0] - 1] + gd[m.cellsOnEdge[:, 1] - 1])
+    assert rel_l2(d["gd_edge"], plain) > 1e-9
+    r = g.stage()
+    for k, ref in (("tend_gd", tg), ("tend_iap", ti), ("gd_edge", d["gd_edge"]), ("u", d["u"]),
+                   ("tend_q", d["tend_q"])):
+        err = rel_l2(r[k], ref)
+        assert err <= 1e-12, (k, err)
+    gd1, iap1, q1, inv, beta = o.run(3, gd, iap, ghs, q)
+    ginv, gbeta = g.step(3)
+    st = g.download()
+    for k, ref in (("gd", gd1), ("iap_u", iap1), ("tracers", q1)):
+        err = rel_l2(st[k], ref)
+        assert err <= 1e-12, (k, err)
+    np.testing.assert_allclose(gbeta, beta, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(ginv[:, :3], inv[:, :3], rtol=1e-12)
     g.close()
 
 

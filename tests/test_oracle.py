@@ -5,7 +5,8 @@ here (its modules need the absent flogger/fiona libraries).  Each oracle
 function is therefore pinned by what the mathematics fixes: telescoping
 identities, constants preserved by the interpolations, the discrete energy
 and mass conservation the scheme is built for, a library routine
-(numpy.linalg.inv) and an independent numpy least-squares fit.  Each check is
+(numpy.linalg.inv), an independent numpy least-squares fit and closed-form
+derivatives along great circles.  Each check is
 chosen so that a dropped term, a wrong sign or a transposed index fails it.
 """
 import ctypes as C
@@ -169,6 +170,60 @@ def test_poly_fit_stencils():
     # cell second ring: 1 + 6 + 12 = 19 for hexagons
     assert np.bincount(r["nFit4Cells"])[19] > 0.85 * m.nCells
     assert r["nFit4Cells"].min() >= 15  # 14 parameters need at least 14 neighbours
+
+
+def _upwind_terms(level):
+    """om_upwind_cell on f = (b.x)^2 (x the unit position vector) against the
+    closed form.  Along the great circle from a cell centre c in direction n,
+    g(s) = (b.c cos(s/R) + b.n sin(s/R))^2 has g'' = 2((b.n)^2 - (b.c)^2)/R^2 and
+    fourth derivative 8((b.c)^2 - (b.n)^2)/R^4 at s = 0; the correction built
+    from them, with n the direction of the edge's other cell, is what the
+    fitted deriv2OnCell/deriv4OnCell stencils approximate.  Returns the
+    relative L2 errors of four separately isolated terms: the d2 sum (order 4),
+    the sign(u) * coef2 term (order 3 minus order 4), the d4 terms of orders 5
+    and 6."""
+    m = Mesh(level)
+    f = m.fit(max_fit=31, order4=True)
+    P = np.stack([m.xCell, m.yCell, m.zCell], 1)
+    P /= np.linalg.norm(P, axis=1)[:, None]
+    b = np.array([0.3, -0.7, 0.5])
+    fc = (P @ b) ** 2
+    c1, c2 = m.cellsOnEdge[:, 0] - 1, m.cellsOnEdge[:, 1] - 1
+
+    def derivs(ca, cb):
+        pa, pb = P[ca], P[cb]
+        n = pb - np.sum(pa * pb, 1)[:, None] * pa
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        A, B = pa @ b, n @ b
+        return 2 * (B**2 - A**2) / RADIUS**2, 8 * (A**2 - B**2) / RADIUS**4
+
+    D1, Q1 = derivs(c1, c2)
+    D2, Q2 = derivs(c2, c1)
+    dc = m.dcEdge * RADIUS
+    u = np.where(np.random.default_rng(0).random(m.nEdges) < 0.5, -1.0, 1.0)
+    dp = C.POINTER(C.c_double)
+    r = {}
+    for order, mono in ((3, False), (4, True), (5, True), (6, True)):
+        o = Oracle(m, f, adv_order=order, adv_mono=mono)
+        fe = np.zeros(m.nEdges)
+        lib().om_upwind_cell(o.h, fc.ctypes.data_as(dp), u.ctypes.data_as(dp), fe.ctypes.data_as(dp))
+        r[order] = fe
+    rel = lambda a, ref: np.linalg.norm(a - ref) / np.linalg.norm(ref)  # noqa: E731
+    return (rel(r[4], -(dc * dc) * (D1 + D2) / 12.0),
+            rel(r[3] - r[4], (dc * dc) * (u * (D2 - D1)) / 12.0),
+            rel(r[5] - r[4], (dc * dc) ** 2 * ((Q1 + Q2) - u * 0.25 * (Q2 - Q1)) / 60.0),
+            rel(r[6] - r[4], (dc * dc) ** 2 * (Q1 + Q2) / 60.0))
+
+
+def test_upwind_cell_correction_converges_to_closed_form():
+    """Pins add_upwind_correction_on_cell (orders 3-6) and the cell fits it
+    reads: every term tends to the closed form under refinement (measured
+    errors at level 3 / 4: d2 sum 8.9e-3 / 2.9e-3, sign term 4.9e-2 / 3.3e-2,
+    d4 terms 2.2e-2 / 9.2e-3).  A swapped side, a flipped upwind sign or a
+    wrong divisor gives errors of order one."""
+    e3, e4 = _upwind_terms(3), _upwind_terms(4)
+    for a, b_, bound in zip(e3, e4, (4e-3, 4e-2, 1.2e-2, 1.2e-2)):
+        assert b_ < a and b_ < bound, (e3, e4)
 
 
 def test_product_poly_fit_equals_oracle():

@@ -91,6 +91,15 @@ __global__ __launch_bounds__(kBlock) void k_op_tvf(DMesh m, const double *u, con
   out[e] = s;
 }
 
+// add_upwind_correction_on_cell (operators_mod.F90:423-472) with the
+// context's adv_order: fe_in is the c2e value, sign taken from u.
+__global__ __launch_bounds__(kBlock) void k_op_upwind_cell(DMesh m, StageBufs<double> b, const double *f,
+                                                           const double *u, const double *fe_in, double *out) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nE) return;
+  out[e] = upwind_cell_e<double, Solo>(m, b, Solo{}, f, 0, e, fe_in[e], sign1v<double, 1>(u[e]));
+}
+
 }  // namespace tmk
 
 namespace {
@@ -178,5 +187,16 @@ extern "C" int tm_op_tangent_vor_flux(tm_ctx *c, const double *u, const double *
                 [&](int nb, std::vector<double *> &d, double *o) {
                   hipLaunchKernelGGL(tmk::k_op_tvf, dim3(nb), dim3(tmk::kBlock), 0, c->stream, c->dm, d[0], d[1],
                                      d[2], o);
+                });
+}
+
+extern "C" int tm_op_upwind_cell(tm_ctx *c, const double *f_cell, const double *u_edge, double *f_edge) {
+  if (!c) return TM_EINVAL;
+  tmk::StageBufs<double> b = {};
+  set_adv(b, c->o);
+  return run_op(c, {{f_cell, kC}, {u_edge, kE}, {f_edge, kE}}, kE, f_edge,
+                [&](int nb, std::vector<double *> &d, double *o) {
+                  hipLaunchKernelGGL(tmk::k_op_upwind_cell, dim3(nb), dim3(tmk::kBlock), 0, c->stream, c->dm, b,
+                                     d[0], d[1], d[2], o);
                 });
 }

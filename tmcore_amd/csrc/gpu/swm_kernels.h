@@ -50,6 +50,9 @@ struct StageBufs {
   double cdt;     // c_k * dt of update_state (c = 1/2, 1/2, 1)
   int want_inv;   // accumulate the invariants of X_k
   int pv_order;   // interp_pv_order (2..4)
+  int adv_order;  // 2..6: add_upwind_correction_on_cell for orders >= 3
+  double adv_coef2, adv_coef4;
+  R *qe;          // [nT][nE] tracer edge values (adv_order >= 3), else null
   double2 *part;  // [kNumSums][part_stride] block partial sums (hi, lo)
   int part_stride;
   int rev;        // sweep the tiles from the last one (see xcd_tile)
@@ -193,14 +196,49 @@ struct Members {
 };
 using Member = Members<1>;
 
+// sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
+// cell fit stencil (f at offset base; padded slots add an exact 0).
+template <int NS, typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> fit_sum(const DMesh &m, const R *f, size_t base, L at, const int *fit,
+                                                  const double *w, int c, int side, int e) {
+  typedef val_t<R, L::W> V;
+  const V fc = at.ld(f, base + c);
+  int idx[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) idx[j] = fit[j * m.nC + c];
+  V s = V(R(0));
+#pragma unroll
+  for (int j = 0; j < NS; ++j) s += (R)w[((size_t)side * NS + j) * m.nE + e] * (at.ld(f, base + idx[j]) - fc);
+  return s;
+}
+
+// add_upwind_correction_on_cell (operators_mod.F90:423-472) at edge e: fe is
+// c2e(f) there, sgn = sign(1, u_e); returns fe unchanged at adv_order 2.
+template <typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const StageBufs<R> &b, L at, const R *f,
+                                                        size_t base, int e, val_t<R, L::W> fe, val_t<R, L::W> sgn) {
+  typedef val_t<R, L::W> V;
+  if (b.adv_order < 3) return fe;
+  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+  const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e);
+  const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c2, 1, e);
+  const R dc = (R)m.dc[e];
+  if (b.adv_order <= 4) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
+  const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c1, 0, e);
+  const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c2, 1, e);
+  return (fe - ((dc * dc) * (d1 + d2)) / R(12)) +
+         (((dc * dc) * (dc * dc)) * ((q1 + q2) - (sgn * (R)b.adv_coef4) * (q2 - q1))) / R(60);
+}
+
 // gd_edge, u and the per-edge products the cell/vertex sums gather; returns
 // the edge part of the energy invariant when b.want_inv.
 template <typename R, typename L>
 __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
   typedef val_t<R, L::W> V;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V gde = R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2));
   const V iap = at.ld(b.iap, e);
+  const V gde = upwind_cell_e<R, L>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
+                                    sign1v<R, L::W>(iap));
   const V u = iap / tm_sqrt<R, L::W>(gde);
   const R dv = (R)m.dv[e];
   at.st(b.gd_e, e, gde);
@@ -208,6 +246,12 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
   at.st(b.Fdv, e, (u * gde) * dv);
   at.st(b.kee, e, (u * u) * (R)m.aE[e]);
   at.st(b.udc, e, u * (R)m.dc[e]);
+  if (b.qe)  // tracer edge values with the upwind correction (adv_scheme_sg11_mod.F90:42-43)
+    for (int t = 0; t < b.nT; ++t) {
+      const size_t tb = (size_t)t * m.nC;
+      const V qe = R(0.5) * (at.ld(b.q, tb + c1) + at.ld(b.q, tb + c2));
+      at.st(b.qe, (size_t)t * m.nE + e, upwind_cell_e<R, L>(m, b, at, b.q, tb, e, qe, sign1v<R, L::W>(u)));
+    }
   const val_t<double, L::W> id = to_d<R, L::W>(iap);
   return b.want_inv ? (id * id) * m.aE[e] : val_t<double, L::W>(0.0);
 }
@@ -300,7 +344,9 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
       V sq = V(R(0));
 #pragma unroll
       for (int s = 0; s < kMaxEoC; ++s) {
-        const V qe = R(0.5) * (q0 + qn[s]);  // c2e is symmetric in its two cells
+        // c2e is symmetric in its two cells; at adv_order >= 3 the corrected
+        // edge value comes from the edge pass
+        const V qe = b.qe ? at.ld(b.qe, (size_t)t * m.nE + unsign(idx[s])) : V(R(0.5) * (q0 + qn[s]));
         const V fl = (uv[s] * qe) * dvv[s];
         const V f = idx[s] < 0 ? -fl : fl;
         sq += s < n ? f : V(R(0));
@@ -439,7 +485,8 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / (R)m.dc[e];
   const V ut = s - dEdx;
   const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
-  const V tge = R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2));
+  const V tge = upwind_cell_e<R, L>(m, b, at, b.tend_gd, 0, e, R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
+                                    sign1v<R, L::W>(at.ld(b.iap, e)));
   const V it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
   if (b.d_pvflx) at.st(b.d_pvflx, e, s);
   if (b.d_utend) at.st(b.d_utend, e, ut);
@@ -661,12 +708,15 @@ __global__ __launch_bounds__(kBlock) void k_finalize(DMesh m, FinalBufs<R> f) {
   }
 }
 
-// iap_u = sqrt(c2e(gd)) * u (tmcore_swm_mod.F90:62-64)
+// iap_u = sqrt(c2e(gd) + upwind correction) * u (tmcore_swm_mod.F90:62-64).
+// The reference applies the correction with iap_u before it is set; with a
+// zeroed array that is sign(1, 0) = +1 on every edge, which b.adv_* and the
+// +1 below reproduce.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, const R *gd, const R *u, R *iap) {
+__global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, StageBufs<R> b, const R *gd, const R *u, R *iap) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nE) return;
-  const R gde = R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]);
+  const R gde = upwind_cell_e<R, Solo>(m, b, Solo{}, gd, 0, e, R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]), R(1));
   iap[e] = tm_sqrt(gde) * u[e];
 }
 

@@ -12,6 +12,8 @@ constexpr int kXcd = 8;         // MI355X: 8 XCDs, workgroups dealt round-robin
 constexpr int kMaxEoC = 6;      // maxEdges
 constexpr int kMaxEoE = 10;     // edgesOnEdge slots used by hex/pent Voronoi meshes
 constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding centre)
+constexpr int kFit2C = 6;       // cell second-order fit stencil (excluding centre)
+constexpr int kFit4C = 18;      // cell fourth-order fit stencil (excluding centre)
 constexpr int kNumSums = 13;    // global sums per step (see SumId)
 
 enum SumId {
@@ -37,6 +39,12 @@ struct DMesh {
   const double *woe;      // [10][nE] weightsOnEdge
   const double *d2v;      // [2][12][nE] deriv2OnVertex slots 1..12
   const float *d2v32;     // the same in FP32 (precision-32 contexts), else null
+  // cell fits of add_upwind_correction_on_cell (adv_order 3..6), else null:
+  // unused slots point at the cell itself with weight 0
+  const int *fit2C;       // [6][nC] fit2Cells slots 1..6
+  const double *d2c;      // [2][6][nE] deriv2OnCell slots 1..6
+  const int *fit4C;       // [18][nC] fit4Cells slots 1..18 (adv_order 5, 6)
+  const double *d4c;      // [2][18][nE] deriv4OnCell slots 1..18
   const int *eovS;        // [3][nV] signed edgesOnVertex
   const int *cov;         // [3][nV] cellsOnVertex
   const double *kite;     // [3][nV] kiteAreasOnVertex

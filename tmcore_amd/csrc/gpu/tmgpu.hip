@@ -99,6 +99,7 @@ struct tm_ctx {
   void *gd_old = nullptr, *iap_old = nullptr, *q_old = nullptr, *ghs = nullptr;
   void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *qS[2] = {nullptr, nullptr};
   void *acc_gd = nullptr, *acc_iap = nullptr, *acc_q = nullptr;
+  void *qe = nullptr;  // [nT][nE] corrected tracer edge values (adv_order >= 3)
   void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
   void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr;
   void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
@@ -198,6 +199,15 @@ int download_state(tm_ctx *c, double *dst, const void *src, const std::vector<in
   return TM_OK;
 }
 
+// add_upwind_correction_on_cell coefficients (operators_mod.F90:436-441;
+// coef4 keeps its initial 0.25 unless adv_order is 6)
+template <typename R>
+void set_adv(StageBufs<R> &b, const tm_opts &o) {
+  b.adv_order = o.adv_order;
+  b.adv_coef2 = o.adv_order == 3 ? (o.adv_mono ? 0.25 : 1.0) : 0.0;
+  b.adv_coef4 = o.adv_order == 6 ? 0.0 : 0.25;
+}
+
 template <typename R>
 StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   StageBufs<R> b{};
@@ -238,6 +248,8 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.cdt = cf[k] * c->o.dt;
   b.want_inv = want_inv ? 1 : 0;
   b.pv_order = c->o.pv_scheme == 3 ? c->o.interp_pv_order : 2;
+  set_adv(b, c->o);
+  b.qe = P(c->qe);
   b.part = c->part;
   b.part_stride = c->part_stride;
   return b;
@@ -501,7 +513,7 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
   if (o.pv_scheme != 3) return failv(TM_EINVAL, "only pv_scheme = 3 is implemented on the device");
   if (o.interp_pv_order < 2 || o.interp_pv_order > 4)
     return failv(TM_EINVAL, "interp_pv_order must be 2, 3 or 4 on the device");
-  if (o.adv_order != 2) return failv(TM_EINVAL, "only adv_order = 2 is implemented on the device");
+  if (o.adv_order < 2 || o.adv_order > 6) return failv(TM_EINVAL, "adv_order must be 2..6");
   if (o.precision != 64 && o.precision != 32) return failv(TM_EINVAL, "precision must be 64 or 32");
   if (o.ntracers < 0 || o.ntracers > 64) return failv(TM_EINVAL, "ntracers out of range");
   if (!(o.dt > 0.0) || !(o.radius > 0.0)) return failv(TM_EINVAL, "dt and radius must be positive");
@@ -511,6 +523,10 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
   const bool need_fit = o.interp_pv_order >= 3;
   if (need_fit && (!f || !f->nFit2Vertices || !f->fit2Vertices || !f->deriv2OnVertex || f->maxFit < 2))
     return failv(TM_EINVAL, "interp_pv_order >= 3 needs nFit2Vertices, fit2Vertices and deriv2OnVertex");
+  if (o.adv_order >= 3 && (!f || !f->nFit2Cells || !f->fit2Cells || !f->deriv2OnCell || f->maxFit < 2))
+    return failv(TM_EINVAL, "adv_order >= 3 needs nFit2Cells, fit2Cells and deriv2OnCell");
+  if (o.adv_order >= 5 && (!f->nFit4Cells || !f->fit4Cells || !f->deriv4OnCell))
+    return failv(TM_EINVAL, "adv_order >= 5 needs nFit4Cells, fit4Cells and deriv4OnCell");
   const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices, ME = m->maxEdges, ME2 = m->maxEdges2;
   auto inr = [](int v, int n) { return v >= 1 && v <= n; };
   for (int i = 0; i < nC; ++i) {
@@ -541,6 +557,17 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
         if (!inr(f->fit2Vertices[(size_t)v * f->maxFit + j], nV))
           return failv(TM_ERANGE, "fit2Vertices index out of range");
     }
+  for (int order = 2; order <= 4; order += 2) {
+    if (o.adv_order < order + 1) break;
+    const int *nf = order == 2 ? f->nFit2Cells : f->nFit4Cells, *fc = order == 2 ? f->fit2Cells : f->fit4Cells;
+    const int ns = order == 2 ? kFit2C : kFit4C;
+    for (int c = 0; c < nC; ++c) {
+      int n = nf[c];
+      if (n < 1 || n - 1 > ns || n > f->maxFit) return failv(TM_ESTENCIL, "cell fit stencil exceeds the device layout");
+      for (int j = 1; j < n; ++j)
+        if (!inr(fc[(size_t)c * f->maxFit + j], nC)) return failv(TM_ERANGE, "cell fit index out of range");
+    }
+  }
 
   return TM_OK;
 }
@@ -705,6 +732,35 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   } else {
     d.d2v32 = nullptr;
   }
+  // cell fits of the upwind correction, new numbering, padded to the device slots
+  for (int order = 2; order <= 4; order += 2) {
+    if (o.adv_order < order + 1) break;
+    const int ns = order == 2 ? kFit2C : kFit4C;
+    const int *nf = order == 2 ? f->nFit2Cells : f->nFit4Cells, *fc = order == 2 ? f->fit2Cells : f->fit4Cells;
+    const double *dw = order == 2 ? f->deriv2OnCell : f->deriv4OnCell;
+    std::vector<int> fitc((size_t)ns * nC);
+    std::vector<double> dc((size_t)2 * ns * nE, 0.0);
+    for (int ncell = 0; ncell < nC; ++ncell) {
+      const int oc = md.pC[ncell], n = nf[oc];
+      for (int j = 0; j < ns; ++j)
+        fitc[(size_t)j * nC + ncell] = j + 1 < n ? md.iC[fc[(size_t)oc * f->maxFit + j + 1] - 1] : ncell;
+    }
+    for (int ne = 0; ne < nE; ++ne) {
+      const int oe = md.pE[ne];
+      for (int k = 0; k < 2; ++k) {
+        const int n = nf[m->cellsOnEdge[2 * (size_t)oe + k] - 1];
+        for (int j = 0; j < ns; ++j)
+          dc[((size_t)k * ns + j) * nE + ne] = j + 1 < n ? dw[((size_t)oe * 2 + k) * f->maxFit + j + 1] : 0.0;
+      }
+    }
+    if (order == 2) {
+      UP(fit2C, fitc, int);
+      UP(d2c, dc, double);
+    } else {
+      UP(fit4C, fitc, int);
+      UP(d4c, dc, double);
+    }
+  }
   UP(eovS, eovS, int);
   UP(cov, cov, int);
   UP(kite, kite, double);
@@ -776,6 +832,8 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   void **rbufs_q[] = {&c->q_old, &c->qS[0], &c->qS[1], &c->acc_q};
   for (void **p : rbufs_q)
     if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
+  if (o.adv_order >= 3 && c->nT > 0 && !(c->qe = c->alloc_r((size_t)c->nT * nE)))
+    return c->fail(TM_ENOMEM, "hipMalloc tracer edge values");
   c->part_stride = std::max(d.nbE, std::max(d.nbC, d.nbV));
   if (!(c->part = c->alloc<double2>((size_t)kNumSums * c->part_stride))) return c->fail(TM_ENOMEM, "hipMalloc part");
   HIPCHK(c, hipMemsetAsync(c->part, 0, sizeof(double2) * kNumSums * c->part_stride, c->stream));
@@ -843,11 +901,13 @@ extern "C" int tm_state_upload(tm_ctx *c, const double *gd, const double *u, con
     if ((rc = upload_state(c, c->scratch_e[0], u, c->pE, 1, c->nE))) return rc;
     const int nb = (c->nE + kBlock - 1) / kBlock;
     if (c->rsz == 8)
-      hipLaunchKernelGGL(k_iap_from_u<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, (const double *)c->gd_old,
-                         (const double *)c->scratch_e[0], (double *)c->iap_old);
+      hipLaunchKernelGGL(k_iap_from_u<double>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm,
+                         stage_bufs<double>(c, 0, false), (const double *)c->gd_old, (const double *)c->scratch_e[0],
+                         (double *)c->iap_old);
     else
-      hipLaunchKernelGGL(k_iap_from_u<float>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm, (const float *)c->gd_old,
-                         (const float *)c->scratch_e[0], (float *)c->iap_old);
+      hipLaunchKernelGGL(k_iap_from_u<float>, dim3(nb), dim3(kBlock), 0, c->stream, c->dm,
+                         stage_bufs<float>(c, 0, false), (const float *)c->gd_old, (const float *)c->scratch_e[0],
+                         (float *)c->iap_old);
     HIPCHK(c, hipGetLastError());
     HIPCHK(c, hipStreamSynchronize(c->stream));
   }

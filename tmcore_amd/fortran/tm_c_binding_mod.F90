@@ -184,6 +184,12 @@ module tm_c_binding_mod
       real(c_double) :: u(*), gd(*), pv(*), flux(*)
     end function tm_op_tangent_vor_flux
 
+    integer(c_int) function tm_op_upwind_cell(ctx, f_cell, u_edge, f_edge) bind(c, name='tm_op_upwind_cell')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: f_cell(*), u_edge(*), f_edge(*)
+    end function tm_op_upwind_cell
+
     ! learned closure (include/tmcore_gpu.h): the per-edge MLP and its attachment
     integer(c_int) function tm_mlp_create(mlp, device, seed) bind(c, name='tm_mlp_create')
       import :: c_int, c_ptr, c_int64_t

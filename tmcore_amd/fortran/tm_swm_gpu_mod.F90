@@ -26,7 +26,7 @@ module tm_swm_gpu_mod
   public tm_swm_init, tm_swm_run, tm_swm_final
   public div_operator, curl_operator, scalar_c2e_interp_operator, scalar_c2v_interp_operator
   public scalar_v2c_interp_operator, scalar_v2e_interp_operator, calc_kinetic_energy
-  public calc_tangent_wind, calc_tangent_vor_flux
+  public calc_tangent_wind, calc_tangent_vor_flux, add_upwind_correction_on_cell
   public nCells, nEdges, nVertices, gd, u, iap_u, ghs, tracers, run_inv, run_beta, run_ms
 
   ! Mesh (mesh_mod.F90:10-71 names), unit-sphere values as a grid file holds them.
@@ -298,5 +298,11 @@ contains
     real(c_double), intent(out) :: tangent_vor_flux(:)
     call check(tm_op_tangent_vor_flux(ctx, u_edge, gd_edge, pv_edge, tangent_vor_flux), 'tm_op_tangent_vor_flux')
   end subroutine calc_tangent_vor_flux
+
+  subroutine add_upwind_correction_on_cell(f_cell, u_edge, f_edge)
+    real(c_double), intent(in) :: f_cell(:), u_edge(:)
+    real(c_double), intent(inout) :: f_edge(:)
+    call check(tm_op_upwind_cell(ctx, f_cell, u_edge, f_edge), 'tm_op_upwind_cell')
+  end subroutine add_upwind_correction_on_cell
 
 end module tm_swm_gpu_mod

@@ -69,6 +69,7 @@ def gpu_lib() -> C.CDLL:
         for n in ("div", "curl", "c2e", "c2v", "v2c", "v2e", "ke", "tangent_wind"):
             getattr(L, "tm_op_" + n).argtypes = [_vp, _dp, _dp]
         L.tm_op_tangent_vor_flux.argtypes = [_vp, _dp, _dp, _dp, _dp]
+        L.tm_op_upwind_cell.argtypes = [_vp, _dp, _dp, _dp]
         for f in ("tm_ctx_create", "tm_ctx_destroy", "tm_state_upload", "tm_state_download", "tm_step",
                   "tm_step_async", "tm_sync", "tm_invariants", "tm_stage", "tm_last_elapsed_ms", "tm_step_bytes"):
             getattr(L, f).restype = C.c_int
@@ -240,3 +241,13 @@ class SwmGpu:
 
     def calc_tangent_vor_flux(self, u_edge, gd_edge, pv_edge):
         return self._op("tangent_vor_flux", "e", u_edge, gd_edge, pv_edge)
+
+    def add_upwind_correction_on_cell(self, f_cell, u_edge, f_edge):
+        """f_edge plus the upwind correction of f_cell at the context's adv_order
+        (a new array; f_edge itself is left unchanged)."""
+        out = np.array(f_edge, dtype=np.float64)
+        if out.shape != (self.nE,):
+            raise ValueError("f_edge must be [nEdges]")
+        self._check(gpu_lib().tm_op_upwind_cell(self.h, _p(_f64(f_cell)), _p(_f64(u_edge)), _p(out)),
+                    "tm_op_upwind_cell")
+        return out
