@@ -109,6 +109,17 @@ int tm_op_tangent_vor_flux(tm_ctx *ctx, const double *u_edge, const double *gd_e
 /* f_edge += the upwind correction of f_cell at the context's adv_order
  * (unchanged at adv_order 2), upwinded by sign(u_edge).                        :423-472 */
 int tm_op_upwind_cell(tm_ctx *ctx, const double *f_cell, const double *u_edge, double *f_edge);
+int tm_op_pv_vertex(tm_ctx *ctx, const double *vor_vertex, const double *gd_vertex,
+                    double *pv_vertex);                                        /* :262-270 */
+/* The context's pv_scheme (1 APVM, 2 conservative APVM, 3 high-order upwind
+ * with its interp_pv_order); each scheme reads the inputs the reference's does. :273-412 */
+int tm_op_pv_edge(tm_ctx *ctx, const double *u_edge, const double *v_edge, const double *gd_edge,
+                  const double *gd_tend_vertex, const double *pv_vertex, const double *pv_cell,
+                  double *pv_edge);
+/* calc_u_tend_on_edge (tmcore_swm_mod.F90:155-201, flux_4th_order_correct off) */
+int tm_op_u_tend(tm_ctx *ctx, const double *u_edge, const double *ke_cell, const double *gd_cell,
+                 const double *gd_edge, const double *ghs_cell, const double *pv_flx_edge,
+                 const double *gd_tend_edge, double *u_tend_edge, double *iap_u_tend_edge);
 
 /* Device timing of the last tm_step/tm_step_async batch, in milliseconds,
  * measured with HIP events on the context's stream. */

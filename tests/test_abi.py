@@ -46,7 +46,7 @@ def test_ctx_create_rejects_bad_mesh_before_touching_the_device():
     assert not h.value
 
 
-@pytest.mark.parametrize("field,value", [("time_scheme", 9), ("pv_scheme", 1), ("adv_order", 7),
+@pytest.mark.parametrize("field,value", [("time_scheme", 9), ("pv_scheme", 4), ("interp_pv_order", 7), ("adv_order", 7),
                                          ("adv_order", 1),
                                          ("precision", 16), ("ntracers", -1), ("dt", -1.0)])
 def test_ctx_create_rejects_unsupported_options(field, value):
@@ -63,19 +63,22 @@ def test_ctx_create_rejects_unsupported_options(field, value):
     assert not h.value
 
 
-def test_ctx_create_rejects_missing_cell_fits():
-    """adv_order 5 and 6 read the order-4 cell fits; a fit table without them is refused."""
+def test_ctx_create_rejects_missing_order4_fits():
+    """adv_order 5/6 and interp_pv_order 5/6 read the order-4 fits; a fit table
+    without them is refused."""
     from tmcore_amd.swm import Opts, gpu_lib
     m = Mesh(1)
     L = gpu_lib()
     o = Opts()
     L.tm_opts_default(C.byref(o))
-    o.adv_order = 5
-    h = C.c_void_p()
     md = m.desc()
     fd = m.fit().desc()
-    assert L.tm_ctx_create(C.byref(h), C.byref(md), C.byref(fd), C.byref(o)) == -1  # TM_EINVAL
-    assert not h.value
+    for field, value in (("adv_order", 5), ("interp_pv_order", 5)):
+        L.tm_opts_default(C.byref(o))
+        setattr(o, field, value)
+        h = C.c_void_p()
+        assert L.tm_ctx_create(C.byref(h), C.byref(md), C.byref(fd), C.byref(o)) == -1  # TM_EINVAL
+        assert not h.value
 
 
 def test_host_library_errors():

@@ -108,6 +108,42 @@ def test_upwind_cell_correction_matches_oracle(adv_order, adv_mono):
     g.close()
 
 
+@pytest.mark.parametrize("pv_scheme,order", [(1, 3), (2, 3), (3, 2), (3, 4), (3, 5), (3, 6)])
+def test_pv_schemes_match_oracle(pv_scheme, order):
+    """calc_pv_on_edge (operators_mod.F90:273-412): APVM (scheme 1), the
+    conservative APVM with its global eps (scheme 2), and the high-order upwind
+    scheme 3 at interp_pv_order 2, 4, 5, 6 (order 3 is the default of every
+    other test).  Level 4, 1 tracer: the stage outputs, three steps, and the
+    single operator on random inputs."""
+    m = mesh(4)
+    f = m.fit(max_fit=31, order4=order >= 5)
+    gd, u, ghs, q = seeded_state(m, 3, [201])
+    kw = dict(pv_scheme=pv_scheme, interp_pv_order=order)
+    o = Oracle(m, f, dt=600.0, **kw)
+    iap = o.initial_iap(gd, u)
+    g = SwmGpu(m, f, dt=600.0, ntracers=1, **kw)
+    g.upload(gd, ghs, u=u, tracers=q)
+    rng = np.random.default_rng(10 * pv_scheme + order)
+    ue, ve, gde = rng.standard_normal(m.nEdges), rng.standard_normal(m.nEdges), 1 + rng.random(m.nEdges)
+    tgv, pvv, pvc = rng.standard_normal(m.nVertices), rng.standard_normal(m.nVertices), rng.standard_normal(m.nCells)
+    ref = o.op("pv_edge", ue, ve, gde, tgv, pvv, pvc, out_where="e")
+    assert rel_l2(g.calc_pv_on_edge(ue, ve, gde, tgv, pvv, pvc), ref) <= 1e-13
+    tg, ti, d = o.stage(gd, iap, ghs, q)
+    r = g.stage()
+    for k in ("pv_edge", "pv_flx", "u_tend"):
+        err = rel_l2(r[k], d[k])
+        assert err <= 1e-12, (k, err)
+    assert rel_l2(r["tend_iap"], ti) <= 1e-12 and rel_l2(r["tend_gd"], tg) <= 1e-12
+    gd1, iap1, q1, inv, beta = o.run(3, gd, iap, ghs, q)
+    ginv, gbeta = g.step(3)
+    st = g.download()
+    for k, ref in (("gd", gd1), ("iap_u", iap1), ("tracers", q1)):
+        err = rel_l2(st[k], ref)
+        assert err <= 1e-12, (k, err)
+    np.testing.assert_allclose(ginv[:, :3], inv[:, :3], rtol=1e-12)
+    g.close()
+
+
 def test_single_operators_match_oracle():
     m, o, g, (gd, u, iap, ghs, q) = setup(4)
     rng = np.random.default_rng(7)
@@ -124,6 +160,17 @@ def test_single_operators_match_oracle():
         (g.calc_tangent_wind(fe), o.op("tangent_wind", fe, out_where="e")),
         (g.calc_tangent_vor_flux(fe, gde, fv[:0].sum() + fe), o.op("tangent_vor_flux", fe, gde, fe, out_where="e")),
     ]
+    # PV on vertices and edges (scheme 3, order 3) and the u/IAP tendency
+    gdv = 1.0 + np.abs(rng.standard_normal(m.nVertices))
+    ve, pvc = rng.standard_normal(m.nEdges), rng.standard_normal(m.nCells)
+    pairs.append((g.calc_pv_on_vertex(fv, gdv), o.op("pv_vertex", fv, gdv, out_where="v")))
+    pairs.append((g.calc_pv_on_edge(fe, ve, gde, fv, fv, pvc), o.op("pv_edge", fe, ve, gde, fv, fv, pvc, out_where="e")))
+    ke, gdc, ghs_ = (rng.standard_normal(m.nCells) for _ in range(3))
+    pvf, tge = rng.standard_normal(m.nEdges), rng.standard_normal(m.nEdges)
+    ut, it = g.calc_u_tend_on_edge(fe, ke, gdc, gde, ghs_, pvf, tge)
+    ut_o, it_o = np.zeros(m.nEdges), np.zeros(m.nEdges)
+    oracle_lib().om_u_tend(o.h, *[_op(a) for a in (fe, ke, gdc, gde, ghs_, pvf, tge)], _op(ut_o), _op(it_o))
+    pairs += [(ut, ut_o), (it, it_o)]
     for i, (a, b) in enumerate(pairs):
         assert rel_l2(a, b) <= 1e-13, (i, rel_l2(a, b))
     g.close()

@@ -226,6 +226,106 @@ def test_upwind_cell_correction_converges_to_closed_form():
         assert b_ < a and b_ < bound, (e3, e4)
 
 
+def _unit(a):
+    return a / np.linalg.norm(a, axis=1)[:, None]
+
+
+def _pv_terms(level):
+    """calc_pv_on_edge against closed forms on a level-`level` mesh.  Scheme 3:
+    pv = (b.x)^2 at the vertices, the second/fourth derivatives along the great
+    circle from each vertex towards the edge's other vertex as in
+    _upwind_terms, orders differenced to isolate the d2 sum, the sign(v) term
+    and the order-5/6 d4 terms.  Scheme 1 (APVM): pv = b.x at vertices and
+    cells, dpv recovered from the result tends to u n.grad(pv) + v t.grad(pv)
+    at the edge (n from cell 1 to cell 2, t from vertex 1 to vertex 2)."""
+    m = Mesh(level)
+    f = m.fit(max_fit=31, order4=True)
+    Pv = _unit(np.stack([m.xVertex, m.yVertex, m.zVertex], 1))
+    Pc = _unit(np.stack([m.xCell, m.yCell, m.zCell], 1))
+    Pe = _unit(np.stack([m.xEdge, m.yEdge, m.zEdge], 1))
+    b = np.array([0.3, -0.7, 0.5])
+    v1, v2 = m.verticesOnEdge[:, 0] - 1, m.verticesOnEdge[:, 1] - 1
+    c1, c2 = m.cellsOnEdge[:, 0] - 1, m.cellsOnEdge[:, 1] - 1
+
+    def derivs(pa, pb):
+        n = _unit(pb - np.sum(pa * pb, 1)[:, None] * pa)
+        A, B = pa @ b, n @ b
+        return 2 * (B**2 - A**2) / RADIUS**2, 8 * (A**2 - B**2) / RADIUS**4
+
+    D1, Q1 = derivs(Pv[v1], Pv[v2])
+    D2, Q2 = derivs(Pv[v2], Pv[v1])
+    dv = m.dvEdge * RADIUS
+    rng = np.random.default_rng(0)
+    sv = np.where(rng.random(m.nEdges) < 0.5, -1.0, 1.0)
+    z, zv, zc = np.zeros(m.nEdges), np.zeros(m.nVertices), np.zeros(m.nCells)
+    pv2 = (Pv @ b) ** 2
+    r = {order: Oracle(m, f, pv_scheme=3, interp_pv_order=order).op("pv_edge", z, sv, z + 1, zv, pv2, zc,
+                                                                     out_where="e")
+         for order in (2, 3, 4, 5, 6)}
+    rel = lambda a, ref: np.linalg.norm(a - ref) / np.linalg.norm(ref)  # noqa: E731
+    out = [rel(r[4] - r[2], -(dv * dv) * (D1 + D2) / 12.0),
+           rel(r[3] - r[4], (dv * dv) * (sv * (D2 - D1)) / 12.0),
+           rel(r[5] - r[4], (dv * dv) ** 2 * ((Q1 + Q2) - sv * 0.25 * (Q2 - Q1)) / 60.0),
+           rel(r[6] - r[4], (dv * dv) ** 2 * (Q1 + Q2) / 60.0)]
+    pvv, pvc = Pv @ b, Pc @ b
+    u, v = rng.standard_normal(m.nEdges), rng.standard_normal(m.nEdges)
+    w, dt = 0.5, 600.0
+    p1 = Oracle(m, f, pv_scheme=1, dt=dt, apvm_weight=w).op("pv_edge", u, v, z + 1, zv, pvv, pvc, out_where="e")
+    dpv = (0.5 * (pvv[v1] + pvv[v2]) - p1) / (w * dt)
+    grad = (b[None, :] - (Pe @ b)[:, None] * Pe) / RADIUS
+    exact = u * np.sum(_unit(Pc[c2] - Pc[c1]) * grad, 1) + v * np.sum(_unit(Pv[v2] - Pv[v1]) * grad, 1)
+    out.append(rel(dpv, exact))
+    return out
+
+
+def test_pv_edge_schemes_converge_to_closed_form():
+    """Pins calc_pv_on_edge schemes 1 and 3 and the vertex fits
+    (deriv2OnVertex, deriv4OnVertex).  Measured at level 3 / 4: d2 sum
+    1.1e-2 / 3.3e-3, sign(v) term 0.15 / 0.16 (a difference of two fitted
+    second derivatives at the two vertices, whose O(h) errors do not cancel
+    on the alternating triangles: bounded, not convergent), d4 terms
+    1.4e-2 / 6.0e-3 and 8.8e-3 / 2.8e-3, APVM 2.2e-3 / 7.5e-4.  A flipped
+    sign gives errors near 2, a swapped dv/dc or u/v of order one."""
+    e3, e4 = _pv_terms(3), _pv_terms(4)
+    for i, (a, b_, bound) in enumerate(zip(e3, e4, (5e-3, 0.25, 1e-2, 5e-3, 1.5e-3))):
+        assert b_ < bound, (i, e3, e4)
+        if i != 1:
+            assert b_ < a, (i, e3, e4)
+    assert e3[1] < 0.25
+
+
+def test_conservative_apvm_cancels_the_enstrophy_tendency():
+    """Scheme 2 (operators_mod.F90:318-357) on random inputs at level 3: the
+    correction is a multiple of scheme 1's dpv, and with the corrected PV the
+    potential-enstrophy tendency sum(pv^2 gd_tend_v A) - 2 sum(pv curl(F) A),
+    F = calc_tangent_vor_flux(u, gd_edge, pv_edge), vanishes; eps is defined by
+    that and the flux is linear in pv_edge."""
+    m = Mesh(3)
+    rng = np.random.default_rng(5)
+    u, v, gde = rng.standard_normal(m.nEdges), rng.standard_normal(m.nEdges), 1 + rng.random(m.nEdges)
+    tgv, pvv, pvc = rng.standard_normal(m.nVertices), rng.standard_normal(m.nVertices), rng.standard_normal(m.nCells)
+    dt, w = 600.0, 0.5
+    o1 = Oracle(m, None, pv_scheme=1, dt=dt, apvm_weight=w, interp_pv_order=2)
+    o2 = Oracle(m, None, pv_scheme=2, dt=dt, apvm_weight=w, interp_pv_order=2)
+    args = (u, v, gde, tgv, pvv, pvc)
+    p1, p2 = o1.op("pv_edge", *args, out_where="e"), o2.op("pv_edge", *args, out_where="e")
+    v2e = o1.op("v2e", pvv, out_where="e")
+    a1, a2 = v2e - p1, v2e - p2
+    c = np.dot(a1, a2) / np.dot(a1, a1)
+    assert abs(c) > 1e-8
+    assert np.linalg.norm(a2 - c * a1) <= 1e-12 * np.linalg.norm(a2)
+    A = m.areaTriangle
+
+    def ens_tend(pe):
+        vt = o2.op("curl", o2.op("tangent_vor_flux", u, gde, pe, out_where="e"), out_where="v")
+        return np.sum(((pvv * pvv) * tgv) * A), np.sum((pvv * vt) * A)
+
+    s1, s2 = ens_tend(p2)
+    _, s2_0 = ens_tend(v2e)
+    assert abs(s1 - 2 * s2) <= 1e-10 * (abs(s1) + 2 * abs(s2_0))
+    assert abs(s1 - 2 * s2_0) > 1e-3 * (abs(s1) + 2 * abs(s2_0))  # the uncorrected PV does not
+
+
 def test_product_poly_fit_equals_oracle():
     m = Mesh(3)
     r = oracle_poly_fit(m, max_fit=31, order4=True)

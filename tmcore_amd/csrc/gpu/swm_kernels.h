@@ -49,7 +49,14 @@ struct StageBufs {
   int k;          // 1..4: CRK4 stage; 0: diagnostics/invariants only
   double cdt;     // c_k * dt of update_state (c = 1/2, 1/2, 1)
   int want_inv;   // accumulate the invariants of X_k
-  int pv_order;   // interp_pv_order (2..4)
+  int pv_order;   // interp_pv_order (2..6) of pv_scheme 3
+  int pv_scheme;  // 3: high-order upwind; 1: APVM; 2: conservative APVM
+  double apvm_wdt;  // apvm_weight * dt (pv_scheme 1)
+  double dt;
+  R *pv_c, *tgd_v;  // pv_cell (schemes 1, 2), gd_tend_vertex (scheme 2), else null
+  R *dpv, *pvf, *dpvf;  // scheme 2: APVM difference term and the two TRiSK fluxes
+  double2 *cv_part;  // scheme 2: [3][nbV] block partials of the eps sums
+  double *cv_eps;    // scheme 2: eps
   int adv_order;  // 2..6: add_upwind_correction_on_cell for orders >= 3
   double adv_coef2, adv_coef4;
   R *qe;          // [nT][nE] tracer edge values (adv_order >= 3), else null
@@ -197,15 +204,16 @@ struct Members {
 using Member = Members<1>;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
-// cell fit stencil (f at offset base; padded slots add an exact 0).
+// fit stencil of element c (n elements of its kind; f at offset base; padded
+// slots add an exact 0).
 template <int NS, typename R, typename L>
 __device__ __forceinline__ val_t<R, L::W> fit_sum(const DMesh &m, const R *f, size_t base, L at, const int *fit,
-                                                  const double *w, int c, int side, int e) {
+                                                  const double *w, int c, int side, int e, int n) {
   typedef val_t<R, L::W> V;
   const V fc = at.ld(f, base + c);
   int idx[NS];
 #pragma unroll
-  for (int j = 0; j < NS; ++j) idx[j] = fit[j * m.nC + c];
+  for (int j = 0; j < NS; ++j) idx[j] = fit[j * n + c];
   V s = V(R(0));
 #pragma unroll
   for (int j = 0; j < NS; ++j) s += (R)w[((size_t)side * NS + j) * m.nE + e] * (at.ld(f, base + idx[j]) - fc);
@@ -220,12 +228,12 @@ __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const St
   typedef val_t<R, L::W> V;
   if (b.adv_order < 3) return fe;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e);
-  const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c2, 1, e);
+  const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e, m.nC);
+  const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c2, 1, e, m.nC);
   const R dc = (R)m.dc[e];
   if (b.adv_order <= 4) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
-  const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c1, 0, e);
-  const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c2, 1, e);
+  const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c1, 0, e, m.nC);
+  const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c2, 1, e, m.nC);
   return (fe - ((dc * dc) * (d1 + d2)) / R(12)) +
          (((dc * dc) * (dc * dc)) * ((q1 + q2) - (sgn * (R)b.adv_coef4) * (q2 - q1))) / R(60);
 }
@@ -367,6 +375,12 @@ struct VertSums {
   D ens, av;
 };
 
+// calc_pv_on_vertex (operators_mod.F90:262-270)
+template <typename R, typename V>
+__device__ __forceinline__ V pv_vertex_at(const DMesh &m, int v, V vor, V gdv) {
+  return ((R)m.fV[v] + vor) / gdv;
+}
+
 template <typename R, typename L>
 __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m, const StageBufs<R> &b, L at,
                                                                   int v) {
@@ -385,7 +399,7 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
 #pragma unroll
   for (int j = 0; j < 3; ++j) g = g + at.ld(b.gd, m.cov[j * m.nV + v]) * (R)m.kite[j * m.nV + v];
   const V gdv = g / (R)At;
-  const V pv = ((R)m.fV[v] + vor) / gdv;
+  const V pv = pv_vertex_at<R, V>(m, v, vor, gdv);
   at.st(b.pv_v, v, pv);
   if (b.d_vor) at.st(b.d_vor, v, vor);
   if (b.d_gdv) at.st(b.d_gdv, v, gdv);
@@ -408,27 +422,32 @@ __device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
   return (R)m.d2v[i];
 }
 
-// Tangent wind + scheme-3 PV on the edge.  Padded stencil slots point at the
-// element itself with weight 0, so every load is issued unconditionally and
-// a padded term adds an exact +0.
+// PV on the edge (calc_pv_on_edge, operators_mod.F90:273-412) from b.pv_v
+// and the tangent wind v: scheme 3 (high-order upwind, :359-412) in full;
+// schemes 1/2 read pv_cell (b.pv_c) and u, and scheme 2 leaves v2e(pv) here
+// and its correction to k_cvor_*.  Padded
+// stencil slots point at the element itself with weight 0, so every load is
+// issued unconditionally and a padded term adds an exact +0.
 template <typename R, typename L>
-__device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+__device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
+                                                     val_t<R, L::W> v) {
   typedef val_t<R, L::W> V;
-  V v = V(R(0));
-  {
-    int ep[kMaxEoE];
-    V uv[kMaxEoE];
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
-#pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
-  }
   const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
   const V p1 = at.ld(b.pv_v, v1), p2 = at.ld(b.pv_v, v2);
   V pve = R(0.5) * (p1 + p2);
-  if (b.pv_order == 3 || b.pv_order == 4) {
+  if (b.pv_scheme != 3) {
+    // APVM (:296-316) and conservative APVM (:318-357): v2e(pv) minus a
+    // multiple of dpv = v dpv/dt-along + u dpv/dn
+    const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+    const V dpv = (v * (p2 - p1)) / (R)m.dv[e] +
+                  (at.ld(b.u, e) * (at.ld(b.pv_c, c2) - at.ld(b.pv_c, c1))) / (R)m.dc[e];
+    if (b.pv_scheme == 2) {
+      at.st(b.dpv, e, dpv);  // eps comes from global sums (k_cvor_*)
+      return pve;
+    }
+    return b.apvm_wdt != 0.0 ? pve - (R)b.apvm_wdt * dpv : pve;
+  }
+  if (b.pv_order >= 3) {
     const R coef2 = b.pv_order == 3 ? R(1) : R(0);
     int f1[kFit2], f2[kFit2];
 #pragma unroll
@@ -449,9 +468,35 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
       d2 += d2v_at<R>(m, (size_t)(kFit2 + j) * m.nE + e) * (q2[j] - p2);
     }
     const R dv = (R)m.dv[e];
-    pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
+    if (b.pv_order <= 4) {
+      pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
+    } else {
+      const V q1 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, v1, 0, e, m.nV);
+      const V q2 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, v2, 1, e, m.nV);
+      const R coef4 = b.pv_order == 6 ? R(0) : R(0.25);
+      pve = (pve - ((dv * dv) * (d1 + d2)) / R(12)) +
+            (((dv * dv) * (dv * dv)) * ((q1 + q2) - (sign1v<R, L::W>(v) * coef4) * (q2 - q1))) / R(60);
+    }
   }
-  at.st(b.pv_e, e, pve);
+  return pve;
+}
+
+// Tangent wind (calc_tangent_wind) + scheme-3 PV on the edge.
+template <typename R, typename L>
+__device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+  typedef val_t<R, L::W> V;
+  V v = V(R(0));
+  {
+    int ep[kMaxEoE];
+    V uv[kMaxEoE];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
+  }
+  at.st(b.pv_e, e, pv_edge_at<R, L>(m, b, at, e, v));
   if (b.d_v) at.st(b.d_v, e, v);
 }
 
@@ -459,6 +504,21 @@ template <typename D>
 struct EdgeSums {
   D rp, r44;
 };
+
+// calc_u_tend_on_edge (tmcore_swm_mod.F90:155-201, flux_4th_order_correct
+// off) from E = ke + gd + ghs in b.E, gd_edge, u, the PV flux s and the edge
+// thickness tendency tge: returns u_tend, sets the IAP tendency.
+template <typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
+                                                    val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it) {
+  typedef val_t<R, L::W> V;
+  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / (R)m.dc[e];
+  const V ut = s - dEdx;
+  const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
+  it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
+  return ut;
+}
 
 // TRiSK PV flux, edge gradient, IAP tendency and the RK update of iap_u.
 template <typename R, typename L>
@@ -482,12 +542,10 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   for (int q = 0; q < kMaxEoE; ++q)
     s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / (R)m.dc[e];
-  const V ut = s - dEdx;
-  const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
   const V tge = upwind_cell_e<R, L>(m, b, at, b.tend_gd, 0, e, R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
                                     sign1v<R, L::W>(at.ld(b.iap, e)));
-  const V it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
+  V it;
+  const V ut = u_tend_at<R, L>(m, b, at, e, s, tge, it);
   if (b.d_pvflx) at.st(b.d_pvflx, e, s);
   if (b.d_utend) at.st(b.d_utend, e, ut);
   if (b.d_tiap) at.st(b.d_tiap, e, it);
@@ -585,6 +643,113 @@ __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
       block_sums<1>(v, ids, b.part, b.part_stride, tile);
     }
   }
+}
+
+// ------------------------------------------------ pv_scheme 1 and 2 (FP R)
+// pv_cell = scalar_v2c(pv_vertex) (operators_mod.F90:125-141; the reference's
+// scatter over cellsOnVertex, as a gather over verticesOnCell) and, for scheme
+// 2, gd_tend_vertex = scalar_c2v(tend_gd) (tmcore_swm_mod.F90:109).  Blocks
+// [0, nbC) do cells, the rest vertices.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_pv_cell(DMesh m, StageBufs<R> b) {
+  if ((int)blockIdx.x < m.nbC) {
+    const int c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= m.nC) return;
+    const R A = (R)m.areaCell[c];
+    R s = R(0);
+    for (int q = 0; q < m.nEoC[c]; ++q) s = s + (b.pv_v[m.vocS[q * m.nC + c]] * (R)m.kiteC[q * m.nC + c]) / A;
+    b.pv_c[c] = s;
+  } else {
+    const int v = (blockIdx.x - m.nbC) * kBlock + threadIdx.x;
+    if (v >= m.nV) return;
+    R g = R(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g = g + b.tend_gd[m.cov[j * m.nV + v]] * (R)m.kite[j * m.nV + v];
+    b.tgd_v[v] = g / (R)m.areaTri[v];
+  }
+}
+
+// Conservative APVM (operators_mod.F90:318-357), after k_edge_pv left
+// pv_e = v2e(pv) and dpv: the TRiSK fluxes of both (calc_tangent_vor_flux,
+// :239-260) ...
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cvor_flux(DMesh m, StageBufs<R> b) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nE) return;
+  const R p0 = b.pv_e[e], d0 = b.dpv[e];
+  R s = R(0), sd = R(0);
+#pragma unroll
+  for (int q = 0; q < kMaxEoE; ++q) {
+    const int ep = m.eoe[q * m.nE + e];
+    const R f = (((R)m.woe[q * m.nE + e] * b.u[ep]) * b.gd_e[ep]) * R(0.5);
+    s = s + f * (p0 + b.pv_e[ep]);
+    sd = sd + f * (d0 + b.dpv[ep]);
+  }
+  b.pvf[e] = s;
+  b.dpvf[e] = sd;
+}
+
+// ... their vertex curls (calc_vor_tend_on_vertex, :414-421) and the block
+// partials of sum(pv^2 gd_tend_v A), sum(pv vor_tend A), sum(pv dvor_tend A) ...
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cvor_sums(DMesh m, StageBufs<R> b) {
+  const int v = blockIdx.x * kBlock + threadIdx.x;
+  double t[3] = {0.0, 0.0, 0.0};
+  if (v < m.nV) {
+    R s = R(0), sd = R(0);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int idx = m.eovS[j * m.nV + v], e = unsign(idx);
+      const R f = b.pvf[e] * (R)m.dc[e], fd = b.dpvf[e] * (R)m.dc[e];
+      s += idx < 0 ? -f : f;
+      sd += idx < 0 ? -fd : fd;
+    }
+    const double At = m.areaTri[v];
+    const double vt = (double)((-s) / (R)At), dvt = (double)((-sd) / (R)At);
+    const double p = (double)b.pv_v[v];
+    t[0] = ((p * p) * (double)b.tgd_v[v]) * At;
+    t[1] = (p * vt) * At;
+    t[2] = (p * dvt) * At;
+  }
+  const int ids[3] = {0, 1, 2};
+  block_sums<3>(t, ids, b.cv_part, m.nbV, blockIdx.x);
+}
+
+// ... eps = -(s1 - 2 s2) / (2 dt s3) from the double-double partials (one block) ...
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cvor_eps(DMesh m, StageBufs<R> b) {
+  __shared__ double2 sh[3][kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double2 s[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    double2 a = make_double2(0.0, 0.0);
+    for (int k = threadIdx.x; k < m.nbV; k += kBlock) a = dd_add(a, b.cv_part[i * m.nbV + k]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      a = dd_add(a, make_double2(__shfl_xor(a.x, off, 64), __shfl_xor(a.y, off, 64)));
+    if (lane == 0) sh[i][w] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      s[i] = sh[i][0];
+#pragma unroll
+      for (int q = 1; q < kBlock / 64; ++q) s[i] = dd_add(s[i], sh[i][q]);
+    }
+    const double s1 = s[0].x + s[0].y, s2 = s[1].x + s[1].y, s3 = s[2].x + s[2].y;
+    b.cv_eps[0] = -(s1 - s2 * 2.0) / ((2.0 * b.dt) * s3);
+  }
+}
+
+// ... and pv_edge = pv_edge - eps dt dpv.
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cvor_apply(DMesh m, StageBufs<R> b) {
+  const int e = blockIdx.x * kBlock + threadIdx.x;
+  if (e >= m.nE) return;
+  const R f = (R)(b.cv_eps[0] * b.dt);
+  b.pv_e[e] = b.pv_e[e] - f * b.dpv[e];
 }
 
 // Global sums: block i of k_reduce folds the block partials of sum i

@@ -14,6 +14,7 @@ constexpr int kMaxEoE = 10;     // edgesOnEdge slots used by hex/pent Voronoi me
 constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding centre)
 constexpr int kFit2C = 6;       // cell second-order fit stencil (excluding centre)
 constexpr int kFit4C = 18;      // cell fourth-order fit stencil (excluding centre)
+constexpr int kFit4V = 21;      // vertex fourth-order fit stencil (excluding centre)
 constexpr int kNumSums = 13;    // global sums per step (see SumId)
 
 enum SumId {
@@ -52,6 +53,8 @@ struct DMesh {
   const double *fV;       // [nV] 2*Omega*sin(latVertex)
   const int8_t *nFit2V;   // [nV] nFit2Vertices
   const int *fit2V;       // [12][nV] fit2Vertices slots 1..12
+  const int *fit4V;       // [21][nV] fit4Vertices slots 1..21 (interp_pv_order 5, 6), else null
+  const double *d4v;      // [2][21][nE] deriv4OnVertex slots 1..21
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
   const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
   const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))

@@ -100,6 +100,10 @@ struct tm_ctx {
   void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *qS[2] = {nullptr, nullptr};
   void *acc_gd = nullptr, *acc_iap = nullptr, *acc_q = nullptr;
   void *qe = nullptr;  // [nT][nE] corrected tracer edge values (adv_order >= 3)
+  // pv_scheme 1/2 work arrays (null at scheme 3)
+  void *pv_c = nullptr, *tgd_v = nullptr, *dpv = nullptr, *pvf = nullptr, *dpvf = nullptr;
+  double2 *cv_part = nullptr;
+  double *cv_eps = nullptr;
   void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
   void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr;
   void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
@@ -248,6 +252,16 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.cdt = cf[k] * c->o.dt;
   b.want_inv = want_inv ? 1 : 0;
   b.pv_order = c->o.pv_scheme == 3 ? c->o.interp_pv_order : 2;
+  b.pv_scheme = c->o.pv_scheme;
+  b.apvm_wdt = c->o.apvm_weight * c->o.dt;
+  b.dt = c->o.dt;
+  b.pv_c = P(c->pv_c);
+  b.tgd_v = P(c->tgd_v);
+  b.dpv = P(c->dpv);
+  b.pvf = P(c->pvf);
+  b.dpvf = P(c->dpvf);
+  b.cv_part = c->cv_part;
+  b.cv_eps = c->cv_eps;
   set_adv(b, c->o);
   b.qe = P(c->qe);
   b.part = c->part;
@@ -266,8 +280,17 @@ int launch_stage(tm_ctx *c, const StageBufs<R> &b0, bool tail = true) {
   b.rev = c->serpentine;
   hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
   if (tail) {
+    const int ps = c->o.pv_scheme;
+    if (ps != 3)  // pv_cell (and gd_tend_vertex for scheme 2)
+      hipLaunchKernelGGL(k_pv_cell<R>, dim3(m.nbC + (ps == 2 ? m.nbV : 0)), dim3(kBlock), 0, c->stream, m, b);
     b.rev = 0;
     hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    if (ps == 2) {
+      hipLaunchKernelGGL(k_cvor_flux<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+      hipLaunchKernelGGL(k_cvor_sums<R>, dim3(m.nbV), dim3(kBlock), 0, c->stream, m, b);
+      hipLaunchKernelGGL(k_cvor_eps<R>, dim3(1), dim3(kBlock), 0, c->stream, m, b);
+      hipLaunchKernelGGL(k_cvor_apply<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+    }
     b.rev = c->serpentine;
     hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
   }
@@ -510,9 +533,8 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
     return code;
   };
   if (o.time_scheme != 2) return failv(TM_EINVAL, "only time_scheme = 2 (CRK4) is implemented on the device");
-  if (o.pv_scheme != 3) return failv(TM_EINVAL, "only pv_scheme = 3 is implemented on the device");
-  if (o.interp_pv_order < 2 || o.interp_pv_order > 4)
-    return failv(TM_EINVAL, "interp_pv_order must be 2, 3 or 4 on the device");
+  if (o.pv_scheme < 1 || o.pv_scheme > 3) return failv(TM_EINVAL, "pv_scheme must be 1, 2 or 3");
+  if (o.interp_pv_order < 2 || o.interp_pv_order > 6) return failv(TM_EINVAL, "interp_pv_order must be 2..6");
   if (o.adv_order < 2 || o.adv_order > 6) return failv(TM_EINVAL, "adv_order must be 2..6");
   if (o.precision != 64 && o.precision != 32) return failv(TM_EINVAL, "precision must be 64 or 32");
   if (o.ntracers < 0 || o.ntracers > 64) return failv(TM_EINVAL, "ntracers out of range");
@@ -520,9 +542,12 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
   if (!m || m->nCells <= 0 || m->nEdges <= 0 || m->nVertices <= 0 || m->vertexDegree != 3 || m->maxEdges > 6 ||
       m->maxEdges < 3 || m->maxEdges2 < 1)
     return failv(TM_EINVAL, "bad mesh dimensions");
-  const bool need_fit = o.interp_pv_order >= 3;
+  const bool need_fit = o.pv_scheme == 3 && o.interp_pv_order >= 3;
   if (need_fit && (!f || !f->nFit2Vertices || !f->fit2Vertices || !f->deriv2OnVertex || f->maxFit < 2))
     return failv(TM_EINVAL, "interp_pv_order >= 3 needs nFit2Vertices, fit2Vertices and deriv2OnVertex");
+  const bool need_fit4v = o.pv_scheme == 3 && o.interp_pv_order >= 5;
+  if (need_fit4v && (!f->nFit4Vertices || !f->fit4Vertices || !f->deriv4OnVertex))
+    return failv(TM_EINVAL, "interp_pv_order >= 5 needs nFit4Vertices, fit4Vertices and deriv4OnVertex");
   if (o.adv_order >= 3 && (!f || !f->nFit2Cells || !f->fit2Cells || !f->deriv2OnCell || f->maxFit < 2))
     return failv(TM_EINVAL, "adv_order >= 3 needs nFit2Cells, fit2Cells and deriv2OnCell");
   if (o.adv_order >= 5 && (!f->nFit4Cells || !f->fit4Cells || !f->deriv4OnCell))
@@ -556,6 +581,14 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
       for (int j = 1; j < n; ++j)
         if (!inr(f->fit2Vertices[(size_t)v * f->maxFit + j], nV))
           return failv(TM_ERANGE, "fit2Vertices index out of range");
+    }
+  if (need_fit4v)
+    for (int v = 0; v < nV; ++v) {
+      int n = f->nFit4Vertices[v];
+      if (n < 1 || n - 1 > kFit4V || n > f->maxFit) return failv(TM_ESTENCIL, "nFit4Vertices exceeds 22");
+      for (int j = 1; j < n; ++j)
+        if (!inr(f->fit4Vertices[(size_t)v * f->maxFit + j], nV))
+          return failv(TM_ERANGE, "fit4Vertices index out of range");
     }
   for (int order = 2; order <= 4; order += 2) {
     if (o.adv_order < order + 1) break;
@@ -761,6 +794,27 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
       UP(d4c, dc, double);
     }
   }
+  // vertex fourth-order fits of the scheme-3 PV (interp_pv_order 5, 6)
+  if (o.pv_scheme == 3 && o.interp_pv_order >= 5) {
+    std::vector<int> fitv((size_t)kFit4V * nV);
+    std::vector<double> dv4((size_t)2 * kFit4V * nE, 0.0);
+    for (int nv = 0; nv < nV; ++nv) {
+      const int ov = md.pV[nv], n = f->nFit4Vertices[ov];
+      for (int j = 0; j < kFit4V; ++j)
+        fitv[(size_t)j * nV + nv] = j + 1 < n ? md.iV[f->fit4Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
+    }
+    for (int ne = 0; ne < nE; ++ne) {
+      const int oe = md.pE[ne];
+      for (int k = 0; k < 2; ++k) {
+        const int n = f->nFit4Vertices[m->verticesOnEdge[2 * (size_t)oe + k] - 1];
+        for (int j = 0; j < kFit4V; ++j)
+          dv4[((size_t)k * kFit4V + j) * nE + ne] =
+              j + 1 < n ? f->deriv4OnVertex[((size_t)oe * 2 + k) * f->maxFit + j + 1] : 0.0;
+      }
+    }
+    UP(fit4V, fitv, int);
+    UP(d4v, dv4, double);
+  }
   UP(eovS, eovS, int);
   UP(cov, cov, int);
   UP(kite, kite, double);
@@ -834,6 +888,12 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
   if (o.adv_order >= 3 && c->nT > 0 && !(c->qe = c->alloc_r((size_t)c->nT * nE)))
     return c->fail(TM_ENOMEM, "hipMalloc tracer edge values");
+  if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
+  if (o.pv_scheme == 2 &&
+      (!(c->tgd_v = c->alloc_r(nV)) || !(c->dpv = c->alloc_r(nE)) || !(c->pvf = c->alloc_r(nE)) ||
+       !(c->dpvf = c->alloc_r(nE)) || !(c->cv_part = c->alloc<double2>((size_t)3 * d.nbV)) ||
+       !(c->cv_eps = c->alloc<double>(1))))
+    return c->fail(TM_ENOMEM, "hipMalloc conservative APVM work arrays");
   c->part_stride = std::max(d.nbE, std::max(d.nbC, d.nbV));
   if (!(c->part = c->alloc<double2>((size_t)kNumSums * c->part_stride))) return c->fail(TM_ENOMEM, "hipMalloc part");
   HIPCHK(c, hipMemsetAsync(c->part, 0, sizeof(double2) * kNumSums * c->part_stride, c->stream));

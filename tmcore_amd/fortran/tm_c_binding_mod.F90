@@ -190,6 +190,27 @@ module tm_c_binding_mod
       real(c_double) :: f_cell(*), u_edge(*), f_edge(*)
     end function tm_op_upwind_cell
 
+    integer(c_int) function tm_op_pv_vertex(ctx, vor_vertex, gd_vertex, pv_vertex) bind(c, name='tm_op_pv_vertex')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: vor_vertex(*), gd_vertex(*), pv_vertex(*)
+    end function tm_op_pv_vertex
+
+    integer(c_int) function tm_op_pv_edge(ctx, u_edge, v_edge, gd_edge, gd_tend_vertex, pv_vertex, pv_cell, pv_edge) &
+        bind(c, name='tm_op_pv_edge')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: u_edge(*), v_edge(*), gd_edge(*), gd_tend_vertex(*), pv_vertex(*), pv_cell(*), pv_edge(*)
+    end function tm_op_pv_edge
+
+    integer(c_int) function tm_op_u_tend(ctx, u_edge, ke_cell, gd_cell, gd_edge, ghs_cell, pv_flx_edge, gd_tend_edge, &
+        u_tend_edge, iap_u_tend_edge) bind(c, name='tm_op_u_tend')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: u_edge(*), ke_cell(*), gd_cell(*), gd_edge(*), ghs_cell(*), pv_flx_edge(*), gd_tend_edge(*)
+      real(c_double) :: u_tend_edge(*), iap_u_tend_edge(*)
+    end function tm_op_u_tend
+
     ! learned closure (include/tmcore_gpu.h): the per-edge MLP and its attachment
     integer(c_int) function tm_mlp_create(mlp, device, seed) bind(c, name='tm_mlp_create')
       import :: c_int, c_ptr, c_int64_t

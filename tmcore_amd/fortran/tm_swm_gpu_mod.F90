@@ -27,6 +27,7 @@ module tm_swm_gpu_mod
   public div_operator, curl_operator, scalar_c2e_interp_operator, scalar_c2v_interp_operator
   public scalar_v2c_interp_operator, scalar_v2e_interp_operator, calc_kinetic_energy
   public calc_tangent_wind, calc_tangent_vor_flux, add_upwind_correction_on_cell
+  public calc_pv_on_vertex, calc_pv_on_edge, calc_u_tend_on_edge
   public nCells, nEdges, nVertices, gd, u, iap_u, ghs, tracers, run_inv, run_beta, run_ms
 
   ! Mesh (mesh_mod.F90:10-71 names), unit-sphere values as a grid file holds them.
@@ -304,5 +305,27 @@ contains
     real(c_double), intent(inout) :: f_edge(:)
     call check(tm_op_upwind_cell(ctx, f_cell, u_edge, f_edge), 'tm_op_upwind_cell')
   end subroutine add_upwind_correction_on_cell
+
+  subroutine calc_pv_on_vertex(vor_vertex, gd_vertex, pv_vertex)
+    real(c_double), intent(in) :: vor_vertex(:), gd_vertex(:)
+    real(c_double), intent(out) :: pv_vertex(:)
+    call check(tm_op_pv_vertex(ctx, vor_vertex, gd_vertex, pv_vertex), 'tm_op_pv_vertex')
+  end subroutine calc_pv_on_vertex
+
+  subroutine calc_pv_on_edge(u_edge, v_edge, gd_edge, gd_tend_vertex, pv_vertex, pv_cell, pv_edge)
+    real(c_double), intent(in) :: u_edge(:), v_edge(:), gd_edge(:), gd_tend_vertex(:), pv_vertex(:), pv_cell(:)
+    real(c_double), intent(out) :: pv_edge(:)
+    call check(tm_op_pv_edge(ctx, u_edge, v_edge, gd_edge, gd_tend_vertex, pv_vertex, pv_cell, pv_edge), &
+               'tm_op_pv_edge')
+  end subroutine calc_pv_on_edge
+
+  subroutine calc_u_tend_on_edge(u_edge, ke_cell, gd_cell, gd_edge, ghs_cell, pv_flx_edge, gd_tend_edge, &
+                                 u_tend_edge, iap_u_tend_edge)
+    real(c_double), intent(in) :: u_edge(:), ke_cell(:), gd_cell(:), gd_edge(:), ghs_cell(:), pv_flx_edge(:)
+    real(c_double), intent(in) :: gd_tend_edge(:)
+    real(c_double), intent(out) :: u_tend_edge(:), iap_u_tend_edge(:)
+    call check(tm_op_u_tend(ctx, u_edge, ke_cell, gd_cell, gd_edge, ghs_cell, pv_flx_edge, gd_tend_edge, &
+                            u_tend_edge, iap_u_tend_edge), 'tm_op_u_tend')
+  end subroutine calc_u_tend_on_edge
 
 end module tm_swm_gpu_mod

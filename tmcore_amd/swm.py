@@ -70,6 +70,9 @@ def gpu_lib() -> C.CDLL:
             getattr(L, "tm_op_" + n).argtypes = [_vp, _dp, _dp]
         L.tm_op_tangent_vor_flux.argtypes = [_vp, _dp, _dp, _dp, _dp]
         L.tm_op_upwind_cell.argtypes = [_vp, _dp, _dp, _dp]
+        L.tm_op_pv_vertex.argtypes = [_vp, _dp, _dp, _dp]
+        L.tm_op_pv_edge.argtypes = [_vp] + [_dp] * 7
+        L.tm_op_u_tend.argtypes = [_vp] + [_dp] * 9
         for f in ("tm_ctx_create", "tm_ctx_destroy", "tm_state_upload", "tm_state_download", "tm_step",
                   "tm_step_async", "tm_sync", "tm_invariants", "tm_stage", "tm_last_elapsed_ms", "tm_step_bytes"):
             getattr(L, f).restype = C.c_int
@@ -241,6 +244,19 @@ class SwmGpu:
 
     def calc_tangent_vor_flux(self, u_edge, gd_edge, pv_edge):
         return self._op("tangent_vor_flux", "e", u_edge, gd_edge, pv_edge)
+
+    def calc_pv_on_vertex(self, vor_vertex, gd_vertex):
+        return self._op("pv_vertex", "v", vor_vertex, gd_vertex)
+
+    def calc_pv_on_edge(self, u_edge, v_edge, gd_edge, gd_tend_vertex, pv_vertex, pv_cell):
+        return self._op("pv_edge", "e", u_edge, v_edge, gd_edge, gd_tend_vertex, pv_vertex, pv_cell)
+
+    def calc_u_tend_on_edge(self, u_edge, ke_cell, gd_cell, gd_edge, ghs_cell, pv_flx_edge, gd_tend_edge):
+        """(u_tend_edge, iap_u_tend_edge)"""
+        ins = [_f64(a) for a in (u_edge, ke_cell, gd_cell, gd_edge, ghs_cell, pv_flx_edge, gd_tend_edge)]
+        ut, it = np.zeros(self.nE), np.zeros(self.nE)
+        self._check(gpu_lib().tm_op_u_tend(self.h, *[_p(a) for a in ins], _p(ut), _p(it)), "tm_op_u_tend")
+        return ut, it
 
     def add_upwind_correction_on_cell(self, f_cell, u_edge, f_edge):
         """f_edge plus the upwind correction of f_cell at the context's adv_order
