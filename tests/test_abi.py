@@ -81,6 +81,21 @@ def test_ctx_create_rejects_missing_order4_fits():
         assert not h.value
 
 
+def test_ctx_create_rejects_tracers_outside_crk4():
+    """The reference advances tracers with CRK4 only; CRK3/PC2 with tracers is refused."""
+    from tmcore_amd.swm import Opts, gpu_lib
+    m = Mesh(1)
+    L = gpu_lib()
+    o = Opts()
+    L.tm_opts_default(C.byref(o))
+    o.time_scheme, o.ntracers = 1, 1
+    h = C.c_void_p()
+    md = m.desc()
+    fd = m.fit().desc()
+    assert L.tm_ctx_create(C.byref(h), C.byref(md), C.byref(fd), C.byref(o)) == -1  # TM_EINVAL
+    assert not h.value
+
+
 def test_host_library_errors():
     lib = _ffi.host()
     n = C.c_int32()

@@ -326,6 +326,33 @@ def test_conservative_apvm_cancels_the_enstrophy_tendency():
     assert abs(s1 - 2 * s2_0) > 1e-3 * (abs(s1) + 2 * abs(s2_0))  # the uncorrected PV does not
 
 
+def test_crk3_and_pc2_orders_and_energy_conservation():
+    """Pins time_integrate_CRK3 and time_integrate_PC2 (time_scheme_mod.F90:65-107,
+    156-184) in the oracle: the thickness error after 2 h against a dt = 37.5 s
+    CRK4 run falls by ~2^3 per halving of dt for CRK3 (measured 8.5, 8.2) and
+    tends to 2^2 for PC2 (3.0, 3.5), which a wrong stage coefficient destroys;
+    with conserve_energy the energy is conserved to round-off (|drift| ~1e-16
+    against ~1e-8 without), which a wrong beta formula breaks."""
+    m = Mesh(3)
+    f = m.fit()
+    gd, u, ghs, _ = seeded_state(m, 4)
+
+    def run(ts, dt, ce=False):
+        o = Oracle(m, f, dt=dt, time_scheme=ts, conserve_energy=ce)
+        iap = o.initial_iap(gd, u)
+        return o.run(int(round(7200.0 / dt)), gd, iap, ghs)
+
+    ref = run(2, 37.5)[0]
+    for ts, lo, hi in ((1, 6.5, 10.0), (3, 2.6, 5.0)):
+        errs = [np.linalg.norm(run(ts, dt)[0] - ref) / np.linalg.norm(ref) for dt in (600.0, 300.0, 150.0)]
+        ratios = [errs[0] / errs[1], errs[1] / errs[2]]
+        assert all(lo < r < hi for r in ratios), (ts, errs, ratios)
+        inv_b = run(ts, 600.0, True)[3]
+        inv_n = run(ts, 600.0, False)[3]
+        assert abs(inv_b[-1, 1] / inv_b[0, 1] - 1) < 1e-13
+        assert abs(inv_n[-1, 1] / inv_n[0, 1] - 1) > 1e-10
+
+
 def test_product_poly_fit_equals_oracle():
     m = Mesh(3)
     r = oracle_poly_fit(m, max_fit=31, order4=True)

@@ -715,8 +715,8 @@ int ens_create_levels(tm_ens *E) {
 
 extern "C" int tm_ens_create(tm_ens **out, int nlevels, const tm_ens_level *levels_in, const tm_opts *o) {
   if (!out || !levels_in || !o || nlevels < 1 || nlevels > kMaxLevels) return TM_EINVAL;
-  if (o->adv_order != 2 || o->pv_scheme != 3 || o->interp_pv_order > 4) {
-    std::fprintf(stderr, "tm_ens_create: the ensemble runs adv_order 2, pv_scheme 3, interp_pv_order <= 4\n");
+  if (o->adv_order != 2 || o->pv_scheme != 3 || o->interp_pv_order > 4 || o->time_scheme != 2) {
+    std::fprintf(stderr, "tm_ens_create: the ensemble runs CRK4, adv_order 2, pv_scheme 3, interp_pv_order <= 4\n");
     return TM_EINVAL;
   }
   *out = nullptr;

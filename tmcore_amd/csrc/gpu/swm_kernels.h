@@ -60,6 +60,14 @@ struct StageBufs {
   int adv_order;  // 2..6: add_upwind_correction_on_cell for orders >= 3
   double adv_coef2, adv_coef4;
   R *qe;          // [nT][nE] tracer edge values (adv_order >= 3), else null
+  // stage descriptor of the kXExt kernels (time_scheme_mod.F90:65-184):
+  // X_next = (base [+ cprev_dt T_prev]) + cdt T_cur unless ts_last; the
+  // accumulator: ts_acc 0 untouched, 1 acc = T, 2 acc += ts_w T, 3 acc =
+  // (acc + T) / 6; sums: T_prev.T into rid_prev, T_first.T into rid_first
+  // (edge ids; cells +1; -1 = none), acc.acc into kR44 at the last stage
+  int ts_last, ts_acc, rid_prev, rid_first;
+  double ts_w, cprev_dt;
+  const R *tfirst_gd, *tfirst_iap;
   double2 *part;  // [kNumSums][part_stride] block partial sums (hi, lo)
   int part_stride;
   int rev;        // sweep the tiles from the last one (see xcd_tile)
@@ -203,6 +211,14 @@ struct Members {
 };
 using Member = Members<1>;
 
+// Compile-time variants of the stage code: X = 0 is the default configuration
+// (adv_order 2, pv_scheme 3 with interp_pv_order <= 4, CRK4) and carries none
+// of the code below, so its kernels keep their register budget; kXAdv adds the
+// upwind cell correction (adv_order >= 3), kXExt the PV schemes 1/2, the
+// order-5/6 PV and the stage-descriptor time integration (CRK3, PC2; CRK4
+// too in that variant), all still checked at run time inside.
+constexpr int kXAdv = 1, kXExt = 2;
+
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (n elements of its kind; f at offset base; padded
 // slots add an exact 0).
@@ -222,10 +238,11 @@ __device__ __forceinline__ val_t<R, L::W> fit_sum(const DMesh &m, const R *f, si
 
 // add_upwind_correction_on_cell (operators_mod.F90:423-472) at edge e: fe is
 // c2e(f) there, sgn = sign(1, u_e); returns fe unchanged at adv_order 2.
-template <typename R, typename L>
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const StageBufs<R> &b, L at, const R *f,
                                                         size_t base, int e, val_t<R, L::W> fe, val_t<R, L::W> sgn) {
   typedef val_t<R, L::W> V;
+  if constexpr ((X & kXAdv) == 0) return fe;
   if (b.adv_order < 3) return fe;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e, m.nC);
@@ -240,13 +257,13 @@ __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const St
 
 // gd_edge, u and the per-edge products the cell/vertex sums gather; returns
 // the edge part of the energy invariant when b.want_inv.
-template <typename R, typename L>
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
   typedef val_t<R, L::W> V;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V iap = at.ld(b.iap, e);
-  const V gde = upwind_cell_e<R, L>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
-                                    sign1v<R, L::W>(iap));
+  const V gde = upwind_cell_e<R, L, X>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
+                                       sign1v<R, L::W>(iap));
   const V u = iap / tm_sqrt<R, L::W>(gde);
   const R dv = (R)m.dv[e];
   at.st(b.gd_e, e, gde);
@@ -254,12 +271,14 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
   at.st(b.Fdv, e, (u * gde) * dv);
   at.st(b.kee, e, (u * u) * (R)m.aE[e]);
   at.st(b.udc, e, u * (R)m.dc[e]);
-  if (b.qe)  // tracer edge values with the upwind correction (adv_scheme_sg11_mod.F90:42-43)
-    for (int t = 0; t < b.nT; ++t) {
-      const size_t tb = (size_t)t * m.nC;
-      const V qe = R(0.5) * (at.ld(b.q, tb + c1) + at.ld(b.q, tb + c2));
-      at.st(b.qe, (size_t)t * m.nE + e, upwind_cell_e<R, L>(m, b, at, b.q, tb, e, qe, sign1v<R, L::W>(u)));
-    }
+  if constexpr ((X & kXAdv) != 0) {
+    if (b.qe)  // tracer edge values with the upwind correction (adv_scheme_sg11_mod.F90:42-43)
+      for (int t = 0; t < b.nT; ++t) {
+        const size_t tb = (size_t)t * m.nC;
+        const V qe = R(0.5) * (at.ld(b.q, tb + c1) + at.ld(b.q, tb + c2));
+        at.st(b.qe, (size_t)t * m.nE + e, upwind_cell_e<R, L, X>(m, b, at, b.q, tb, e, qe, sign1v<R, L::W>(u)));
+      }
+  }
   const val_t<double, L::W> id = to_d<R, L::W>(iap);
   return b.want_inv ? (id * id) * m.aE[e] : val_t<double, L::W>(0.0);
 }
@@ -282,14 +301,44 @@ __device__ __forceinline__ val_t<R, L::W> rk_accumulate(const StageBufs<R> &b, R
 
 template <typename D>
 struct CellSums {
-  D mass, enc, rp, r44;
+  D mass, enc, rp, r44, rf;
 };
 
+// One element's RK update of a stage in the kXExt kernels (stage descriptor
+// above); rp, rf, r44 are its parts of the stage's inner products.
 template <typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> rk_update_ext(const StageBufs<R> &b, L at, size_t i, val_t<R, L::W> t,
+                                                        const R *base, R *next, R *acc_arr, const R *tprev,
+                                                        const R *tfirst, R *tcur, double w, val_t<double, L::W> &rp,
+                                                        val_t<double, L::W> &rf, val_t<double, L::W> &r44) {
+  typedef val_t<R, L::W> V;
+  if (!b.ts_last) {
+    V x = at.ld(base, i);
+    if (b.cprev_dt != 0.0) x = x + (R)b.cprev_dt * at.ld(tprev, i);
+    at.st(next, i, x + (R)b.cdt * t);
+  }
+  V acc = t;
+  if (b.ts_acc == 1) {
+    at.st(acc_arr, i, t);
+  } else if (b.ts_acc == 2) {
+    acc = at.ld(acc_arr, i) + (R)b.ts_w * t;
+    at.st(acc_arr, i, acc);
+  } else if (b.ts_acc == 3) {
+    acc = (at.ld(acc_arr, i) + t) / R(6);
+    at.st(acc_arr, i, acc);
+  }
+  if (b.rid_prev >= 0) rp = (to_d<R, L::W>(at.ld(tprev, i)) * to_d<R, L::W>(t)) * w;
+  if (b.rid_first >= 0) rf = (to_d<R, L::W>(at.ld(tfirst, i)) * to_d<R, L::W>(t)) * w;
+  if (b.ts_last) r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * w;
+  if (!b.ts_last) at.st(tcur, i, t);
+  return acc;
+}
+
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, const StageBufs<R> &b, L at, int c) {
   typedef val_t<R, L::W> V;
   typedef val_t<double, L::W> D;
-  CellSums<D> r{D(0.0), D(0.0), D(0.0), D(0.0)};
+  CellSums<D> r{D(0.0), D(0.0), D(0.0), D(0.0), D(0.0)};
   // Unused slots of pentagons hold index 0 (a valid edge): every gather is
   // issued unconditionally so they overlap, and the unused terms are
   // replaced by an exact +0 afterwards.
@@ -325,11 +374,16 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
     r.enc = ((g * g) + (2.0 * g) * to_d<R, L::W>(ghs)) * A;
   }
   if (b.k >= 1) {
-    if (b.k < 4) at.st(b.gd_next, c, at.ld(b.gd_base, c) + (R)b.cdt * tend);
-    const V acc = rk_accumulate<R, L>(b, b.acc_gd, at, c, tend);
-    if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_gd, c)) * to_d<R, L::W>(tend)) * A;
-    if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * A;
-    if (b.k < 4) at.st(b.tcur_gd, c, tend);
+    if constexpr ((X & kXExt) != 0) {
+      (void)rk_update_ext<R, L>(b, at, c, tend, b.gd_base, b.gd_next, b.acc_gd, b.tprev_gd, b.tfirst_gd, b.tcur_gd,
+                                A, r.rp, r.rf, r.r44);
+    } else {
+      if (b.k < 4) at.st(b.gd_next, c, at.ld(b.gd_base, c) + (R)b.cdt * tend);
+      const V acc = rk_accumulate<R, L>(b, b.acc_gd, at, c, tend);
+      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_gd, c)) * to_d<R, L::W>(tend)) * A;
+      if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * A;
+      if (b.k < 4) at.st(b.tcur_gd, c, tend);
+    }
   }
   if (b.nT > 0) {
     int nb[kMaxEoC];
@@ -354,7 +408,9 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
       for (int s = 0; s < kMaxEoC; ++s) {
         // c2e is symmetric in its two cells; at adv_order >= 3 the corrected
         // edge value comes from the edge pass
-        const V qe = b.qe ? at.ld(b.qe, (size_t)t * m.nE + unsign(idx[s])) : V(R(0.5) * (q0 + qn[s]));
+        V qe = R(0.5) * (q0 + qn[s]);
+        if constexpr ((X & kXAdv) != 0)
+          if (b.qe) qe = at.ld(b.qe, (size_t)t * m.nE + unsign(idx[s]));
         const V fl = (uv[s] * qe) * dvv[s];
         const V f = idx[s] < 0 ? -fl : fl;
         sq += s < n ? f : V(R(0));
@@ -428,14 +484,14 @@ __device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
 // and its correction to k_cvor_*.  Padded
 // stencil slots point at the element itself with weight 0, so every load is
 // issued unconditionally and a padded term adds an exact +0.
-template <typename R, typename L>
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
                                                      val_t<R, L::W> v) {
   typedef val_t<R, L::W> V;
   const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
   const V p1 = at.ld(b.pv_v, v1), p2 = at.ld(b.pv_v, v2);
   V pve = R(0.5) * (p1 + p2);
-  if (b.pv_scheme != 3) {
+  if ((X & kXExt) != 0 && b.pv_scheme != 3) {
     // APVM (:296-316) and conservative APVM (:318-357): v2e(pv) minus a
     // multiple of dpv = v dpv/dt-along + u dpv/dn
     const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
@@ -468,7 +524,7 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
       d2 += d2v_at<R>(m, (size_t)(kFit2 + j) * m.nE + e) * (q2[j] - p2);
     }
     const R dv = (R)m.dv[e];
-    if (b.pv_order <= 4) {
+    if ((X & kXExt) == 0 || b.pv_order <= 4) {
       pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
     } else {
       const V q1 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, v1, 0, e, m.nV);
@@ -482,7 +538,7 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
 }
 
 // Tangent wind (calc_tangent_wind) + scheme-3 PV on the edge.
-template <typename R, typename L>
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
   typedef val_t<R, L::W> V;
   V v = V(R(0));
@@ -496,13 +552,13 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
   }
-  at.st(b.pv_e, e, pv_edge_at<R, L>(m, b, at, e, v));
+  at.st(b.pv_e, e, pv_edge_at<R, L, X>(m, b, at, e, v));
   if (b.d_v) at.st(b.d_v, e, v);
 }
 
 template <typename D>
 struct EdgeSums {
-  D rp, r44;
+  D rp, r44, rf;
 };
 
 // calc_u_tend_on_edge (tmcore_swm_mod.F90:155-201, flux_4th_order_correct
@@ -521,7 +577,7 @@ __device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageB
 }
 
 // TRiSK PV flux, edge gradient, IAP tendency and the RK update of iap_u.
-template <typename R, typename L>
+template <typename R, typename L, int X = 0>
 __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh &m, const StageBufs<R> &b, L at,
                                                                      int e) {
   typedef val_t<R, L::W> V;
@@ -542,32 +598,38 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   for (int q = 0; q < kMaxEoE; ++q)
     s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V tge = upwind_cell_e<R, L>(m, b, at, b.tend_gd, 0, e, R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
-                                    sign1v<R, L::W>(at.ld(b.iap, e)));
+  const V tge = upwind_cell_e<R, L, X>(m, b, at, b.tend_gd, 0, e,
+                                       R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
+                                       sign1v<R, L::W>(at.ld(b.iap, e)));
   V it;
   const V ut = u_tend_at<R, L>(m, b, at, e, s, tge, it);
   if (b.d_pvflx) at.st(b.d_pvflx, e, s);
   if (b.d_utend) at.st(b.d_utend, e, ut);
   if (b.d_tiap) at.st(b.d_tiap, e, it);
-  EdgeSums<D> r{D(0.0), D(0.0)};
+  EdgeSums<D> r{D(0.0), D(0.0), D(0.0)};
   if (b.k >= 1) {
     const double aE = m.aE[e];
-    if (b.k < 4) at.st(b.iap_next, e, at.ld(b.iap_base, e) + (R)b.cdt * it);
-    const V acc = rk_accumulate<R, L>(b, b.acc_iap, at, e, it);
-    if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
-    if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * aE;
-    if (b.k < 4) at.st(b.tcur_iap, e, it);
+    if constexpr ((X & kXExt) != 0) {
+      (void)rk_update_ext<R, L>(b, at, e, it, b.iap_base, b.iap_next, b.acc_iap, b.tprev_iap, b.tfirst_iap,
+                                b.tcur_iap, aE, r.rp, r.rf, r.r44);
+    } else {
+      if (b.k < 4) at.st(b.iap_next, e, at.ld(b.iap_base, e) + (R)b.cdt * it);
+      const V acc = rk_accumulate<R, L>(b, b.acc_iap, at, e, it);
+      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
+      if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * aE;
+      if (b.k < 4) at.st(b.tcur_iap, e, it);
+    }
   }
   return r;
 }
 
 // ---------------------------------------------------------------- kernels
-template <typename R>
+template <typename R, int X>
 __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
   const int e = tile * kBlock + threadIdx.x;
   double en = 0.0;
-  if (e < m.nE) en = edge_prep_e<R, Solo>(m, b, Solo{}, e);
+  if (e < m.nE) en = edge_prep_e<R, Solo, X>(m, b, Solo{}, e);
   if (b.want_inv) {
     const double v[1] = {en};
     const int ids[1] = {kEnergyE};
@@ -575,17 +637,44 @@ __global__ __launch_bounds__(kBlock) void k_edge_prep(DMesh m, StageBufs<R> b) {
   }
 }
 
+// Block partials of the stage's inner products in the kXExt kernels: the
+// descriptor's sums, edge ids + off (0 edges, 1 cells).
 template <typename R>
+__device__ __forceinline__ void rk_sums_ext(const StageBufs<R> &b, double rp, double rf, double r44, int off,
+                                            int tile) {
+  double v[3];
+  int ids[3], n = 0;
+  if (b.rid_prev >= 0) v[n] = rp, ids[n++] = b.rid_prev + off;
+  if (b.rid_first >= 0) v[n] = rf, ids[n++] = b.rid_first + off;
+  if (b.ts_last) v[n] = r44, ids[n++] = kR44e + off;
+  if (n == 1) {
+    const double v1[1] = {v[0]};
+    const int i1[1] = {ids[0]};
+    block_sums<1>(v1, i1, b.part, b.part_stride, tile);
+  } else if (n == 2) {
+    const double v2[2] = {v[0], v[1]};
+    const int i2[2] = {ids[0], ids[1]};
+    block_sums<2>(v2, i2, b.part, b.part_stride, tile);
+  } else if (n == 3) {
+    const double v3[3] = {v[0], v[1], v[2]};
+    const int i3[3] = {ids[0], ids[1], ids[2]};
+    block_sums<3>(v3, i3, b.part, b.part_stride, tile);
+  }
+}
+
+template <typename R, int X>
 __device__ __forceinline__ void cell_part(const DMesh &m, const StageBufs<R> &b, int tile) {
   const int c = tile * kBlock + threadIdx.x;
-  CellSums<double> r{0.0, 0.0, 0.0, 0.0};
-  if (c < m.nC) r = cell_e<R, Solo>(m, b, Solo{}, c);
+  CellSums<double> r{0.0, 0.0, 0.0, 0.0, 0.0};
+  if (c < m.nC) r = cell_e<R, Solo, X>(m, b, Solo{}, c);
   if (b.want_inv) {
     const double v[2] = {r.mass, r.enc};
     const int ids[2] = {kMass, kEnergyC};
     block_sums<2>(v, ids, b.part, b.part_stride, tile);
   }
-  if (b.k >= 2) {
+  if constexpr ((X & kXExt) != 0) {
+    if (b.k >= 1) rk_sums_ext<R>(b, r.rp, r.rf, r.r44, 1, tile);
+  } else if (b.k >= 2) {
     const int rid = b.k == 2 ? kR12c : (b.k == 3 ? kR23c : kR34c);
     if (b.k == 4) {
       const double v[2] = {r.rp, r.r44};
@@ -611,27 +700,29 @@ __device__ __forceinline__ void vertex_part(const DMesh &m, const StageBufs<R> &
   }
 }
 
-template <typename R>
+template <typename R, int X>
 __global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b) {
   if ((int)blockIdx.x < m.nbC)
-    cell_part<R>(m, b, xcd_tile(blockIdx.x, m.nbC, b.rev));
+    cell_part<R, X>(m, b, xcd_tile(blockIdx.x, m.nbC, b.rev));
   else
     vertex_part<R>(m, b, xcd_tile(blockIdx.x - m.nbC, m.nbV, b.rev));
 }
 
-template <typename R>
+template <typename R, int X>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
   const int e = xcd_tile(blockIdx.x, m.nbE, b.rev) * kBlock + threadIdx.x;
-  if (e < m.nE) edge_pv_e<R, Solo>(m, b, Solo{}, e);
+  if (e < m.nE) edge_pv_e<R, Solo, X>(m, b, Solo{}, e);
 }
 
-template <typename R>
+template <typename R, int X>
 __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
   const int e = tile * kBlock + threadIdx.x;
-  EdgeSums<double> r{0.0, 0.0};
-  if (e < m.nE) r = edge_tend_e<R, Solo>(m, b, Solo{}, e);
-  if (b.k >= 2) {
+  EdgeSums<double> r{0.0, 0.0, 0.0};
+  if (e < m.nE) r = edge_tend_e<R, Solo, X>(m, b, Solo{}, e);
+  if constexpr ((X & kXExt) != 0) {
+    if (b.k >= 1) rk_sums_ext<R>(b, r.rp, r.rf, r.r44, 0, tile);
+  } else if (b.k >= 2) {
     const int rid = b.k == 2 ? kR12e : (b.k == 3 ? kR23e : kR34e);
     if (b.k == 4) {
       const double v[2] = {r.rp, r.r44};
@@ -795,6 +886,7 @@ struct StepOut {
   const double *sums;
   double inv_total_area, inv_sum_area_tri;
   int conserve_energy;
+  int time_scheme;  // 1 CRK3, 2 CRK4, 3 PC2
   int have_inv;
   int *hist_idx;      // device step counter of the current batch
   double *beta_hist;  // [cap]
@@ -802,8 +894,8 @@ struct StepOut {
   int hist_cap;
 };
 
-// inner_product_tend (operators_mod.F90:187-194) and the CRK4 beta
-// (time_scheme_mod.F90:140-150).
+// inner_product_tend (operators_mod.F90:187-194) and beta of CRK4
+// (time_scheme_mod.F90:140-150), CRK3 and PC2.
 __device__ __forceinline__ double crk4_beta(const StepOut &o) {
   if (!o.conserve_energy) return 1.0;
   const double *r = o.sums;
@@ -811,6 +903,10 @@ __device__ __forceinline__ double crk4_beta(const StepOut &o) {
   const double R12 = (r[kR12e] + r[kR12c]) * o.inv_total_area;
   const double R23 = (r[kR23e] + r[kR23c]) * o.inv_total_area;
   const double R34 = (r[kR34e] + r[kR34c]) * o.inv_total_area;
+  if (o.time_scheme == 1)  // CRK3 (time_scheme_mod.F90:94-100): R13 is held in the R34 slots
+    return ((2.0 * R12 - R34) + 2.0 * R23) / (3.0 * R44);
+  if (o.time_scheme == 3)  // PC2 (:174-177): ip1 in the R34 slots, ip2 in R44
+    return (R34 != 0.0 && R44 != 0.0) ? R34 / R44 : 1.0;
   return (R12 + R23 + R34) / (3.0 * R44);
 }
 
@@ -881,7 +977,8 @@ template <typename R>
 __global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, StageBufs<R> b, const R *gd, const R *u, R *iap) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nE) return;
-  const R gde = upwind_cell_e<R, Solo>(m, b, Solo{}, gd, 0, e, R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]), R(1));
+  const R gde =
+      upwind_cell_e<R, Solo, kXAdv>(m, b, Solo{}, gd, 0, e, R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]), R(1));
   iap[e] = tm_sqrt(gde) * u[e];
 }
 

@@ -123,6 +123,8 @@ struct tm_ctx {
   bool have_state = false;
   bool timed = false;
   int serpentine = 1;  // alternate the tile sweep direction between kernels
+  int xmask = 0;       // compile-time variant of the stage kernels (kXAdv, kXExt)
+  int nstages = 4;     // stages per step: 4 (CRK4), 3 (CRK3, PC2)
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
   int cl_interval = 1;  // MLP every cl_interval steps
@@ -250,6 +252,31 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.k = k;
   const double cf[5] = {0.0, 0.5, 0.5, 1.0, 0.0};
   b.cdt = cf[k] * c->o.dt;
+  // stage descriptor (kXExt kernels): time_integrate_CRK4/CRK3/PC2,
+  // time_scheme_mod.F90:109-154, 65-107, 156-184
+  const int ts = c->o.time_scheme;
+  b.ts_last = k == c->nstages;
+  b.rid_prev = b.rid_first = -1;
+  b.cprev_dt = 0.0;
+  b.ts_w = 0.0;
+  b.tfirst_gd = P(c->t_gd[1]);  // stage 1 writes slot 1 & 1
+  b.tfirst_iap = P(c->t_iap[1]);
+  if (k >= 1 && ts == 2) {
+    b.ts_acc = k == 1 ? 1 : (k < 4 ? 2 : 3);
+    b.ts_w = 2.0;
+    b.rid_prev = k == 2 ? kR12e : (k == 3 ? kR23e : (k == 4 ? kR34e : -1));
+  } else if (k >= 1 && ts == 1) {  // X2 = X1 + dt/2 T1; X3 = (X1 - dt T1) + 2 dt T2; (T1 + 4 T2 + T3) / 6
+    b.cdt = (k == 1 ? 0.5 : 2.0) * c->o.dt;
+    b.cprev_dt = k == 2 ? -c->o.dt : 0.0;
+    b.ts_acc = k == 1 ? 1 : (k == 2 ? 2 : 3);
+    b.ts_w = 4.0;
+    b.rid_prev = k == 2 ? kR12e : (k == 3 ? kR23e : -1);
+    b.rid_first = k == 3 ? kR34e : -1;  // R13
+  } else if (k >= 1 && ts == 3) {  // X_new = X_old + dt/2 T twice, then T_new; ip1 = T_b.T_new
+    b.cdt = 0.5 * c->o.dt;
+    b.ts_acc = k == 3 ? 1 : 0;
+    b.rid_prev = k == 3 ? kR34e : -1;
+  }
   b.want_inv = want_inv ? 1 : 0;
   b.pv_order = c->o.pv_scheme == 3 ? c->o.interp_pv_order : 2;
   b.pv_scheme = c->o.pv_scheme;
@@ -269,30 +296,55 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   return b;
 }
 
+// Stage part `which` (0 edge prep, 1 cells + vertices, 2 PV on edges, 3 edge
+// tendencies) with the kernels' compile-time variant X (kXAdv, kXExt).  Part 2
+// is k_edge_pv plus, at pv_scheme 1/2, k_pv_cell before it and, at scheme 2,
+// the conservative-APVM kernels after it.
+template <typename R, int X>
+void launch_part_x(tm_ctx *c, int which, const StageBufs<R> &b) {
+  const DMesh &m = c->dm;
+  const int ps = c->o.pv_scheme;
+  switch (which) {
+    case 0:
+      hipLaunchKernelGGL((k_edge_prep<R, X>), dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+      break;
+    case 1:
+      hipLaunchKernelGGL((k_cell_vertex<R, X>), dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
+      break;
+    case 2:
+      if (ps != 3)  // pv_cell (and gd_tend_vertex for scheme 2)
+        hipLaunchKernelGGL(k_pv_cell<R>, dim3(m.nbC + (ps == 2 ? m.nbV : 0)), dim3(kBlock), 0, c->stream, m, b);
+      hipLaunchKernelGGL((k_edge_pv<R, X>), dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+      if (ps == 2) {
+        hipLaunchKernelGGL(k_cvor_flux<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+        hipLaunchKernelGGL(k_cvor_sums<R>, dim3(m.nbV), dim3(kBlock), 0, c->stream, m, b);
+        hipLaunchKernelGGL(k_cvor_eps<R>, dim3(1), dim3(kBlock), 0, c->stream, m, b);
+        hipLaunchKernelGGL(k_cvor_apply<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+      }
+      break;
+    default:
+      hipLaunchKernelGGL((k_edge_tend<R, X>), dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+  }
+}
+
+template <typename R>
+void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
+  switch (c->xmask) {
+    case 0: launch_part_x<R, 0>(c, which, b); break;
+    case kXAdv: launch_part_x<R, kXAdv>(c, which, b); break;
+    case kXExt: launch_part_x<R, kXExt>(c, which, b); break;
+    default: launch_part_x<R, kXAdv | kXExt>(c, which, b);
+  }
+}
+
 // The four kernels sweep forward, backward, forward, backward (see
 // xcd_tile): the tiles each starts on were the last the previous one touched.
 template <typename R>
 int launch_stage(tm_ctx *c, const StageBufs<R> &b0, bool tail = true) {
-  const DMesh &m = c->dm;
   StageBufs<R> b = b0;
-  b.rev = 0;
-  hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-  b.rev = c->serpentine;
-  hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
-  if (tail) {
-    const int ps = c->o.pv_scheme;
-    if (ps != 3)  // pv_cell (and gd_tend_vertex for scheme 2)
-      hipLaunchKernelGGL(k_pv_cell<R>, dim3(m.nbC + (ps == 2 ? m.nbV : 0)), dim3(kBlock), 0, c->stream, m, b);
-    b.rev = 0;
-    hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-    if (ps == 2) {
-      hipLaunchKernelGGL(k_cvor_flux<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-      hipLaunchKernelGGL(k_cvor_sums<R>, dim3(m.nbV), dim3(kBlock), 0, c->stream, m, b);
-      hipLaunchKernelGGL(k_cvor_eps<R>, dim3(1), dim3(kBlock), 0, c->stream, m, b);
-      hipLaunchKernelGGL(k_cvor_apply<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-    }
-    b.rev = c->serpentine;
-    hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
+  for (int which = 0; which < (tail ? 4 : 2); ++which) {
+    b.rev = (which & 1) ? c->serpentine : 0;
+    launch_part<R>(c, which, b);
   }
   HIPCHK(c, hipGetLastError());
   return TM_OK;
@@ -317,6 +369,7 @@ StepOut step_out(tm_ctx *c, bool have_inv) {
   o.inv_total_area = 1.0 / c->totalArea;  // inner_product divides by totalArea
   o.inv_sum_area_tri = 1.0 / c->sumAreaTri;
   o.conserve_energy = c->o.conserve_energy;
+  o.time_scheme = c->o.time_scheme;
   o.have_inv = have_inv ? 1 : 0;
   o.hist_idx = c->hist_idx;
   o.beta_hist = c->beta_hist;
@@ -362,7 +415,7 @@ int enqueue_cl_features(tm_ctx *c, float *X, void *sq) {
 
 template <typename R>
 int enqueue_step(tm_ctx *c, bool infer, int apply_factor) {
-  for (int k = 1; k <= 4; ++k) {
+  for (int k = 1; k <= c->nstages; ++k) {
     StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
     if (k == 1 && c->cl && infer) b.d_vor = (R *)c->cl_vor;  // closure input
     int rc = launch_stage<R>(c, b);
@@ -532,7 +585,8 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
     err = msg;
     return code;
   };
-  if (o.time_scheme != 2) return failv(TM_EINVAL, "only time_scheme = 2 (CRK4) is implemented on the device");
+  if (o.time_scheme < 1 || o.time_scheme > 3) return failv(TM_EINVAL, "time_scheme must be 1 (CRK3), 2 (CRK4) or 3 (PC2)");
+  if (o.time_scheme != 2 && o.ntracers > 0) return failv(TM_EINVAL, "tracers are advanced with time_scheme 2 (CRK4) only");
   if (o.pv_scheme < 1 || o.pv_scheme > 3) return failv(TM_EINVAL, "pv_scheme must be 1, 2 or 3");
   if (o.interp_pv_order < 2 || o.interp_pv_order > 6) return failv(TM_EINVAL, "interp_pv_order must be 2..6");
   if (o.adv_order < 2 || o.adv_order > 6) return failv(TM_EINVAL, "adv_order must be 2..6");
@@ -888,6 +942,9 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
   if (o.adv_order >= 3 && c->nT > 0 && !(c->qe = c->alloc_r((size_t)c->nT * nE)))
     return c->fail(TM_ENOMEM, "hipMalloc tracer edge values");
+  c->xmask = (o.adv_order >= 3 ? kXAdv : 0) |
+             (o.pv_scheme != 3 || o.interp_pv_order >= 5 || o.time_scheme != 2 ? kXExt : 0);
+  c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
   if (o.pv_scheme == 2 &&
       (!(c->tgd_v = c->alloc_r(nV)) || !(c->dpv = c->alloc_r(nE)) || !(c->pvf = c->alloc_r(nE)) ||
@@ -1078,21 +1135,15 @@ static int profile_one_step(tm_ctx *c, std::vector<hipEvent_t> &ev, double *ms, 
   };
   const DMesh &m = c->dm;
   int rc = TM_OK;
-  for (int k = 1; k <= 4; ++k) {
+  for (int k = 1; k <= c->nstages; ++k) {
     StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
-    if ((rc = mark(TM_K_EDGE_PREP))) return rc;
-    b.rev = 0;
-    hipLaunchKernelGGL(k_edge_prep<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-    if ((rc = done()) || (rc = mark(TM_K_CELL_VERTEX))) return rc;
-    b.rev = c->serpentine;
-    hipLaunchKernelGGL(k_cell_vertex<R>, dim3(m.nbC + m.nbV), dim3(kBlock), 0, c->stream, m, b);
-    if ((rc = done()) || (rc = mark(TM_K_EDGE_PV))) return rc;
-    b.rev = 0;
-    hipLaunchKernelGGL(k_edge_pv<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-    if ((rc = done()) || (rc = mark(TM_K_EDGE_TEND))) return rc;
-    b.rev = c->serpentine;
-    hipLaunchKernelGGL(k_edge_tend<R>, dim3(m.nbE), dim3(kBlock), 0, c->stream, m, b);
-    if ((rc = done())) return rc;
+    const int ids[4] = {TM_K_EDGE_PREP, TM_K_CELL_VERTEX, TM_K_EDGE_PV, TM_K_EDGE_TEND};
+    for (int which = 0; which < 4; ++which) {
+      if ((rc = mark(ids[which]))) return rc;
+      b.rev = (which & 1) ? c->serpentine : 0;
+      launch_part<R>(c, which, b);
+      if ((rc = done())) return rc;
+    }
   }
   if ((rc = mark(TM_K_REDUCE))) return rc;
   hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 1, true));
