@@ -30,6 +30,8 @@ extern "C" {
 #define TM_ENOMEM (-6)   /* allocation failed                                */
 #define TM_ENONFINITE (-7) /* NaN/Inf found in a state or invariant          */
 #define TM_ESTATE (-8)   /* call out of order (e.g. step before upload)      */
+#define TM_EIO (-9)      /* file unreadable, not NetCDF classic, or a grid
+                            variable missing / of the wrong shape or type    */
 
 /*
  * MPAS mesh arrays read by the hot path.  Replaces the NetCDF reads of
@@ -84,6 +86,22 @@ int tm_mesh_icos_dims(int level, int32_t *nCells, int32_t *nEdges,
  * with MPAS orderings and TRiSK weightsOnEdge.  No reference counterpart: the
  * reference only reads such meshes (mesh_mod.F90:79-167). */
 int tm_mesh_icos_generate(int level, tm_mesh_desc *m);
+
+/*
+ * MPAS grid files (NetCDF classic CDF-1, CDF-2 and CDF-5; NetCDF-4/HDF5 is not
+ * read), replacing the io_get_dim / io_input calls of mesh_init
+ * (mesh_mod.F90:79-164) without a NetCDF library.
+ * tm_mesh_mpas_dims: dims = {nCells, nEdges, nVertices, maxEdges, maxEdges2,
+ * vertexDegree} and the global attribute sphere_radius (1.0 when absent:
+ * geometry on the unit sphere).
+ * tm_mesh_mpas_read: the arrays mesh_init reads into the caller-allocated
+ * members of m (its dimensions must equal the file's; NULL members skipped).
+ * Integer variables must be NC_INT, real ones NC_DOUBLE or NC_FLOAT (widened).
+ * tm_mesh_mpas_error: text of the last TM_EIO of the calling thread.
+ */
+int tm_mesh_mpas_dims(const char *path, int32_t dims[6], double *sphere_radius);
+int tm_mesh_mpas_read(const char *path, tm_mesh_desc *m);
+const char *tm_mesh_mpas_error(void);
 
 /* poly_fit_run (poly_fit_mod.F90:119-609) on radius-scaled geometry.
  * radius: the scale applied to the unit-sphere mesh (const_mod.F90:11). */

@@ -23,9 +23,8 @@ output_file_prefix = 'test'
 conserve_energy = .true.
 time_scheme     = 2
 pv_scheme       = 3
-adv_order       = 2
 interp_pv_order = 3
-/
+{tmcore}/
 &test_case
 case_name = 'seeded'
 /
@@ -58,7 +57,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
         pytest.skip("Fortran driver not built (amdflang absent when building)")
     level, dt, hours = 4, 600.0, 2
     nml = tmp_path / "namelist.tmcore_sw"
-    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=""))
+    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore=""))
     r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = parse(r.stdout)
@@ -105,7 +104,7 @@ def test_fortran_driver_attaches_closure_file(tmp_path):
             np.concatenate([p, scales]).astype("<f4").tofile(f)
             extra = f"closure_file = '{f}'\n"
         nml = tmp_path / f"nml_{name}"
-        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=extra))
+        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=extra, tmcore=""))
         r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         outs[name] = parse(r.stdout)
@@ -113,3 +112,43 @@ def test_fortran_driver_attaches_closure_file(tmp_path):
     last = max(outs["plain"])
     assert outs["random"][last][1] != outs["plain"][last][1]  # energy changed
     assert abs(outs["random"][last][0] / outs["plain"][last][0] - 1) < 1e-13  # mass kept
+
+
+def test_fortran_driver_reports_an_unreadable_grid_file(tmp_path):
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built")
+    nml = tmp_path / "nml"
+    nml.write_text(NAMELIST.format(level=3, dt=600, hours=1, extra="",
+                                   tmcore=f"mesh_file_path = '{tmp_path / 'absent.grid.nc'}'\n"))
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "tm_mesh_mpas_dims" in r.stdout and "cannot open" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_driver_reads_mpas_grid_file_and_upwind_order(tmp_path):
+    """mesh_file_path (mesh_init, mesh_mod.F90:79-164): a level-4 grid written
+    as an MPAS grid file gives the same invariant lines as the generated mesh;
+    with adv_order = 4 the run matches the oracle at that order."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang absent when building)")
+    level, dt, hours = 4, 600.0, 1
+    m = Mesh(level)
+    grid = tmp_path / "x4.grid.nc"
+    m.to_mpas(grid)
+    outs = {}
+    for name, tm in (("generated", "adv_order = 4\n"),
+                     ("file", f"adv_order = 4\nmesh_file_path = '{grid}'\n")):
+        nml = tmp_path / f"nml_{name}"
+        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore=tm))
+        r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[name] = parse(r.stdout)
+    assert outs["file"] == outs["generated"]
+    nsteps = int(hours * 3600 / dt)
+    gd, u, ghs, q = seeded_state(m, 7, [77])
+    o = Oracle(m, m.fit(), dt=dt, adv_order=4)
+    iap = o.initial_iap(gd, u)
+    _, _, _, inv, beta = o.run(nsteps, gd, iap, ghs, q)
+    for s in range(1, nsteps + 1):
+        np.testing.assert_allclose(outs["file"][s][:3], inv[s - 1, :3], rtol=1e-12)

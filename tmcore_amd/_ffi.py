@@ -24,6 +24,7 @@ TM_ERRORS = {
     -6: "allocation failed",
     -7: "non-finite value",
     -8: "call out of order",
+    -9: "grid file error",
 }
 
 
@@ -87,8 +88,12 @@ def host() -> C.CDLL:
         lib.tm_case_seeded.argtypes = [C.POINTER(MeshDesc), C.c_double, C.c_uint64, C.c_int,
                                        C.POINTER(C.c_uint64), _dp, _dp, _dp, _dp]
         lib.tm_case_steady_geostrophic.argtypes = [C.POINTER(MeshDesc), C.c_double, _dp, _dp, _dp]
+        lib.tm_mesh_mpas_dims.argtypes = [C.c_char_p, _ip, _dp]
+        lib.tm_mesh_mpas_read.argtypes = [C.c_char_p, C.POINTER(MeshDesc)]
+        lib.tm_mesh_mpas_error.restype = C.c_char_p
         for f in (lib.tm_mesh_icos_dims, lib.tm_mesh_icos_generate, lib.tm_poly_fit,
-                  lib.tm_case_seeded, lib.tm_case_steady_geostrophic):
+                  lib.tm_case_seeded, lib.tm_case_steady_geostrophic, lib.tm_mesh_mpas_dims,
+                  lib.tm_mesh_mpas_read):
             f.restype = C.c_int
         _host = lib
     return _host

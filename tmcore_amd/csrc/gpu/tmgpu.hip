@@ -593,8 +593,8 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
   if (o.precision != 64 && o.precision != 32) return failv(TM_EINVAL, "precision must be 64 or 32");
   if (o.ntracers < 0 || o.ntracers > 64) return failv(TM_EINVAL, "ntracers out of range");
   if (!(o.dt > 0.0) || !(o.radius > 0.0)) return failv(TM_EINVAL, "dt and radius must be positive");
-  if (!m || m->nCells <= 0 || m->nEdges <= 0 || m->nVertices <= 0 || m->vertexDegree != 3 || m->maxEdges > 6 ||
-      m->maxEdges < 3 || m->maxEdges2 < 1)
+  if (!m || m->nCells <= 0 || m->nEdges <= 0 || m->nVertices <= 0 || m->vertexDegree != 3 || m->maxEdges < 3 ||
+      m->maxEdges2 < 1)
     return failv(TM_EINVAL, "bad mesh dimensions");
   const bool need_fit = o.pv_scheme == 3 && o.interp_pv_order >= 3;
   if (need_fit && (!f || !f->nFit2Vertices || !f->fit2Vertices || !f->deriv2OnVertex || f->maxFit < 2))
@@ -611,6 +611,7 @@ int validate_mesh(const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts &o,
   for (int i = 0; i < nC; ++i) {
     int n = m->nEdgesOnCell[i];
     if (n < 3 || n > ME) return failv(TM_ERANGE, "nEdgesOnCell out of range");
+    if (n > kMaxEoC) return failv(TM_ESTENCIL, "a cell has more than 6 edges");
     for (int s = 0; s < n; ++s)
       if (!inr(m->edgesOnCell[(size_t)i * ME + s], nE) || !inr(m->verticesOnCell[(size_t)i * ME + s], nV))
         return failv(TM_ERANGE, "edgesOnCell/verticesOnCell index out of range");

@@ -52,6 +52,23 @@ module tm_c_binding_mod
       type(tm_mesh_desc) :: m
     end function tm_mesh_icos_generate
 
+    integer(c_int) function tm_mesh_mpas_dims(path, dims, sphere_radius) bind(c, name='tm_mesh_mpas_dims')
+      import :: c_int, c_int32_t, c_double, c_char
+      character(kind=c_char) :: path(*)
+      integer(c_int32_t) :: dims(6)
+      real(c_double) :: sphere_radius
+    end function tm_mesh_mpas_dims
+
+    integer(c_int) function tm_mesh_mpas_read(path, m) bind(c, name='tm_mesh_mpas_read')
+      import :: c_int, c_char, tm_mesh_desc
+      character(kind=c_char) :: path(*)
+      type(tm_mesh_desc) :: m
+    end function tm_mesh_mpas_read
+
+    type(c_ptr) function tm_mesh_mpas_error() bind(c, name='tm_mesh_mpas_error')
+      import :: c_ptr
+    end function tm_mesh_mpas_error
+
     integer(c_int) function tm_poly_fit(m, radius, f) bind(c, name='tm_poly_fit')
       import :: c_int, c_double, tm_mesh_desc, tm_fit_desc
       type(tm_mesh_desc) :: m
@@ -260,10 +277,15 @@ contains
   function tm_error_text(ctx) result(res)
     type(c_ptr), intent(in) :: ctx
     character(len=:), allocatable :: res
-    type(c_ptr) :: p
+    res = c_text(tm_last_error(ctx))
+  end function tm_error_text
+
+  ! A NUL-terminated C string as a Fortran string ('' for NULL).
+  function c_text(p) result(res)
+    type(c_ptr), intent(in) :: p
+    character(len=:), allocatable :: res
     character(kind=c_char), pointer :: chars(:)
     integer :: n, i
-    p = tm_last_error(ctx)
     if (.not. c_associated(p)) then
       res = ''
       return
@@ -274,6 +296,6 @@ contains
     do i = 1, n
       res(i:i) = chars(i)
     end do
-  end function tm_error_text
+  end function c_text
 
 end module tm_c_binding_mod

@@ -32,6 +32,7 @@ module tm_swm_gpu_mod
 
   ! Mesh (mesh_mod.F90:10-71 names), unit-sphere values as a grid file holds them.
   integer(c_int32_t) :: nCells, nEdges, nVertices
+  integer(c_int32_t) :: maxEdges = 6, maxEdges2 = 12, vertexDegree = 3
   real(c_double), allocatable, target :: latCell(:), lonCell(:), xCell(:), yCell(:), zCell(:)
   real(c_double), allocatable, target :: latEdge(:), lonEdge(:), xEdge(:), yEdge(:), zEdge(:)
   real(c_double), allocatable, target :: latVertex(:), lonVertex(:), xVertex(:), yVertex(:), zVertex(:)
@@ -43,10 +44,13 @@ module tm_swm_gpu_mod
   real(c_double), allocatable, target :: kiteAreasOnVertex(:,:)
   real(c_double), allocatable, target :: dvEdge(:), dv1Edge(:), dv2Edge(:), dcEdge(:), angleEdge(:)
   real(c_double), allocatable, target :: areaCell(:), areaTriangle(:)
-  ! poly_fit_mod.F90:35-57 (second-order vertex fit, slots 0:12)
-  integer, parameter :: max_fit = 13
-  integer(c_int32_t), allocatable, target :: nFit2Vertices(:), fit2Vertices(:,:)
-  real(c_double), allocatable, target :: deriv2OnVertex(:,:,:)
+  ! poly_fit_mod.F90:35-57: the fit families the options read (slots 0:max_fit-1;
+  ! 13 holds the second-order stencils, the fourth-order ones need up to 22)
+  integer :: max_fit = 13
+  integer(c_int32_t), allocatable, target :: nFit2Vertices(:), fit2Vertices(:,:), nFit4Vertices(:), fit4Vertices(:,:)
+  integer(c_int32_t), allocatable, target :: nFit2Cells(:), fit2Cells(:,:), nFit4Cells(:), fit4Cells(:,:)
+  real(c_double), allocatable, target :: deriv2OnVertex(:,:,:), deriv4OnVertex(:,:,:)
+  real(c_double), allocatable, target :: deriv2OnCell(:,:,:), deriv4OnCell(:,:,:)
   ! Prognostics of state(old) and static%cell%ghs
   real(c_double), allocatable, target :: gd(:), u(:), iap_u(:), ghs(:), tracers(:,:)
   real(c_double), allocatable :: run_inv(:,:), run_beta(:)
@@ -69,28 +73,70 @@ contains
     end if
   end subroutine check
 
+  subroutine check_file(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(*), intent(in) :: where
+    if (rc /= TM_OK) then
+      write(*, '(A,I0,A)') '[Error]: ' // where // ' returned ', rc, ': ' // c_text(tm_mesh_mpas_error())
+      stop 1
+    end if
+  end subroutine check_file
+
   subroutine tm_swm_init(namelist_file_path)
 
     character(*), intent(in) :: namelist_file_path
     integer(c_int64_t), allocatable :: tseeds(:)
     integer :: k
+    integer(c_int32_t) :: dims(6)
+    real(c_double) :: sphere_r
 
     call params_parse_namelist(namelist_file_path)
     if (trim(mesh_file_path) /= 'N/A') then
-      write(*, '(A)') '[Notice]: mesh_file_path is ignored; the icosahedral mesh of gpu_control%mesh_level is generated.'
+      ! mesh_init (mesh_mod.F90:79-164): an MPAS grid file on the unit sphere
+      call check_file(tm_mesh_mpas_dims(trim(mesh_file_path) // c_null_char, dims, sphere_r), 'tm_mesh_mpas_dims')
+      if (sphere_r /= 1.0d0) then
+        write(*, '(A)') '[Error]: ' // trim(mesh_file_path) // ' is not on the unit sphere (sphere_radius /= 1)'
+        stop 1
+      end if
+      nCells = dims(1); nEdges = dims(2); nVertices = dims(3)
+      maxEdges = dims(4); maxEdges2 = dims(5); vertexDegree = dims(6)
+      call allocate_mesh()
+      call check_file(tm_mesh_mpas_read(trim(mesh_file_path) // c_null_char, mesh), 'tm_mesh_mpas_read')
+    else
+      call check(tm_mesh_icos_dims(mesh_level, nCells, nEdges, nVertices), 'tm_mesh_icos_dims')
+      call allocate_mesh()
+      call check(tm_mesh_icos_generate(mesh_level, mesh), 'tm_mesh_icos_generate')
     end if
-    call check(tm_mesh_icos_dims(mesh_level, nCells, nEdges, nVertices), 'tm_mesh_icos_dims')
-    call allocate_mesh()
-    call check(tm_mesh_icos_generate(mesh_level, mesh), 'tm_mesh_icos_generate')
 
-    allocate(nFit2Vertices(nVertices), fit2Vertices(0:max_fit-1, nVertices), deriv2OnVertex(0:max_fit-1, 2, nEdges))
+    ! fit families: vertex order 2 (pv_scheme 3, order >= 3), vertex order 4
+    ! (order >= 5), cell orders 2 and 4 (adv_order >= 3, >= 5)
+    if (adv_order >= 5 .or. (pv_scheme == 3 .and. interp_pv_order >= 5)) max_fit = 31
     fit%maxFit = max_fit
     fit%nFit2Cells = c_null_ptr;    fit%fit2Cells = c_null_ptr;    fit%deriv2OnCell = c_null_ptr
     fit%nFit4Cells = c_null_ptr;    fit%fit4Cells = c_null_ptr;    fit%deriv4OnCell = c_null_ptr
     fit%nFit4Vertices = c_null_ptr; fit%fit4Vertices = c_null_ptr; fit%deriv4OnVertex = c_null_ptr
+    allocate(nFit2Vertices(nVertices), fit2Vertices(0:max_fit-1, nVertices), deriv2OnVertex(0:max_fit-1, 2, nEdges))
     fit%nFit2Vertices = c_loc(nFit2Vertices)
     fit%fit2Vertices = c_loc(fit2Vertices)
     fit%deriv2OnVertex = c_loc(deriv2OnVertex)
+    if (pv_scheme == 3 .and. interp_pv_order >= 5) then
+      allocate(nFit4Vertices(nVertices), fit4Vertices(0:max_fit-1, nVertices), deriv4OnVertex(0:max_fit-1, 2, nEdges))
+      fit%nFit4Vertices = c_loc(nFit4Vertices)
+      fit%fit4Vertices = c_loc(fit4Vertices)
+      fit%deriv4OnVertex = c_loc(deriv4OnVertex)
+    end if
+    if (adv_order >= 3) then
+      allocate(nFit2Cells(nCells), fit2Cells(0:max_fit-1, nCells), deriv2OnCell(0:max_fit-1, 2, nEdges))
+      fit%nFit2Cells = c_loc(nFit2Cells)
+      fit%fit2Cells = c_loc(fit2Cells)
+      fit%deriv2OnCell = c_loc(deriv2OnCell)
+    end if
+    if (adv_order >= 5) then
+      allocate(nFit4Cells(nCells), fit4Cells(0:max_fit-1, nCells), deriv4OnCell(0:max_fit-1, 2, nEdges))
+      fit%nFit4Cells = c_loc(nFit4Cells)
+      fit%fit4Cells = c_loc(fit4Cells)
+      fit%deriv4OnCell = c_loc(deriv4OnCell)
+    end if
     call check(tm_poly_fit(mesh, radius, fit), 'tm_poly_fit')
 
     allocate(gd(nCells), u(nEdges), iap_u(nEdges), ghs(nCells), tracers(nCells, max(ntracers, 1)))
@@ -131,8 +177,13 @@ contains
     call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
     if (trim(closure_file) /= 'N/A') call attach_closure(trim(closure_file))
 
-    write(*, '(A,I0,A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh level ', mesh_level, ': ', nCells, ' cells, ', &
-      nEdges, ' edges, ', nVertices, ' vertices; dt = ', dt, ' s; precision ', precision
+    if (trim(mesh_file_path) /= 'N/A') then
+      write(*, '(A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh ' // trim(mesh_file_path) // ': ', nCells, ' cells, ', &
+        nEdges, ' edges, ', nVertices, ' vertices; dt = ', dt, ' s; precision ', precision
+    else
+      write(*, '(A,I0,A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh level ', mesh_level, ': ', nCells, ' cells, ', &
+        nEdges, ' edges, ', nVertices, ' vertices; dt = ', dt, ' s; precision ', precision
+    end if
 
   end subroutine tm_swm_init
 
@@ -142,13 +193,15 @@ contains
     allocate(latEdge(nEdges), lonEdge(nEdges), xEdge(nEdges), yEdge(nEdges), zEdge(nEdges))
     allocate(latVertex(nVertices), lonVertex(nVertices), xVertex(nVertices), yVertex(nVertices), zVertex(nVertices))
     allocate(nEdgesOnCell(nCells), nEdgesOnEdge(nEdges))
-    allocate(cellsOnCell(6, nCells), edgesOnCell(6, nCells), verticesOnCell(6, nCells))
-    allocate(cellsOnEdge(2, nEdges), verticesOnEdge(2, nEdges), edgesOnEdge(12, nEdges), weightsOnEdge(12, nEdges))
-    allocate(cellsOnVertex(3, nVertices), edgesOnVertex(3, nVertices), kiteAreasOnVertex(3, nVertices))
+    allocate(cellsOnCell(maxEdges, nCells), edgesOnCell(maxEdges, nCells), verticesOnCell(maxEdges, nCells))
+    allocate(cellsOnEdge(2, nEdges), verticesOnEdge(2, nEdges))
+    allocate(edgesOnEdge(maxEdges2, nEdges), weightsOnEdge(maxEdges2, nEdges))
+    allocate(cellsOnVertex(vertexDegree, nVertices), edgesOnVertex(vertexDegree, nVertices))
+    allocate(kiteAreasOnVertex(vertexDegree, nVertices))
     allocate(dvEdge(nEdges), dv1Edge(nEdges), dv2Edge(nEdges), dcEdge(nEdges), angleEdge(nEdges))
     allocate(areaCell(nCells), areaTriangle(nVertices))
     mesh%nCells = nCells; mesh%nEdges = nEdges; mesh%nVertices = nVertices
-    mesh%maxEdges = 6; mesh%maxEdges2 = 12; mesh%vertexDegree = 3
+    mesh%maxEdges = maxEdges; mesh%maxEdges2 = maxEdges2; mesh%vertexDegree = vertexDegree
     mesh%latCell = c_loc(latCell); mesh%lonCell = c_loc(lonCell)
     mesh%xCell = c_loc(xCell); mesh%yCell = c_loc(yCell); mesh%zCell = c_loc(zCell)
     mesh%latEdge = c_loc(latEdge); mesh%lonEdge = c_loc(lonEdge)

@@ -24,8 +24,13 @@ _F64_EDGE = ("latEdge", "lonEdge", "xEdge", "yEdge", "zEdge",
 _F64_VERT = ("latVertex", "lonVertex", "xVertex", "yVertex", "zVertex", "areaTriangle")
 
 
+_I32_2D = {"cellsOnCell": "maxEdges", "edgesOnCell": "maxEdges", "verticesOnCell": "maxEdges",
+           "edgesOnEdge": "maxEdges2", "cellsOnVertex": "vertexDegree", "edgesOnVertex": "vertexDegree"}
+
+
 class Mesh:
-    """An icosahedral Voronoi mesh generated by libtmhost (level 0..11)."""
+    """An icosahedral Voronoi mesh generated by libtmhost (level 0..11), or
+    an MPAS grid file read by it (``Mesh.from_mpas``)."""
 
     def __init__(self, level: int):
         lib = _ffi.host()
@@ -33,9 +38,66 @@ class Mesh:
         _ffi.check(lib.tm_mesh_icos_dims(level, C.byref(nc), C.byref(ne), C.byref(nv)),
                    "tm_mesh_icos_dims")
         self.level = level
-        self.nCells, self.nEdges, self.nVertices = nc.value, ne.value, nv.value
-        self.maxEdges, self.maxEdges2, self.vertexDegree = 6, 12, 3
-        nC, nE, nV = self.nCells, self.nEdges, self.nVertices
+        self._alloc(nc.value, ne.value, nv.value, 6, 12, 3)
+        d = self.desc()
+        _ffi.check(lib.tm_mesh_icos_generate(level, C.byref(d)), "tm_mesh_icos_generate")
+
+    @classmethod
+    def from_mpas(cls, path: str) -> "Mesh":
+        """An MPAS grid file (NetCDF classic) through tm_mesh_mpas_read; the
+        file's sphere_radius is kept as ``sphere_radius`` (geometry is used as
+        stored: unit-sphere files are scaled by the model radius)."""
+        lib = _ffi.host()
+        dims = np.zeros(6, np.int32)
+        r = C.c_double()
+        bpath = str(path).encode()
+        rc = lib.tm_mesh_mpas_dims(bpath, _ffi.iptr(dims), C.byref(r))
+        _ffi.check(rc, "tm_mesh_mpas_dims", (lib.tm_mesh_mpas_error() or b"").decode() if rc else "")
+        self = cls.__new__(cls)
+        self.level = None
+        self.sphere_radius = r.value
+        self._alloc(*[int(x) for x in dims])
+        d = self.desc()
+        rc = lib.tm_mesh_mpas_read(bpath, C.byref(d))
+        _ffi.check(rc, "tm_mesh_mpas_read", (lib.tm_mesh_mpas_error() or b"").decode() if rc else "")
+        return self
+
+    def to_mpas(self, path: str, version: int = 2):
+        """Write the mesh arrays mesh_init reads (mesh_mod.F90:79-164) as an
+        MPAS grid file, NetCDF classic (version 1) or 64-bit offset (2), with
+        scipy's NetCDF writer; indexToCellID/EdgeID/VertexID are 1..n."""
+        from scipy.io import netcdf_file
+        with netcdf_file(str(path), "w", version=version) as f:
+            f.on_a_sphere = "YES"
+            f.sphere_radius = 1.0
+            dims = {"nCells": self.nCells, "nEdges": self.nEdges, "nVertices": self.nVertices,
+                    "maxEdges": self.maxEdges, "maxEdges2": self.maxEdges2, "TWO": 2,
+                    "vertexDegree": self.vertexDegree}
+            for k, v in dims.items():
+                f.createDimension(k, v)
+            where = {"Cell": "nCells", "Edge": "nEdges", "Vertex": "nVertices", "Triangle": "nVertices"}
+
+            def put(name, arr, shape):
+                v = f.createVariable(name, "d" if arr.dtype == np.float64 else "i", shape)
+                v[:] = arr
+
+            for k in _F64_CELL + _F64_EDGE + _F64_VERT:
+                put(k, getattr(self, k), (next(d for s, d in where.items() if k.endswith(s)),))
+            for k, n in (("indexToCellID", "nCells"), ("indexToEdgeID", "nEdges"), ("indexToVertexID", "nVertices")):
+                put(k, np.arange(1, dims[n] + 1, dtype=np.int32), (n,))
+            put("nEdgesOnCell", self.nEdgesOnCell, ("nCells",))
+            put("nEdgesOnEdge", self.nEdgesOnEdge, ("nEdges",))
+            for k, slot in _I32_2D.items():
+                put(k, getattr(self, k), ("nCells" if k.endswith("Cell") else
+                                          ("nEdges" if k.endswith("Edge") else "nVertices"), slot))
+            put("cellsOnEdge", self.cellsOnEdge, ("nEdges", "TWO"))
+            put("verticesOnEdge", self.verticesOnEdge, ("nEdges", "TWO"))
+            put("weightsOnEdge", self.weightsOnEdge, ("nEdges", "maxEdges2"))
+            put("kiteAreasOnVertex", self.kiteAreasOnVertex, ("nVertices", "vertexDegree"))
+
+    def _alloc(self, nC, nE, nV, maxEdges, maxEdges2, vertexDegree):
+        self.nCells, self.nEdges, self.nVertices = nC, nE, nV
+        self.maxEdges, self.maxEdges2, self.vertexDegree = maxEdges, maxEdges2, vertexDegree
         for k in _F64_CELL:
             setattr(self, k, np.zeros(nC))
         for k in _F64_EDGE:
@@ -45,18 +107,16 @@ class Mesh:
         i32 = np.int32
         self.nEdgesOnCell = np.zeros(nC, i32)
         self.nEdgesOnEdge = np.zeros(nE, i32)
-        self.cellsOnCell = np.zeros((nC, 6), i32)
-        self.edgesOnCell = np.zeros((nC, 6), i32)
-        self.verticesOnCell = np.zeros((nC, 6), i32)
+        self.cellsOnCell = np.zeros((nC, maxEdges), i32)
+        self.edgesOnCell = np.zeros((nC, maxEdges), i32)
+        self.verticesOnCell = np.zeros((nC, maxEdges), i32)
         self.cellsOnEdge = np.zeros((nE, 2), i32)
         self.verticesOnEdge = np.zeros((nE, 2), i32)
-        self.edgesOnEdge = np.zeros((nE, 12), i32)
-        self.weightsOnEdge = np.zeros((nE, 12))
-        self.cellsOnVertex = np.zeros((nV, 3), i32)
-        self.edgesOnVertex = np.zeros((nV, 3), i32)
-        self.kiteAreasOnVertex = np.zeros((nV, 3))
-        d = self.desc()
-        _ffi.check(lib.tm_mesh_icos_generate(level, C.byref(d)), "tm_mesh_icos_generate")
+        self.edgesOnEdge = np.zeros((nE, maxEdges2), i32)
+        self.weightsOnEdge = np.zeros((nE, maxEdges2))
+        self.cellsOnVertex = np.zeros((nV, vertexDegree), i32)
+        self.edgesOnVertex = np.zeros((nV, vertexDegree), i32)
+        self.kiteAreasOnVertex = np.zeros((nV, vertexDegree))
 
     def desc(self) -> _ffi.MeshDesc:
         d = _ffi.MeshDesc()
