@@ -258,6 +258,10 @@ int tm_closure_attach(tm_ctx *ctx, tm_mlp *mlp);
  *               b - Plain^n(a) defines it).
  * Default 1 step (MLP and correction every step). */
 int tm_closure_set_interval(tm_ctx *ctx, int nsteps, int lumped);
+/* Factor (>= 0, default 1) on every correction the attached closure applies:
+ * the deployment's damping of a closure whose full correction, fitted to
+ * short-horizon targets, feeds enstrophy into long runs. */
+int tm_closure_set_gain(tm_ctx *ctx, double gain);
 /* Adds one training sample held on the device: the inputs of the context's
  * current state (normalised with mlp's scales at this call) and its one-step
  * targets in reference order, along the mesh normal,

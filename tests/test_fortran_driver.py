@@ -93,7 +93,11 @@ def test_fortran_driver_attaches_closure_file(tmp_path):
     level, dt, hours = 3, 600.0, 1
     rng = np.random.default_rng(3)
     params = (rng.standard_normal(NPARAMS) * 0.05).astype("<f4")
-    scales = np.concatenate([np.zeros(40), np.ones(40), [1e-6, 1e6]]).astype("<f4")
+    # inputs (u, gd_edge, vorticity, divergence; 10 slots each) normalised to O(1),
+    # outputs small: a velocity tendency of ~1e-6 m s^-2 and an edge flux of ~1
+    shift = np.concatenate([np.zeros(10), np.full(10, 4.9e4), np.zeros(20)])
+    scale = np.repeat([1 / 20.0, 1 / 5e3, 1e5, 1e6], 10)
+    scales = np.concatenate([shift, scale, [1e-6, 1.0]]).astype("<f4")
     zero = params.copy()
     zero[-(128 * 2 + 2):] = 0.0
     outs = {}

@@ -234,6 +234,41 @@ def test_closure_interval_holds_the_correction():
     mlp.close()
 
 
+def test_closure_gain_scales_the_correction():
+    """set_gain(0) is the plain step bit for bit; set_gain(0.5) applies half
+    the oracle's correction; a negative gain is refused."""
+    m, g, mlp, (gd, u, ghs), (sh, sc, out) = setup_closure(3, s_out=(2e-5, 1e-1))
+    p = mlp.params.astype(np.float64)
+    dt = 600.0
+    o = Oracle(m, m.fit(), dt=dt)
+    iap = o.initial_iap(gd, u)
+    tg, _, d = o.stage(gd, iap, ghs)
+    xr = mo.features(m, d["u"], d["gd_edge"], d["vor"], mo.divergence(m, d["u"], RADIUS), sh, sc)
+    Yr, _ = mo.forward(p, xr, bf=True)
+    gd_p, iap_p, _, _, _ = o.run(1, gd, iap, ghs)
+    gd_h, iap_h = mo.apply(m, gd_p, iap_p, d["gd_edge"], Yr, out, 0.5 * dt, RADIUS)
+    clo = Closure(g, mlp)
+    clo.attach()
+    clo.set_gain(0.0)
+    print("closure gain 0 / 0.5, L3", flush=True)
+    g.step(1)
+    g0 = SwmGpu(m, m.fit(), dt=dt)
+    g0.upload(gd, ghs, u=u)
+    g0.step(1)
+    np.testing.assert_array_equal(g.download()["gd"], g0.download()["gd"])
+    g.upload(gd, ghs, u=u)
+    clo.set_gain(0.5)
+    g.step(1)
+    st = g.download()
+    assert rel_l2(st["gd"] - gd_p, gd_h - gd_p) <= 5e-3
+    assert rel_l2(st["iap_u"] - iap_p, iap_h - iap_p) <= 5e-3
+    with pytest.raises(Exception):
+        clo.set_gain(-1.0)
+    g.close()
+    g0.close()
+    mlp.close()
+
+
 def test_closure_training_reduces_loss_and_eval_matches_oracle():
     """Targets from a fixed 'teacher' network on the same inputs; the student
     trained by tm_closure_train on the device must fit them."""

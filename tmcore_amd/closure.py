@@ -46,6 +46,7 @@ def _lib():
         L.tm_closure_features.argtypes = [_vp, _vp, _fp]
         L.tm_closure_attach.argtypes = [_vp, _vp]
         L.tm_closure_set_interval.argtypes = [_vp, C.c_int, C.c_int]
+        L.tm_closure_set_gain.argtypes = [_vp, C.c_double]
         L.tm_closure_dataset_add.argtypes = [_vp, _vp, _dp, _dp, C.c_double]
         L.tm_closure_dataset_size.argtypes = [_vp, C.POINTER(C.c_int)]
         L.tm_closure_dataset_clear.argtypes = [_vp]
@@ -173,6 +174,10 @@ class Closure:
         """Run the MLP every nsteps steps; add its correction after every step
         (lumped False) or once per interval, times nsteps (lumped True)."""
         self.g._check(_lib().tm_closure_set_interval(self.g.h, nsteps, int(lumped)), "tm_closure_set_interval")
+
+    def set_gain(self, gain: float):
+        """Factor on every applied correction (default 1)."""
+        self.g._check(_lib().tm_closure_set_gain(self.g.h, gain), "tm_closure_set_gain")
 
     def add_sample(self, t_u, t_gd, tg_scale: float):
         """Training sample: inputs of the current state plus targets

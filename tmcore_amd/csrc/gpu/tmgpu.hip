@@ -129,6 +129,7 @@ struct tm_ctx {
   tm_mlp *cl = nullptr;
   int cl_interval = 1;  // MLP every cl_interval steps
   int cl_lumped = 0;    // 0: correction added every step; 1: once per interval, times the interval
+  double cl_gain = 1.0; // factor on the applied correction
   long long cl_step = 0;  // steps since the state or the closure changed
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
   void *cl_sq = nullptr, *cl_vor = nullptr, *cl_div = nullptr;
@@ -435,7 +436,7 @@ int enqueue_step(tm_ctx *c, bool infer, int apply_factor) {
     a.Y = c->cl_Y;
     a.sq = (const R *)c->cl_sq;
     a.norm = mlp_state(c->cl).norm;
-    a.dt = c->o.dt * apply_factor;
+    a.dt = c->o.dt * apply_factor * c->cl_gain;
     a.iap = (R *)c->iap_old;
     a.gd = (R *)c->gd_old;
     hipLaunchKernelGGL(k_cl_apply<R>, dim3(c->dm.nbE + c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, a);
@@ -1327,6 +1328,15 @@ extern "C" int tm_closure_attach(tm_ctx *c, tm_mlp *mlp) {
   drop_graphs(c);
   c->cl = mlp;
   c->cl_step = 0;
+  return TM_OK;
+}
+
+extern "C" int tm_closure_set_gain(tm_ctx *c, double gain) {
+  if (!c) return TM_EINVAL;
+  if (!(gain >= 0.0) || !std::isfinite(gain)) return c->fail(TM_EINVAL, "closure gain must be finite and >= 0");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // the applying graphs bake the gain into their time step
+  c->cl_gain = gain;
   return TM_OK;
 }
 

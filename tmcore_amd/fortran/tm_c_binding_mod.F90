@@ -264,6 +264,12 @@ module tm_c_binding_mod
       integer(c_int), value :: nsteps, lumped
     end function tm_closure_set_interval
 
+    integer(c_int) function tm_closure_set_gain(ctx, gain) bind(c, name='tm_closure_set_gain')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: gain
+    end function tm_closure_set_gain
+
     integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
       import :: c_size_t, c_ptr
       type(c_ptr), value :: s

@@ -69,9 +69,11 @@ module tm_params_mod
 
   integer :: closure_interval = 1    ! steps between MLP evaluations
   integer :: closure_lumped = 0      ! 0: correction after every step; 1: once per interval
+  real(8) :: closure_gain = 1.0d0    ! factor on the applied correction
 
   namelist /gpu_control/ mesh_level, precision, ntracers, seed, tracer_seed0, device, renumber, print_every, &
-                         closure_file, closure_interval, closure_lumped
+                         closure_file, closure_interval, closure_lumped, &
+                         closure_gain
 
 contains
 

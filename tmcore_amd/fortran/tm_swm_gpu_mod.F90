@@ -283,6 +283,7 @@ contains
                                  w(nparams + 81:nparams + 82)), 'tm_mlp_set_scales')
     call check(tm_closure_attach(ctx, mlp), 'tm_closure_attach')
     call check(tm_closure_set_interval(ctx, closure_interval, closure_lumped), 'tm_closure_set_interval')
+    call check(tm_closure_set_gain(ctx, closure_gain), 'tm_closure_set_gain')
     write(*, '(A)') '[Notice]: learned closure attached from ' // path
 
   end subroutine attach_closure
