@@ -48,6 +48,34 @@ def cpu_baseline(mesh, fit, state, dt, budget_s=15.0):
             "s_per_step": el / n}
 
 
+PROFILE_MD = os.path.join(ROOT, "profiles", "trisk_step_L8_fp64_rocprof.md")
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (scaled by the copy-probe calibration
+    the summary states), or None when the summary has no row for it."""
+    import re
+    try:
+        text = open(PROFILE_MD).read()
+    except OSError:
+        return None
+    in_table = False
+    for line in text.splitlines():
+        if "HBM MB / launch" in line:
+            in_table = True
+            continue
+        if in_table and not line.startswith("|"):
+            in_table = False
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if in_table and len(cells) == 5 and re.sub(r"<.*", "", cells[0]) == kernel:
+            try:
+                return float(cells[4]) * 1e6
+            except ValueError:
+                return None
+    return None
+
+
 def ensemble_bench(days=2.0):
     from tmcore_amd.ensemble_bench import config4
     return config4(days)
@@ -183,6 +211,7 @@ def main():
         cpu = cpu_baseline(mesh, fit, (gd, u, ghs, q), args.dt)
         log(f"CPU baseline: {cpu}")
 
+    traffic = profiled_traffic(dom) if args.level == 8 and args.ntracers == 4 else None
     out = {
         "metric": "sim-days/hour at 655k cells: FP64 vs FP32 vs closure at equal L2 height error",
         "value": round(sim_days_h, 3),
@@ -201,7 +230,8 @@ def main():
                                f"dt={args.dt:g} s, {args.ntracers} tracers, pv_scheme 3 order 3, FP64",
                    "cells": mesh.nCells, "dt_s": args.dt, "ntracers": args.ntracers, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": os.path.relpath(PROFILE_MD, ROOT) if traffic else None,
                      "bytes_per_launch": kbytes, "avg_launch_us": round(avg_ms * 1e3, 2)},
         "hbm_copy_probe_GBps": round(copy_gbs, 1),
         "step_algorithmic_GBps": round(step_gbs, 1),

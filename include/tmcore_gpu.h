@@ -121,6 +121,18 @@ int tm_op_u_tend(tm_ctx *ctx, const double *u_edge, const double *ke_cell, const
                  const double *gd_edge, const double *ghs_cell, const double *pv_flx_edge,
                  const double *gd_tend_edge, double *u_tend_edge, double *iap_u_tend_edge);
 
+/* History fields of the current state, in reference order (any pointer may be
+ * NULL): the refresh of tmcore_swm_mod.F90:78-84 (u on edges, pv and vor on
+ * vertices), mpas_reconstruct_1d (mpas_vector_reconstruction.F90:195-247) of
+ * u into zonal and meridional cell winds, h = (gd + ghs)/g and
+ * div_operator(u) (history_mod.F90:125,166-168).  tm_history_setup uploads
+ * the reconstruction coefficients of tm_reconstruct_coeffs
+ * ([nCells][maxEdges][3], reference order) with latCell/lonCell; it is
+ * needed before zonal or meridional are asked for. */
+int tm_history_setup(tm_ctx *ctx, const double *coeffs, const double *latCell, const double *lonCell);
+int tm_history_fields(tm_ctx *ctx, double *u, double *zonal, double *meridional, double *h, double *pv,
+                      double *vor, double *div);
+
 /* Device timing of the last tm_step/tm_step_async batch, in milliseconds,
  * measured with HIP events on the context's stream. */
 int tm_last_elapsed_ms(tm_ctx *ctx, double *ms);

@@ -103,6 +103,32 @@ int tm_mesh_mpas_dims(const char *path, int32_t dims[6], double *sphere_radius);
 int tm_mesh_mpas_read(const char *path, tm_mesh_desc *m);
 const char *tm_mesh_mpas_error(void);
 
+/*
+ * Coefficients of the MPAS vector reconstruction used by the history output
+ * (mpas_initialize_vectors, mpas_vector_operations.F90:493-557, and
+ * mpas_init_reconstruct, mpas_vector_reconstruction.F90:52-125, with the
+ * RBF solve of mpas_rbf_interpolation.F90:1076-1142), on the radius-scaled
+ * geometry: coeffs[nCells][maxEdges][3] = coeffs_reconstruct(1:3, slot, cell)
+ * (unused slots 0).  A cell vector is sum_slot coeffs * u(edgesOnCell).
+ */
+int tm_reconstruct_coeffs(const tm_mesh_desc *m, double radius, double *coeffs);
+
+/*
+ * History files in the layout of history_mod (history_mod.F90:27-174): a
+ * NetCDF classic (CDF-2) file with the radius-scaled mesh variables and, per
+ * tm_history_write, one record of Time (s), u [nEdges], zonalWind,
+ * merdionalWind (the reference's spelling), h = (gd + ghs)/g, div [nCells],
+ * pv, vor [nVertices], tm and te.  Arrays in reference order.  The file is
+ * complete after every write (numrecs is rewritten).
+ */
+typedef struct tm_history tm_history;
+int tm_history_create(tm_history **h, const char *path, const tm_mesh_desc *m, double radius, double dt,
+                      int time_scheme, const char *case_name);
+int tm_history_write(tm_history *h, double time_seconds, const double *u, const double *zonal,
+                     const double *meridional, const double *h_cell, const double *pv, const double *vor,
+                     const double *div, double total_mass, double total_energy);
+int tm_history_close(tm_history *h);
+
 /* poly_fit_run (poly_fit_mod.F90:119-609) on radius-scaled geometry.
  * radius: the scale applied to the unit-sphere mesh (const_mod.F90:11). */
 int tm_poly_fit(const tm_mesh_desc *m, double radius, tm_fit_desc *f);

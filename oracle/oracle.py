@@ -68,6 +68,8 @@ def lib():
         L.om_gd_tend.argtypes = [vp, _dp, _dp, _dp]
         L.om_u_tend.argtypes = [vp] + [_dp] * 9
         L.om_adv.argtypes = [vp, _dp, _dp, _dp, _dp]
+        L.om_reconstruct_coeffs.argtypes = [C.POINTER(_ffi.MeshDesc), C.c_double, _dp]
+        L.om_reconstruct.argtypes = [C.POINTER(_ffi.MeshDesc), _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -165,3 +167,19 @@ def oracle_poly_fit(mesh, radius=6371220.0, max_fit=31, order4=True):
     if rc != 0:
         raise RuntimeError(f"om_poly_fit failed ({rc})")
     return r
+
+
+def oracle_reconstruct_coeffs(mesh, radius=6371220.0):
+    """om_reconstruct_coeffs: coeffs_reconstruct as [nCells, maxEdges, 3]."""
+    co = np.zeros((mesh.nCells, mesh.maxEdges, 3))
+    md = mesh.desc()
+    lib().om_reconstruct_coeffs(C.byref(md), radius, co.ctypes.data_as(_dp))
+    return co
+
+
+def oracle_reconstruct(mesh, coeffs, u):
+    """om_reconstruct: (uX, uY, uZ, zonal, meridional), each [nCells]."""
+    out = np.zeros((5, mesh.nCells))
+    md = mesh.desc()
+    lib().om_reconstruct(C.byref(md), _p(coeffs), _p(u), out.ctypes.data_as(_dp))
+    return tuple(out)

@@ -38,6 +38,13 @@ int om_set_params(om_ctx *c, const om_params *p);
  * math_inv_matrix (math_mod.F90:52-133); same layout as tm_poly_fit. */
 int om_poly_fit(const tm_mesh_desc *m, double radius, tm_fit_desc *f);
 
+/* MPAS vector reconstruction (reconstruct_oracle.c): the RBF coefficients
+ * coeffs[nCells][maxEdges][3] of mpas_init_reconstruct on radius-scaled
+ * geometry, and mpas_reconstruct_1d of an edge field u into
+ * out[5][nCells] = uX, uY, uZ, zonal, meridional. */
+int om_reconstruct_coeffs(const tm_mesh_desc *m, double radius, double *coeffs);
+int om_reconstruct(const tm_mesh_desc *m, const double *coeffs, const double *u, double *out);
+
 /* Single operators, same argument order as operators_mod.F90. */
 void om_div(const om_ctx *c, const double *f_edge, double *div);                 /* 40-55   */
 void om_curl(const om_ctx *c, const double *f_edge, double *curl);               /* 76-91   */

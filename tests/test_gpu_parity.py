@@ -191,6 +191,41 @@ def test_grid_file_mesh_runs_identically(tmp_path):
     np.testing.assert_array_equal(out[0][1], out[1][1])
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_history_fields_match_oracle(precision, tmp_path):
+    """tm_history_fields after 3 steps: u, pv, vor of the refreshed state
+    (tmcore_swm_mod.F90:78-84), the reconstructed zonal/meridional winds
+    (mpas_reconstruct_1d), h = (gd + ghs)/g and div_operator(u), against the
+    oracle on the same state; then one history record written and read back."""
+    from oracle.oracle import oracle_reconstruct, oracle_reconstruct_coeffs
+    from tmcore_amd.history import History
+    m, o, g, (gd, u, iap, ghs, q) = setup(4, precision=precision)
+    g.history_setup()
+    g.step(3)
+    st = g.download()
+    f = g.history_fields()
+    gd3, iap3 = st["gd"], st["iap_u"]  # the oracle's fields of the device's own state
+    _, _, d = o.stage(gd3, iap3, ghs)
+    _, _, _, zo, me = oracle_reconstruct(m, oracle_reconstruct_coeffs(m), d["u"])
+    ref = {"u": d["u"], "pv": d["pv_vertex"], "vor": d["vor"], "zonal": zo, "meridional": me,
+           "h": (gd3 + ghs) / 9.80616, "div": o.op("div", d["u"], out_where="c")}
+    # FP32: the fields are formed in FP32 from the FP32 state; the curl and the
+    # divergence (small after 3 steps of a non-divergent start) lose more digits
+    tol = {k: 1e-12 for k in ref} if precision == 64 else dict({k: 1e-5 for k in ref}, vor=1e-4, div=5e-2)
+    for k, r in ref.items():
+        err = rel_l2(f[k], r)
+        assert err <= tol[k], (k, err)
+    h = History(tmp_path / "h0.nc", m, dt=600.0)
+    inv = g.invariants()
+    h.write(1800.0, f, inv[0], inv[1])
+    h.close()
+    from scipy.io import netcdf_file
+    with netcdf_file(str(tmp_path / "h0.nc"), "r", mmap=False) as nc:
+        np.testing.assert_array_equal(nc.variables["zonalWind"][0], f["zonal"])
+        assert nc.variables["tm"][0] == inv[0]
+    g.close()
+
+
 def test_single_operators_match_oracle():
     m, o, g, (gd, u, iap, ghs, q) = setup(4)
     rng = np.random.default_rng(7)

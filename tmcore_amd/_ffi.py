@@ -91,9 +91,14 @@ def host() -> C.CDLL:
         lib.tm_mesh_mpas_dims.argtypes = [C.c_char_p, _ip, _dp]
         lib.tm_mesh_mpas_read.argtypes = [C.c_char_p, C.POINTER(MeshDesc)]
         lib.tm_mesh_mpas_error.restype = C.c_char_p
+        lib.tm_reconstruct_coeffs.argtypes = [C.POINTER(MeshDesc), C.c_double, _dp]
+        lib.tm_history_create.argtypes = [C.POINTER(C.c_void_p), C.c_char_p, C.POINTER(MeshDesc), C.c_double,
+                                          C.c_double, C.c_int, C.c_char_p]
+        lib.tm_history_write.argtypes = [C.c_void_p, C.c_double] + [_dp] * 7 + [C.c_double, C.c_double]
+        lib.tm_history_close.argtypes = [C.c_void_p]
         for f in (lib.tm_mesh_icos_dims, lib.tm_mesh_icos_generate, lib.tm_poly_fit,
                   lib.tm_case_seeded, lib.tm_case_steady_geostrophic, lib.tm_mesh_mpas_dims,
-                  lib.tm_mesh_mpas_read):
+                  lib.tm_mesh_mpas_read, lib.tm_reconstruct_coeffs):
             f.restype = C.c_int
         _host = lib
     return _host

@@ -41,7 +41,7 @@ def _run(cmd, cwd=None):
 
 def build_host():
     os.makedirs(LIB, exist_ok=True)
-    srcs = [os.path.join(CSRC, "host", f) for f in ("mesh_icos.cpp", "mpas_grid.cpp", "poly_fit.cpp", "cases.cpp")]
+    srcs = [os.path.join(CSRC, "host", f) for f in ("mesh_icos.cpp", "mpas_grid.cpp", "poly_fit.cpp", "reconstruct.cpp", "history.cpp", "cases.cpp")]
     _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-std=c++17", "-Wall", "-I" + INC, *srcs,
           "-o", os.path.join(LIB, "libtmhost.so")])
 

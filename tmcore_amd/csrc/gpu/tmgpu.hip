@@ -87,6 +87,7 @@ struct tm_ctx {
   std::string err;
   hipStream_t stream = nullptr;
   int nC = 0, nE = 0, nV = 0, nT = 0;
+  int maxEdges = 6;  // the caller's edgesOnCell stride
   size_t rsz = 8;  // bytes per state value
   std::vector<int> pC, pE, pV;   // new -> old
   DMesh dm{};
@@ -130,6 +131,11 @@ struct tm_ctx {
   int cl_interval = 1;  // MLP every cl_interval steps
   int cl_lumped = 0;    // 0: correction added every step; 1: once per interval, times the interval
   double cl_gain = 1.0; // factor on the applied correction
+  // history output: reconstruction coefficients [3][6][nC] and cos/sin of
+  // latCell, lonCell [4][nC] (new numbering), uploaded by tm_history_setup
+  double *hist_rc = nullptr, *hist_trig = nullptr;
+  double *hist_out = nullptr;  // [4][nC] zonal, meridional, h, div
+  void *hist_vor = nullptr;    // [nV] vorticity
   long long cl_step = 0;  // steps since the state or the closure changed
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
   void *cl_sq = nullptr, *cl_vor = nullptr, *cl_div = nullptr;
@@ -907,6 +913,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices;
   c->nC = nC;
   c->nE = nE;
+  c->maxEdges = m->maxEdges;
   c->nV = nV;
   c->nT = o.ntracers;
   c->rsz = o.precision == 64 ? 8 : 4;
@@ -1240,6 +1247,135 @@ extern "C" int tm_stage(tm_ctx *c, tm_stage_out *out) {
   if (!c || !out) return TM_EINVAL;
   if (!c->have_state) return c->fail(TM_ESTATE, "tm_stage before tm_state_upload");
   return c->rsz == 8 ? stage_impl<double>(c, out) : stage_impl<float>(c, out);
+}
+
+// ------------------------------------------------------------ history fields
+namespace tmk {
+
+template <typename R>
+struct HistArgs {
+  const R *u, *gd, *ghs;
+  const double *rc, *trig;
+  double inv_g;
+  double *zonal, *merid, *h, *div;  // [nC] each, any may be null
+};
+
+// Per cell: mpas_reconstruct_1d (mpas_vector_reconstruction.F90:195-247) of u,
+// h = (gd + ghs) / g and div_operator(u) (history_mod.F90:125,166-168).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_hist(DMesh m, HistArgs<R> a) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= m.nC) return;
+  double x = 0.0, y = 0.0, z = 0.0, s = 0.0;
+  for (int q = 0; q < m.nEoC[c]; ++q) {
+    const int idx = m.eocS[q * m.nC + c], e = unsign(idx);
+    const double ue = (double)a.u[e];
+    x = x + a.rc[(0 * kMaxEoC + q) * (size_t)m.nC + c] * ue;
+    y = y + a.rc[(1 * kMaxEoC + q) * (size_t)m.nC + c] * ue;
+    z = z + a.rc[(2 * kMaxEoC + q) * (size_t)m.nC + c] * ue;
+    const double t = ue * m.dv[e];
+    s += idx < 0 ? -t : t;
+  }
+  const double clat = a.trig[c], slat = a.trig[m.nC + c], clon = a.trig[2 * (size_t)m.nC + c],
+               slon = a.trig[3 * (size_t)m.nC + c];
+  if (a.zonal) a.zonal[c] = -x * slon + y * clon;
+  if (a.merid) a.merid[c] = -(x * clon + y * slon) * slat + z * clat;
+  if (a.h) a.h[c] = ((double)a.gd[c] + (double)a.ghs[c]) * a.inv_g;
+  if (a.div) a.div[c] = (-s) / m.areaCell[c];
+}
+
+}  // namespace tmk
+
+namespace {
+
+template <typename R>
+int history_impl(tm_ctx *c, double *u, double *zonal, double *merid, double *h, double *pv, double *vor,
+                 double *div) {
+  const int nC = c->nC, nE = c->nE, nV = c->nV;
+  // the refreshed fields of state(old) (tmcore_swm_mod.F90:78-84): a
+  // diagnostic stage writes u, pv_vertex and, here, the vorticity
+  StageBufs<R> b = stage_bufs<R>(c, 0, false);
+  if (!c->hist_vor && (!(c->hist_vor = c->alloc_r(nV)) || !(c->hist_out = c->alloc<double>((size_t)4 * nC))))
+    return c->fail(TM_ENOMEM, "hipMalloc history fields");
+  void *dvor = c->hist_vor, *dout = c->hist_out;
+  b.d_vor = (R *)dvor;
+  int rc = launch_stage<R>(c, b, false);
+  if (!rc) {
+    HistArgs<R> a{};
+    a.u = (const R *)c->u;
+    a.gd = (const R *)c->gd_old;
+    a.ghs = (const R *)c->ghs;
+    a.rc = c->hist_rc;
+    a.trig = c->hist_trig;
+    a.inv_g = 1.0 / 9.80616;  // g of const_mod.F90:12
+    double *o = (double *)dout;
+    a.zonal = o;
+    a.merid = o + nC;
+    a.h = o + 2 * (size_t)nC;
+    a.div = o + 3 * (size_t)nC;
+    hipLaunchKernelGGL(k_hist<R>, dim3((nC + kBlock - 1) / kBlock), dim3(kBlock), 0, c->stream, c->dm, a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) rc = c->fail(TM_EDEVICE, std::string("history kernel: ") + hipGetErrorString(e));
+  }
+  auto cell = [&](double *dst, int k) -> int {
+    if (!dst) return TM_OK;
+    std::vector<double> hbuf(nC);
+    HIPCHK(c, hipMemcpyAsync(hbuf.data(), (double *)dout + (size_t)k * nC, nC * sizeof(double),
+                             hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < nC; ++i) dst[c->pC[i]] = hbuf[i];
+    return TM_OK;
+  };
+  if (!rc) rc = cell(zonal, 0);
+  if (!rc) rc = cell(merid, 1);
+  if (!rc) rc = cell(h, 2);
+  if (!rc) rc = cell(div, 3);
+  if (!rc && u) rc = download_state(c, u, c->u, c->pE, 1, nE);
+  if (!rc && pv) rc = download_state(c, pv, c->pv_v, c->pV, 1, nV);
+  if (!rc && vor) rc = download_state(c, vor, dvor, c->pV, 1, nV);
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int tm_history_setup(tm_ctx *c, const double *coeffs, const double *latCell, const double *lonCell) {
+  if (!c) return TM_EINVAL;
+  if (!coeffs || !latCell || !lonCell) return c->fail(TM_EINVAL, "null history setup input");
+  const int nC = c->nC, ME = c->maxEdges;
+  std::vector<double> rc((size_t)3 * kMaxEoC * nC, 0.0), trig((size_t)4 * nC);
+  for (int i = 0; i < nC; ++i) {
+    const int oc = c->pC[i];
+    for (int q = 0; q < kMaxEoC && q < ME; ++q)
+      for (int d = 0; d < 3; ++d) rc[((size_t)d * kMaxEoC + q) * nC + i] = coeffs[((size_t)oc * ME + q) * 3 + d];
+    trig[i] = std::cos(latCell[oc]);
+    trig[(size_t)nC + i] = std::sin(latCell[oc]);
+    trig[2 * (size_t)nC + i] = std::cos(lonCell[oc]);
+    trig[3 * (size_t)nC + i] = std::sin(lonCell[oc]);
+  }
+  if (!c->hist_rc && (!(c->hist_rc = c->alloc<double>(rc.size())) || !(c->hist_trig = c->alloc<double>(trig.size()))))
+    return c->fail(TM_ENOMEM, "hipMalloc history coefficients");
+  HIPCHK(c, hipMemcpyAsync(c->hist_rc, rc.data(), rc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->hist_trig, trig.data(), trig.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+extern "C" int tm_history_fields(tm_ctx *c, double *u, double *zonal, double *meridional, double *h, double *pv,
+                                 double *vor, double *div) {
+  if (!c) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "tm_history_fields before tm_state_upload");
+  if ((zonal || meridional) && !c->hist_rc) return c->fail(TM_ESTATE, "tm_history_fields before tm_history_setup");
+  if (!c->hist_trig) {  // h and div need no coefficients: zero ones keep the kernel's reads valid
+    std::vector<double> zero((size_t)3 * kMaxEoC * c->nC, 0.0);
+    if (!(c->hist_rc = c->alloc<double>(zero.size())) || !(c->hist_trig = c->alloc<double>((size_t)4 * c->nC)))
+      return c->fail(TM_ENOMEM, "hipMalloc history coefficients");
+    HIPCHK(c, hipMemcpyAsync(c->hist_rc, zero.data(), zero.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->hist_trig, zero.data(), (size_t)4 * c->nC * sizeof(double), hipMemcpyHostToDevice,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+  }
+  return c->rsz == 8 ? history_impl<double>(c, u, zonal, meridional, h, pv, vor, div)
+                     : history_impl<float>(c, u, zonal, meridional, h, pv, vor, div);
 }
 
 // ------------------------------------------------------------ learned closure

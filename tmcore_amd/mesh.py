@@ -129,6 +129,15 @@ class Mesh:
         self._desc_keepalive = d
         return d
 
+    @functools.lru_cache(maxsize=2)
+    def reconstruct_coeffs(self, radius: float = RADIUS) -> np.ndarray:
+        """coeffs_reconstruct of the MPAS vector reconstruction, [nCells, maxEdges, 3]
+        (tm_reconstruct_coeffs)."""
+        co = np.zeros((self.nCells, self.maxEdges, 3))
+        d = self.desc()
+        _ffi.check(_ffi.host().tm_reconstruct_coeffs(C.byref(d), radius, _ffi.dptr(co)), "tm_reconstruct_coeffs")
+        return co
+
     @functools.lru_cache(maxsize=4)
     def fit(self, radius: float = RADIUS, max_fit: int = 13, order4: bool = False) -> "Fit":
         return Fit(self, radius, max_fit, order4)

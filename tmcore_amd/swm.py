@@ -70,6 +70,8 @@ def gpu_lib() -> C.CDLL:
             getattr(L, "tm_op_" + n).argtypes = [_vp, _dp, _dp]
         L.tm_op_tangent_vor_flux.argtypes = [_vp, _dp, _dp, _dp, _dp]
         L.tm_op_upwind_cell.argtypes = [_vp, _dp, _dp, _dp]
+        L.tm_history_setup.argtypes = [_vp, _dp, _dp, _dp]
+        L.tm_history_fields.argtypes = [_vp] + [_dp] * 7
         L.tm_op_pv_vertex.argtypes = [_vp, _dp, _dp, _dp]
         L.tm_op_pv_edge.argtypes = [_vp] + [_dp] * 7
         L.tm_op_u_tend.argtypes = [_vp] + [_dp] * 9
@@ -244,6 +246,23 @@ class SwmGpu:
 
     def calc_tangent_vor_flux(self, u_edge, gd_edge, pv_edge):
         return self._op("tangent_vor_flux", "e", u_edge, gd_edge, pv_edge)
+
+    # -- history output (history_mod.F90:125-174)
+    def history_setup(self, radius: float | None = None):
+        """Upload the MPAS reconstruction coefficients of this mesh."""
+        m = self.mesh
+        co = m.reconstruct_coeffs(self.opts.radius if radius is None else radius)
+        self._check(gpu_lib().tm_history_setup(self.h, _p(co), _p(_f64(m.latCell)), _p(_f64(m.lonCell))),
+                    "tm_history_setup")
+
+    def history_fields(self) -> dict:
+        """u, zonal, meridional, h, pv, vor, div of the current state (reference order)."""
+        out = {k: np.zeros(n) for k, n in (("u", self.nE), ("zonal", self.nC), ("meridional", self.nC),
+                                          ("h", self.nC), ("pv", self.nV), ("vor", self.nV), ("div", self.nC))}
+        self._check(gpu_lib().tm_history_fields(self.h, *[_p(out[k]) for k in ("u", "zonal", "meridional", "h",
+                                                                               "pv", "vor", "div")]),
+                    "tm_history_fields")
+        return out
 
     def calc_pv_on_vertex(self, vor_vertex, gd_vertex):
         return self._op("pv_vertex", "v", vor_vertex, gd_vertex)
