@@ -18,7 +18,7 @@ dt        = {dt}
 /
 &io_control
 output_file_prefix = 'test'
-/
+{io}/
 &tmcore
 conserve_energy = .true.
 time_scheme     = 2
@@ -57,7 +57,7 @@ def test_fortran_driver_matches_oracle(tmp_path):
         pytest.skip("Fortran driver not built (amdflang absent when building)")
     level, dt, hours = 4, 600.0, 2
     nml = tmp_path / "namelist.tmcore_sw"
-    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore=""))
+    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore="", io=""))
     r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = parse(r.stdout)
@@ -108,7 +108,7 @@ def test_fortran_driver_attaches_closure_file(tmp_path):
             np.concatenate([p, scales]).astype("<f4").tofile(f)
             extra = f"closure_file = '{f}'\n"
         nml = tmp_path / f"nml_{name}"
-        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=extra, tmcore=""))
+        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra=extra, tmcore="", io=""))
         r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         outs[name] = parse(r.stdout)
@@ -122,7 +122,7 @@ def test_fortran_driver_reports_an_unreadable_grid_file(tmp_path):
     if not os.path.exists(EXE):
         pytest.skip("Fortran driver not built")
     nml = tmp_path / "nml"
-    nml.write_text(NAMELIST.format(level=3, dt=600, hours=1, extra="",
+    nml.write_text(NAMELIST.format(level=3, dt=600, hours=1, extra="", io="",
                                    tmcore=f"mesh_file_path = '{tmp_path / 'absent.grid.nc'}'\n"))
     r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
@@ -144,7 +144,7 @@ def test_fortran_driver_reads_mpas_grid_file_and_upwind_order(tmp_path):
     for name, tm in (("generated", "adv_order = 4\n"),
                      ("file", f"adv_order = 4\nmesh_file_path = '{grid}'\n")):
         nml = tmp_path / f"nml_{name}"
-        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore=tm))
+        nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore=tm, io=""))
         r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
         assert r.returncode == 0, r.stdout + r.stderr
         outs[name] = parse(r.stdout)
@@ -156,3 +156,36 @@ def test_fortran_driver_reads_mpas_grid_file_and_upwind_order(tmp_path):
     _, _, _, inv, beta = o.run(nsteps, gd, iap, ghs, q)
     for s in range(1, nsteps + 1):
         np.testing.assert_allclose(outs["file"][s][:3], inv[s - 1, :3], rtol=1e-12)
+
+
+@pytest.mark.gpu
+def test_fortran_driver_writes_history(tmp_path):
+    """history_interval = '1 hours' (history_mod.F90): records at 0, 1 and 2 h
+    whose tm equals the driver's total_mass lines and whose fields equal
+    tm_history_fields of the same run through the Python binding."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang absent when building)")
+    from scipy.io import netcdf_file
+    from tmcore_amd.swm import SwmGpu
+    level, dt, hours = 3, 600.0, 2
+    nml = tmp_path / "nml"
+    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="", tmcore="",
+                                   io="history_interval = '1 hours'\n"))
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = parse(r.stdout)
+    m = Mesh(level)
+    gd, u, ghs, q = seeded_state(m, 7, [77])
+    g = SwmGpu(m, m.fit(), dt=dt, ntracers=1)
+    g.upload(gd, ghs, u=u, tracers=q)
+    g.history_setup()
+    g.step(12)
+    f = g.history_fields()
+    g.close()
+    with netcdf_file(str(tmp_path / "test.h0.nc"), "r", mmap=False) as nc:
+        np.testing.assert_array_equal(nc.variables["Time"][:], [0.0, 3600.0, 7200.0])
+        for i, step in enumerate((0, 6, 12)):
+            assert nc.variables["tm"][i] == rows[step][0]
+        for k, v in (("u", "u"), ("zonal", "zonalWind"), ("meridional", "merdionalWind"), ("h", "h"),
+                     ("pv", "pv"), ("vor", "vor"), ("div", "div")):
+            np.testing.assert_array_equal(nc.variables[v][2], f[k], err_msg=v)

@@ -264,6 +264,50 @@ module tm_c_binding_mod
       integer(c_int), value :: nsteps, lumped
     end function tm_closure_set_interval
 
+    ! history output (include/tmcore_host.h, include/tmcore_gpu.h)
+    integer(c_int) function tm_reconstruct_coeffs(m, radius, coeffs) bind(c, name='tm_reconstruct_coeffs')
+      import :: c_int, c_double, tm_mesh_desc
+      type(tm_mesh_desc) :: m
+      real(c_double), value :: radius
+      real(c_double) :: coeffs(*)
+    end function tm_reconstruct_coeffs
+
+    integer(c_int) function tm_history_setup(ctx, coeffs, latCell, lonCell) bind(c, name='tm_history_setup')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: coeffs(*), latCell(*), lonCell(*)
+    end function tm_history_setup
+
+    integer(c_int) function tm_history_fields(ctx, u, zonal, meridional, h, pv, vor, div) &
+        bind(c, name='tm_history_fields')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double) :: u(*), zonal(*), meridional(*), h(*), pv(*), vor(*), div(*)
+    end function tm_history_fields
+
+    integer(c_int) function tm_history_create(hist, path, m, radius, dt, time_scheme, case_name) &
+        bind(c, name='tm_history_create')
+      import :: c_int, c_ptr, c_char, c_double, tm_mesh_desc
+      type(c_ptr) :: hist
+      character(kind=c_char) :: path(*), case_name(*)
+      type(tm_mesh_desc) :: m
+      real(c_double), value :: radius, dt
+      integer(c_int), value :: time_scheme
+    end function tm_history_create
+
+    integer(c_int) function tm_history_write(hist, time_seconds, u, zonal, meridional, h, pv, vor, div, &
+        total_mass, total_energy) bind(c, name='tm_history_write')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: hist
+      real(c_double), value :: time_seconds, total_mass, total_energy
+      real(c_double) :: u(*), zonal(*), meridional(*), h(*), pv(*), vor(*), div(*)
+    end function tm_history_write
+
+    integer(c_int) function tm_history_close(hist) bind(c, name='tm_history_close')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: hist
+    end function tm_history_close
+
     integer(c_int) function tm_closure_set_gain(ctx, gain) bind(c, name='tm_closure_set_gain')
       import :: c_int, c_ptr, c_double
       type(c_ptr), value :: ctx

@@ -60,6 +60,10 @@ module tm_swm_gpu_mod
   type(tm_fit_desc), target :: fit
   type(tm_opts) :: opts
   type(c_ptr) :: ctx = c_null_ptr
+  ! history output (history_mod.F90:27-174)
+  type(c_ptr) :: hist = c_null_ptr
+  integer :: hist_every = 0          ! steps between history records (0: none)
+  real(c_double), allocatable :: h_u(:), h_zonal(:), h_merid(:), h_h(:), h_pv(:), h_vor(:), h_div(:)
   type(c_ptr) :: mlp = c_null_ptr
 
 contains
@@ -175,6 +179,7 @@ contains
     end if
     call check(tm_ctx_create(ctx, mesh, fit, opts), 'tm_ctx_create')
     call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
+    if (trim(history_interval(1)) /= 'N/A') call history_init()
     if (trim(closure_file) /= 'N/A') call attach_closure(trim(closure_file))
 
     if (trim(mesh_file_path) /= 'N/A') then
@@ -237,8 +242,10 @@ contains
       ' total_potential_enstropy=', inv0(3), ' total_absolute_vorticity=', inv0(4)
     done = 0
     run_ms = 0
+    if (c_associated(hist)) call history_write(0)
     do while (done < nsteps)
       n = min(max(print_every, 1), nsteps - done)
+      if (hist_every > 0) n = min(n, hist_every - mod(done, hist_every))
       allocate(inv(4, n), beta(n))
       call check(tm_step(ctx, n, inv, beta), 'tm_step')
       call check(tm_last_elapsed_ms(ctx, ms), 'tm_last_elapsed_ms')
@@ -253,6 +260,9 @@ contains
       end do
       deallocate(inv, beta)
       done = done + n
+      if (hist_every > 0) then
+        if (mod(done, hist_every) == 0) call history_write(done)
+      end if
     end do
     call check(tm_state_download(ctx, c_loc(gd), c_loc(u), c_loc(iap_u), c_loc(tracers)), 'tm_state_download')
     write(*, '(A,I0,A,F12.3,A,F14.3)') '[Notice]: ', nsteps, ' steps, device time ', run_ms, ' ms, sim-days/hour ', &
@@ -288,9 +298,70 @@ contains
 
   end subroutine attach_closure
 
+  ! history_init (history_mod.F90:27-98): the interval '<value> <units>' in
+  ! whole steps, the file <output_file_prefix or case_name>.h0.nc and the
+  ! reconstruction coefficients on the device.
+  subroutine history_init()
+
+    character(max_name_len) :: value_s, units_s
+    real(r8) :: seconds
+    real(c_double), allocatable :: coeffs(:)
+    character(max_file_path_len) :: path
+    integer :: ios
+
+    read(history_interval(1), *, iostat=ios) value_s, units_s
+    if (ios == 0) read(value_s, *, iostat=ios) seconds
+    if (ios /= 0) then
+      write(*, '(A)') '[Error]: invalid history interval ' // trim(history_interval(1))
+      stop 1
+    end if
+    select case (trim(units_s))
+    case ('days')
+      seconds = seconds * 86400
+    case ('hours')
+      seconds = seconds * 3600
+    case ('minutes')
+      seconds = seconds * 60
+    case ('seconds')
+    case default
+      write(*, '(A)') '[Error]: invalid history interval ' // trim(history_interval(1))
+      stop 1
+    end select
+    hist_every = max(1, nint(seconds / dt))
+    if (trim(output_file_prefix) /= 'N/A') then
+      path = trim(output_file_prefix) // '.h0.nc'
+    else
+      path = trim(case_name) // '.h0.nc'
+    end if
+    allocate(coeffs(3 * maxEdges * nCells))
+    call check_file(tm_reconstruct_coeffs(mesh, radius, coeffs), 'tm_reconstruct_coeffs')
+    call check(tm_history_setup(ctx, coeffs, latCell, lonCell), 'tm_history_setup')
+    call check_file(tm_history_create(hist, trim(path) // c_null_char, mesh, radius, dt, int(time_scheme, c_int), &
+                                      trim(case_name) // c_null_char), 'tm_history_create')
+    allocate(h_u(nEdges), h_zonal(nCells), h_merid(nCells), h_h(nCells), h_pv(nVertices), h_vor(nVertices), &
+             h_div(nCells))
+    write(*, '(A,I0,A)') '[Notice]: history every ', hist_every, ' steps to ' // trim(path)
+
+  end subroutine history_init
+
+  ! history_write_state (history_mod.F90:106-172) of state(old) after `step` steps
+  subroutine history_write(step)
+
+    integer, intent(in) :: step
+    real(c_double) :: inv(4)
+
+    call check(tm_history_fields(ctx, h_u, h_zonal, h_merid, h_h, h_pv, h_vor, h_div), 'tm_history_fields')
+    call check(tm_invariants(ctx, inv), 'tm_invariants')
+    call check_file(tm_history_write(hist, step * dt, h_u, h_zonal, h_merid, h_h, h_pv, h_vor, h_div, inv(1), inv(2)), &
+                    'tm_history_write')
+
+  end subroutine history_write
+
   subroutine tm_swm_final()
 
     integer(c_int) :: rc
+    if (c_associated(hist)) rc = tm_history_close(hist)
+    hist = c_null_ptr
     if (c_associated(ctx)) rc = tm_ctx_destroy(ctx)
     ctx = c_null_ptr
     if (c_associated(mlp)) rc = tm_mlp_destroy(mlp)
