@@ -29,23 +29,44 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(mesh, fit, state, dt, budget_s=15.0):
-    """Oracle (single-threaded C, FP64) on a bounded number of steps."""
-    from oracle.oracle import Oracle
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mesh, fit, state, dt, budget_s=12.0):
+    """The FP64 oracle (C restatement of the reference's path, gcc -O2) on a
+    bounded number of steps of the same workload: once on one thread, once on
+    every host thread available to this process (OpenMP over the element
+    loops, at most 16), as the reference ran with GCC auto-parallelisation."""
+    from oracle.oracle import Oracle, set_threads
     gd, u, ghs, q = state
     o = Oracle(mesh, fit, dt=dt)
     iap = o.initial_iap(gd, u)
-    t0 = time.perf_counter()
-    o.run(1, gd, iap, ghs, q)
-    t1 = time.perf_counter() - t0
-    n = int(max(2, min(30, budget_s / max(t1, 1e-3))))
-    t0 = time.perf_counter()
-    o.run(n, gd, iap, ghs, q)
-    el = time.perf_counter() - t0
-    return {"value": (n * dt / 86400.0) / (el / 3600.0), "unit": "sim-days/hour", "cores": 1, "kind": "port",
+    nthreads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    res = {}
+    for threads in (1, nthreads):
+        set_threads(threads)
+        t0 = time.perf_counter()
+        o.run(1, gd, iap, ghs, q)
+        t1 = time.perf_counter() - t0
+        n = int(max(2, min(30, budget_s / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        o.run(n, gd, iap, ghs, q)
+        el = time.perf_counter() - t0
+        res[threads] = (n, el, (n * dt / 86400.0) / (el / 3600.0))
+    n, el, v = res[nthreads]
+    n1, el1, v1 = res[1]
+    return {"value": v, "unit": "sim-days/hour", "cores": nthreads, "kind": "port",
             "sample": f"{n} CRK4 steps (dt={dt:g} s, {q.shape[0]} tracers) of the same level-8 workload, "
-                      f"{el:.1f} s wall, single-threaded gcc -O2 oracle",
-            "s_per_step": el / n}
+                      f"{el:.1f} s wall on {nthreads} threads; single thread: {n1} steps in {el1:.1f} s",
+            "single_thread_value": v1, "s_per_step": el / n, "s_per_step_single_thread": el1 / n1,
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": nthreads}
 
 
 PROFILE_MD = os.path.join(ROOT, "profiles", "trisk_step_L8_fp64_rocprof.md")

@@ -70,6 +70,8 @@ def lib():
         L.om_adv.argtypes = [vp, _dp, _dp, _dp, _dp]
         L.om_reconstruct_coeffs.argtypes = [C.POINTER(_ffi.MeshDesc), C.c_double, _dp]
         L.om_reconstruct.argtypes = [C.POINTER(_ffi.MeshDesc), _dp, _dp, _dp]
+        L.om_set_threads.argtypes = [C.c_int]
+        L.om_set_threads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -183,3 +185,9 @@ def oracle_reconstruct(mesh, coeffs, u):
     md = mesh.desc()
     lib().om_reconstruct(C.byref(md), _p(coeffs), _p(u), out.ctypes.data_as(_dp))
     return tuple(out)
+
+
+def set_threads(n: int) -> int:
+    """Threads of the oracle's element loops (n <= 0 keeps the current
+    setting); returns the number in effect."""
+    return int(lib().om_set_threads(int(n)))

@@ -10,6 +10,26 @@
  */
 #include "trisk_oracle.h"
 
+/* Element loops run over OpenMP threads when the oracle is built with
+ * -fopenmp (each element's arithmetic is unchanged; the global sums stay
+ * serial, in element order). */
+#define OM_FOR _Pragma("omp parallel for schedule(static)")
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Threads of the element loops (1 when built without OpenMP). */
+int om_set_threads(int n) {
+#ifdef _OPENMP
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+#else
+  (void)n;
+  return 1;
+#endif
+}
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -158,6 +178,7 @@ void om_destroy(om_ctx *c) {
 
 /* div_operator, operators_mod.F90:40-55 (returns minus the divergence) */
 void om_div(const om_ctx *c, const double *f_edge, double *div) {
+  OM_FOR
   for (int i = 0; i < c->nC; ++i) {
     double s = 0.0;
     for (int k = 0; k < c->nEdgesOnCell[i]; ++k) {
@@ -166,11 +187,13 @@ void om_div(const om_ctx *c, const double *f_edge, double *div) {
     }
     div[i] = -s;
   }
+  OM_FOR
   for (int i = 0; i < c->nC; ++i) div[i] = div[i] / c->areaCell[i];
 }
 
 /* curl_operator, operators_mod.F90:76-91 */
 void om_curl(const om_ctx *c, const double *f_edge, double *curl) {
+  OM_FOR
   for (int v = 0; v < c->nV; ++v) {
     double s = 0.0;
     for (int k = 0; k < 3; ++k) {
@@ -179,22 +202,26 @@ void om_curl(const om_ctx *c, const double *f_edge, double *curl) {
     }
     curl[v] = -s;
   }
+  OM_FOR
   for (int v = 0; v < c->nV; ++v) curl[v] = curl[v] / c->areaTriangle[v];
 }
 
 /* scalar_c2e_interp_operator, operators_mod.F90:93-104 */
 void om_c2e(const om_ctx *c, const double *f_cell, double *f_edge) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e)
     f_edge[e] = 0.5 * (f_cell[c->cellsOnEdge[2 * e]] + f_cell[c->cellsOnEdge[2 * e + 1]]);
 }
 
 /* scalar_c2v_interp_operator, operators_mod.F90:106-123 */
 void om_c2v(const om_ctx *c, const double *f_cell, double *f_vertex) {
+  OM_FOR
   for (int v = 0; v < c->nV; ++v) {
     double s = 0.0;
     for (int k = 0; k < 3; ++k) s = s + f_cell[c->cellsOnVertex[v * 3 + k]] * c->kiteAreasOnVertex[v * 3 + k];
     f_vertex[v] = s;
   }
+  OM_FOR
   for (int v = 0; v < c->nV; ++v) f_vertex[v] = f_vertex[v] / c->areaTriangle[v];
 }
 
@@ -210,20 +237,24 @@ void om_v2c(const om_ctx *c, const double *f_vertex, double *f_cell) {
 
 /* scalar_v2e_interp_operator, operators_mod.F90:143-156 */
 void om_v2e(const om_ctx *c, const double *f_vertex, double *f_edge) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e)
     f_edge[e] = 0.5 * (f_vertex[c->verticesOnEdge[2 * e]] + f_vertex[c->verticesOnEdge[2 * e + 1]]);
 }
 
 /* iap_swm_operator / inverse, operators_mod.F90:158-176 */
 void om_iap(const om_ctx *c, const double *gd_edge, const double *u, double *iap) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) iap[e] = sqrt(gd_edge[e]) * u[e];
 }
 void om_inverse_iap(const om_ctx *c, const double *gd_edge, const double *iap, double *u) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) u[e] = iap[e] / sqrt(gd_edge[e]);
 }
 
 /* calc_kinetic_energy, operators_mod.F90:206-221 */
 void om_ke(const om_ctx *c, const double *u, double *ke) {
+  OM_FOR
   for (int i = 0; i < c->nC; ++i) {
     double s = 0.0;
     for (int k = 0; k < c->nEdgesOnCell[i]; ++k) {
@@ -232,11 +263,13 @@ void om_ke(const om_ctx *c, const double *u, double *ke) {
     }
     ke[i] = s * 0.25;
   }
+  OM_FOR
   for (int i = 0; i < c->nC; ++i) ke[i] = ke[i] / c->areaCell[i];
 }
 
 /* calc_tangent_wind, operators_mod.F90:223-237 */
 void om_tangent_wind(const om_ctx *c, const double *u, double *v) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) {
     double s = 0.0;
     for (int k = 0; k < c->nEdgesOnEdge[e]; ++k)
@@ -248,6 +281,7 @@ void om_tangent_wind(const om_ctx *c, const double *u, double *v) {
 /* calc_tangent_vor_flux, operators_mod.F90:239-260 */
 void om_tangent_vor_flux(const om_ctx *c, const double *u, const double *gd_edge, const double *pv_edge,
                          double *flux) {
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) {
     double s = 0.0;
     for (int k = 0; k < c->nEdgesOnEdge[e]; ++k) {
@@ -260,6 +294,7 @@ void om_tangent_vor_flux(const om_ctx *c, const double *u, const double *gd_edge
 
 /* calc_pv_on_vertex, operators_mod.F90:262-270 */
 void om_pv_vertex(const om_ctx *c, const double *vor, const double *gd_vertex, double *pv) {
+  OM_FOR
   for (int v = 0; v < c->nV; ++v) pv[v] = (c->fVertex[v] + vor[v]) / gd_vertex[v];
 }
 
@@ -276,6 +311,7 @@ static inline double stencil_d(const double *w, const int *fit, int nfit, const 
 static void pv_edge_upwind(om_ctx *c, const double *v_edge, const double *pv_vertex, double *pv_edge) {
   const int o = c->p.interp_pv_order;
   const size_t MF = c->MF;
+  OM_FOR
   for (int e = 0; e < c->nE; ++e)
     pv_edge[e] = 0.5 * (pv_vertex[c->verticesOnEdge[2 * e]] + pv_vertex[c->verticesOnEdge[2 * e + 1]]);
   double coef2 = 0.0;
@@ -283,6 +319,7 @@ static void pv_edge_upwind(om_ctx *c, const double *v_edge, const double *pv_ver
   if (o >= 4) coef2 = 0.0;
   if (o == 6) c->pv_coef4 = 0.0;
   if (o == 3 || o == 4) {
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) {
       int v1 = c->verticesOnEdge[2 * e], v2 = c->verticesOnEdge[2 * e + 1];
       double d1 = stencil_d(c->deriv2OnVertex + (e * 2 + 0) * MF, c->fit2Vertices + v1 * MF,
@@ -293,6 +330,7 @@ static void pv_edge_upwind(om_ctx *c, const double *v_edge, const double *pv_ver
       pv_edge[e] = pv_edge[e] - (dv * dv) * ((d1 + d2) - (fsign1(v_edge[e]) * coef2) * (d2 - d1)) / 12.0;
     }
   } else if (o == 5 || o == 6) {
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) {
       int v1 = c->verticesOnEdge[2 * e], v2 = c->verticesOnEdge[2 * e + 1];
       double d1 = stencil_d(c->deriv2OnVertex + (e * 2 + 0) * MF, c->fit2Vertices + v1 * MF,
@@ -317,6 +355,7 @@ void om_pv_edge(om_ctx *c, const double *u, const double *v, const double *gd_ed
   if (c->p.pv_scheme == 1) { /* APVM 296-316 */
     om_v2e(c, pv_vertex, pv_edge);
     if (c->p.apvm_weight != 0.0)
+      OM_FOR
       for (int e = 0; e < c->nE; ++e) {
         int v1 = c->verticesOnEdge[2 * e], v2 = c->verticesOnEdge[2 * e + 1];
         int c1 = c->cellsOnEdge[2 * e], c2 = c->cellsOnEdge[2 * e + 1];
@@ -331,6 +370,7 @@ void om_pv_edge(om_ctx *c, const double *u, const double *v, const double *gd_ed
     double *vt = (double *)malloc(c->nV * sizeof(double));
     double *dvt = (double *)malloc(c->nV * sizeof(double));
     om_v2e(c, pv_vertex, pv_edge);
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) {
       int v1 = c->verticesOnEdge[2 * e], v2 = c->verticesOnEdge[2 * e + 1];
       int c1 = c->cellsOnEdge[2 * e], c2 = c->cellsOnEdge[2 * e + 1];
@@ -346,6 +386,7 @@ void om_pv_edge(om_ctx *c, const double *u, const double *v, const double *gd_ed
     for (int k = 0; k < c->nV; ++k) s2 += (pv_vertex[k] * vt[k]) * c->areaTriangle[k];
     for (int k = 0; k < c->nV; ++k) s3 += (pv_vertex[k] * dvt[k]) * c->areaTriangle[k];
     double eps = -(s1 - s2 * 2.0) / ((2.0 * dt) * s3);
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) pv_edge[e] = pv_edge[e] - (eps * dt) * dpv[e];
     free(dpv);
     free(pvf);
@@ -367,6 +408,7 @@ void om_upwind_cell(om_ctx *c, const double *f_cell, const double *u_edge, doubl
   if (o >= 4) coef2 = 0.0;
   if (o == 6) c->adv_coef4 = 0.0;
   if (o == 3 || o == 4) {
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) {
       int c1 = c->cellsOnEdge[2 * e], c2 = c->cellsOnEdge[2 * e + 1];
       double d1 = stencil_d(c->deriv2OnCell + (e * 2 + 0) * MF, c->fit2Cells + c1 * MF, c->nFit2Cells[c1], f_cell, c1);
@@ -375,6 +417,7 @@ void om_upwind_cell(om_ctx *c, const double *f_cell, const double *u_edge, doubl
       f_edge[e] = f_edge[e] - (dc * dc) * ((d1 + d2) - (fsign1(u_edge[e]) * coef2) * (d2 - d1)) / 12.0;
     }
   } else if (o == 5 || o == 6) {
+    OM_FOR
     for (int e = 0; e < c->nE; ++e) {
       int c1 = c->cellsOnEdge[2 * e], c2 = c->cellsOnEdge[2 * e + 1];
       double d1 = stencil_d(c->deriv2OnCell + (e * 2 + 0) * MF, c->fit2Cells + c1 * MF, c->nFit2Cells[c1], f_cell, c1);
@@ -391,6 +434,7 @@ void om_upwind_cell(om_ctx *c, const double *f_cell, const double *u_edge, doubl
 /* calc_gd_tend_on_cell, tmcore_swm_mod.F90:127-139 */
 void om_gd_tend(const om_ctx *c, const double *u, const double *gd_edge, double *tend) {
   double *flux = (double *)malloc(c->nE * sizeof(double));
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) flux[e] = u[e] * gd_edge[e];
   om_div(c, flux, tend);
   free(flux);
@@ -401,7 +445,9 @@ void om_u_tend(const om_ctx *c, const double *u, const double *ke, const double 
                const double *ghs, const double *pv_flx, const double *gd_tend_edge, double *u_tend,
                double *iap_tend) {
   double *E = (double *)malloc(c->nC * sizeof(double));
+  OM_FOR
   for (int i = 0; i < c->nC; ++i) E[i] = (ke[i] + gd[i]) + ghs[i];
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) {
     double sg = sqrt(gd_edge[e]);
     double dEdx = (E[c->cellsOnEdge[2 * e + 1]] - E[c->cellsOnEdge[2 * e]]) / c->dcEdge[e];
@@ -417,6 +463,7 @@ void om_adv(om_ctx *c, const double *f_cell, const double *u, double *f_edge, do
   double *flux = (double *)malloc(c->nE * sizeof(double));
   om_c2e(c, f_cell, f_edge);
   om_upwind_cell(c, f_cell, u, f_edge);
+  OM_FOR
   for (int e = 0; e < c->nE; ++e) flux[e] = u[e] * f_edge[e];
   om_div(c, flux, tend);
   free(flux);

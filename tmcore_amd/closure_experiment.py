@@ -167,6 +167,7 @@ class Evaluator:
         self.closure = Closure(self.corr, self.mlp)
         self.closure.attach()
         self.closure.set_interval(int(self.meta.get("horizon", 1)), lumped=True)
+        self.closure.set_gain(float(self.meta.get("gain", 1.0)))  # the deployment gain stored with the weights
         # finer plain runs: compared with the truth filtered to their own level
         # (fine -> Lr), both then mapped to the coarse grid (Lr -> coarse), so
         # the partition noise of the Lr -> coarse map applies to run and truth alike
