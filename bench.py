@@ -134,7 +134,8 @@ def closure_bench(days=1, seed=6000):
         speed[name] = {"ms_per_step": round(ms / n, 5), "sim_days_per_hour": round(1.0 / (ms / 3.6e6), 1)}
     errs, _ = ev.run_seed(seed, days)
     return {"speed": speed, f"l2_h_rel_day{days}_seed{seed}": {k: v[-1] for k, v in errs.items()},
-            "trained_on": ev.meta["seeds"], "weights": os.path.relpath(CLOSURE_WEIGHTS, ROOT)}
+            "trained_on": ev.meta.get("seeds_all", ev.meta["seeds"]), "deployment_gain": ev.meta.get("gain", 1.0),
+            "weights": os.path.relpath(CLOSURE_WEIGHTS, ROOT)}
 
 
 def main():
@@ -151,7 +152,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=2.0)
     ap.add_argument("--no-closure", action="store_true")
-    ap.add_argument("--closure-days", type=int, default=1)
+    ap.add_argument("--closure-days", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
