@@ -476,10 +476,10 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
 // memory-side cache (measured at level 8: 1.552 -> 1.523 ms per FP64 step).
 template <typename R>
 __device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
-  if constexpr (sizeof(R) == 4) {
-    if (m.d2v32) return __builtin_nontemporal_load(&m.d2v32[i]);
-  }
-  return (R)__builtin_nontemporal_load(&m.d2v[i]);
+  if constexpr (sizeof(R) == 4)
+    return __builtin_nontemporal_load(&m.d2v32[i]);  // every FP32 mesh upload carries the copy
+  else
+    return (R)__builtin_nontemporal_load(&m.d2v[i]);
 }
 
 // PV on the edge (calc_pv_on_edge, operators_mod.F90:273-412) from b.pv_v
