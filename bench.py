@@ -156,7 +156,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the path shards by Hilbert-ordered ranges with point-to-point halos; see the design notes)",
+              "(the path would shard by Hilbert-ordered ranges with point-to-point halo exchange)",
               file=sys.stderr)
         sys.exit(2)
 

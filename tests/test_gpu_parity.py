@@ -1,6 +1,6 @@
 """HIP path (libtmgpu.so) against the oracle on identical seeded inputs.
 
-Bars (stated in the north star): the plain FP64 path agrees with the FP64
+Bars: the plain FP64 path agrees with the FP64
 oracle to <= 1e-12 relative L2 after one step and <= 1e-9 after one simulated
 day, with the same invariant drift.  Stage operators are compared to the same
 1e-12 bar (they are expected to agree to the last bit or two: same operation
