@@ -35,6 +35,17 @@ __device__ __forceinline__ R sign1(R x) {  // Fortran sign(1.0d0, x)
 }
 __device__ __forceinline__ int unsign(int idx) { return idx ^ (idx >> 31); }
 
+// Loads of per-element constants that only one kernel of a stage reads
+// (mesh connectivity and weights streamed once per launch): non-temporal, so
+// they do not evict the fields the stage's kernels hand to each other from
+// the memory-side cache.  Measured at level 8: the vertex fit weights in
+// k_edge_pv 1.552 -> 1.523 ms per FP64 step, the cell/vertex constants of
+// k_cell_vertex 1.524 -> 1.508 ms.
+template <typename T>
+__device__ __forceinline__ T nt_ld(const T *p) {
+  return __builtin_nontemporal_load(p);
+}
+
 template <typename R>
 struct StageBufs {
   const R *gd, *iap, *q;                 // stage input state X_k
@@ -345,7 +356,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
   const int n = m.nEoC[c];
   int idx[kMaxEoC];
 #pragma unroll
-  for (int s = 0; s < kMaxEoC; ++s) idx[s] = m.eocS[s * m.nC + c];
+  for (int s = 0; s < kMaxEoC; ++s) idx[s] = nt_ld(&m.eocS[s * m.nC + c]);
   V fv[kMaxEoC], kv[kMaxEoC];
 #pragma unroll
   for (int s = 0; s < kMaxEoC; ++s) {
@@ -360,7 +371,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
     sdiv += s < n ? f : V(R(0));
     ske += s < n ? kv[s] : V(R(0));
   }
-  const double A = m.areaCell[c];
+  const double A = nt_ld(&m.areaCell[c]);
   const V tend = (-sdiv) / (R)A;
   const V ke = (ske * R(0.25)) / (R)A;
   const V gd = at.ld(b.gd, c);
@@ -392,7 +403,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
 #pragma unroll
     for (int s = 0; s < kMaxEoC; ++s) {
       const int e = unsign(idx[s]);
-      nb[s] = m.cocN[s * m.nC + c];
+      nb[s] = nt_ld(&m.cocN[s * m.nC + c]);
       uv[s] = at.ld(b.u, e);
       dvv[s] = (R)m.dv[e];
     }
@@ -445,15 +456,15 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
   V s = V(R(0));
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
-    const int idx = m.eovS[j * m.nV + v];
+    const int idx = nt_ld(&m.eovS[j * m.nV + v]);
     const V f = at.ld(b.udc, unsign(idx));
     s += idx < 0 ? -f : f;
   }
-  const double At = m.areaTri[v];
+  const double At = nt_ld(&m.areaTri[v]);
   const V vor = (-s) / (R)At;
   V g = V(R(0));
 #pragma unroll
-  for (int j = 0; j < 3; ++j) g = g + at.ld(b.gd, m.cov[j * m.nV + v]) * (R)m.kite[j * m.nV + v];
+  for (int j = 0; j < 3; ++j) g = g + at.ld(b.gd, nt_ld(&m.cov[j * m.nV + v])) * (R)nt_ld(&m.kite[j * m.nV + v]);
   const V gdv = g / (R)At;
   const V pv = pv_vertex_at<R, V>(m, v, vor, gdv);
   at.st(b.pv_v, v, pv);
@@ -469,17 +480,15 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
 }
 
 // Fit weight deriv2OnVertex: FP32 copy in the precision form (a stencil
-// coefficient, not geometry or a TRiSK weight), FP64 otherwise.  The weights
-// are the largest stream of a stage (192 B per edge in FP64) and are read
-// once per stage, so they are loaded non-temporally: they do not push
-// edgesOnEdge/weightsOnEdge, which the next kernel reads again, out of the
-// memory-side cache (measured at level 8: 1.552 -> 1.523 ms per FP64 step).
+// coefficient, not geometry or a TRiSK weight), FP64 otherwise: the largest
+// stream of a stage (192 B per edge in FP64), loaded non-temporally so that
+// edgesOnEdge/weightsOnEdge, which k_edge_tend reads again, stay cached.
 template <typename R>
 __device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
   if constexpr (sizeof(R) == 4)
-    return __builtin_nontemporal_load(&m.d2v32[i]);  // every FP32 mesh upload carries the copy
+    return nt_ld(&m.d2v32[i]);  // every FP32 mesh upload carries the copy
   else
-    return (R)__builtin_nontemporal_load(&m.d2v[i]);
+    return (R)nt_ld(&m.d2v[i]);
 }
 
 // PV on the edge (calc_pv_on_edge, operators_mod.F90:273-412) from b.pv_v
