@@ -205,6 +205,16 @@ struct Solo {
   __device__ __forceinline__ void st(R *p, size_t i, R x) const {
     p[i] = x;
   }
+  // streamed state (the RK base, accumulator and stage tendencies, each read
+  // or written once per launch by one kernel): non-temporal
+  template <typename R>
+  __device__ __forceinline__ R ld_s(const R *p, size_t i) const {
+    return __builtin_nontemporal_load(p + i);
+  }
+  template <typename R>
+  __device__ __forceinline__ void st_s(R *p, size_t i, R x) const {
+    __builtin_nontemporal_store(x, p + i);
+  }
 };
 template <int NW>
 struct Members {
@@ -218,6 +228,14 @@ struct Members {
   template <typename R>
   __device__ __forceinline__ void st(R *p, size_t i, val_t<R, NW> x) const {
     *reinterpret_cast<val_t<R, NW> *>(p + (*this)(i)) = x;
+  }
+  template <typename R>
+  __device__ __forceinline__ val_t<R, NW> ld_s(const R *p, size_t i) const {
+    return ld(p, i);
+  }
+  template <typename R>
+  __device__ __forceinline__ void st_s(R *p, size_t i, val_t<R, NW> x) const {
+    st(p, i, x);
   }
 };
 using Member = Members<1>;
@@ -303,10 +321,10 @@ __device__ __forceinline__ val_t<R, L::W> rk_accumulate(const StageBufs<R> &b, R
   if (b.k == 1)
     acc = t;
   else if (b.k < 4)
-    acc = at.ld(acc_arr, i) + R(2) * t;
+    acc = at.ld_s(acc_arr, i) + R(2) * t;
   else
-    acc = (at.ld(acc_arr, i) + t) / R(6);
-  at.st(acc_arr, i, acc);
+    acc = (at.ld_s(acc_arr, i) + t) / R(6);
+  at.st_s(acc_arr, i, acc);
   return acc;
 }
 
@@ -389,11 +407,11 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
       (void)rk_update_ext<R, L>(b, at, c, tend, b.gd_base, b.gd_next, b.acc_gd, b.tprev_gd, b.tfirst_gd, b.tcur_gd,
                                 A, r.rp, r.rf, r.r44);
     } else {
-      if (b.k < 4) at.st(b.gd_next, c, at.ld(b.gd_base, c) + (R)b.cdt * tend);
+      if (b.k < 4) at.st(b.gd_next, c, at.ld_s(b.gd_base, c) + (R)b.cdt * tend);
       const V acc = rk_accumulate<R, L>(b, b.acc_gd, at, c, tend);
-      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_gd, c)) * to_d<R, L::W>(tend)) * A;
+      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld_s(b.tprev_gd, c)) * to_d<R, L::W>(tend)) * A;
       if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * A;
-      if (b.k < 4) at.st(b.tcur_gd, c, tend);
+      if (b.k < 4) at.st_s(b.tcur_gd, c, tend);
     }
   }
   if (b.nT > 0) {
@@ -429,7 +447,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
       const V tq = (-sq) / (R)A;
       if (b.d_tq) at.st(b.d_tq, tb + c, tq);
       if (b.k >= 1) {
-        if (b.k < 4) at.st(b.q_next, tb + c, at.ld(b.q_base, tb + c) + (R)b.cdt * tq);
+        if (b.k < 4) at.st(b.q_next, tb + c, at.ld_s(b.q_base, tb + c) + (R)b.cdt * tq);
         (void)rk_accumulate<R, L>(b, b.acc_q, at, tb + c, tq);
       }
     }
@@ -626,11 +644,11 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
       (void)rk_update_ext<R, L>(b, at, e, it, b.iap_base, b.iap_next, b.acc_iap, b.tprev_iap, b.tfirst_iap,
                                 b.tcur_iap, aE, r.rp, r.rf, r.r44);
     } else {
-      if (b.k < 4) at.st(b.iap_next, e, at.ld(b.iap_base, e) + (R)b.cdt * it);
+      if (b.k < 4) at.st(b.iap_next, e, at.ld_s(b.iap_base, e) + (R)b.cdt * it);
       const V acc = rk_accumulate<R, L>(b, b.acc_iap, at, e, it);
-      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
+      if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld_s(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
       if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * aE;
-      if (b.k < 4) at.st(b.tcur_iap, e, it);
+      if (b.k < 4) at.st_s(b.tcur_iap, e, it);
     }
   }
   return r;
