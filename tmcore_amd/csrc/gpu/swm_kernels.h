@@ -519,7 +519,7 @@ template <typename R, typename L, int X = 0>
 __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
                                                      val_t<R, L::W> v) {
   typedef val_t<R, L::W> V;
-  const int v1 = m.voe[e], v2 = m.voe[m.nE + e];
+  const int v1 = nt_ld(&m.voe[e]), v2 = nt_ld(&m.voe[m.nE + e]);
   const V p1 = at.ld(b.pv_v, v1), p2 = at.ld(b.pv_v, v2);
   V pve = R(0.5) * (p1 + p2);
   if ((X & kXExt) != 0 && b.pv_scheme != 3) {
@@ -617,7 +617,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   int ep[kMaxEoE];
   V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
 #pragma unroll
-  for (int q = 0; q < kMaxEoE; ++q) ep[q] = m.eoe[q * m.nE + e];
+  for (int q = 0; q < kMaxEoE; ++q) ep[q] = nt_ld(&m.eoe[q * m.nE + e]);
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     uv[q] = at.ld(b.u, ep[q]);
@@ -627,7 +627,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   V s = V(R(0));
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q)
-    s = s + ((((R)m.woe[q * m.nE + e] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
+    s = s + ((((R)nt_ld(&m.woe[q * m.nE + e]) * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V tge = upwind_cell_e<R, L, X>(m, b, at, b.tend_gd, 0, e,
                                        R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
