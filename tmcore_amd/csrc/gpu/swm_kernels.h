@@ -1,6 +1,7 @@
 // Fused TRiSK RK-stage kernels for gfx950, templated on the state type R
-// (double: plain FP64 form; float: precision form with FP64 geometry and
-// weights, and FP64 compensated global sums).
+// (double: plain FP64 form; float: precision form, FP32 stencil arithmetic
+// on FP32 copies of the edge weights and lengths, cell and vertex areas in
+// FP64, and FP64 compensated global sums).
 //
 // One RK stage = four launches:
 //   k_edge_prep     gd_edge, u, and the per-edge products the cell/vertex sums
@@ -44,6 +45,21 @@ __device__ __forceinline__ int unsign(int idx) { return idx ^ (idx >> 31); }
 template <typename T>
 __device__ __forceinline__ T nt_ld(const T *p) {
   return __builtin_nontemporal_load(p);
+}
+
+// Edge geometry and weights read in the arithmetic type R: FP32 copies in
+// FP32 contexts (half the bytes, and the very values a cast of the FP64 array
+// at the point of use gives), the FP64 arrays otherwise.  NT: non-temporal
+// load, for the stream's last reader in a stage.
+template <typename R, bool NT = false>
+__device__ __forceinline__ R geo_at(const double *d64, const float *d32, size_t i) {
+  if constexpr (sizeof(R) == 4) {
+    if constexpr (NT) return nt_ld(&d32[i]);
+    return d32[i];
+  } else {
+    if constexpr (NT) return nt_ld(&d64[i]);
+    return d64[i];
+  }
 }
 
 template <typename R>
@@ -276,7 +292,7 @@ __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const St
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e, m.nC);
   const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c2, 1, e, m.nC);
-  const R dc = (R)m.dc[e];
+  const R dc = geo_at<R>(m.dc, m.dc32, e);
   if (b.adv_order <= 4) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
   const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c1, 0, e, m.nC);
   const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c2, 1, e, m.nC);
@@ -294,12 +310,12 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
   const V gde = upwind_cell_e<R, L, X>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
                                        sign1v<R, L::W>(iap));
   const V u = iap / tm_sqrt<R, L::W>(gde);
-  const R dv = (R)m.dv[e];
+  const R dv = geo_at<R>(m.dv, m.dv32, e);
   at.st(b.gd_e, e, gde);
   at.st(b.u, e, u);
   at.st(b.Fdv, e, (u * gde) * dv);
-  at.st(b.kee, e, (u * u) * (R)m.aE[e]);
-  at.st(b.udc, e, u * (R)m.dc[e]);
+  at.st(b.kee, e, (u * u) * geo_at<R>(m.aE, m.aE32, e));
+  at.st(b.udc, e, u * geo_at<R>(m.dc, m.dc32, e));
   if constexpr ((X & kXAdv) != 0) {
     if (b.qe)  // tracer edge values with the upwind correction (adv_scheme_sg11_mod.F90:42-43)
       for (int t = 0; t < b.nT; ++t) {
@@ -423,7 +439,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
       const int e = unsign(idx[s]);
       nb[s] = nt_ld(&m.cocN[s * m.nC + c]);
       uv[s] = at.ld(b.u, e);
-      dvv[s] = (R)m.dv[e];
+      dvv[s] = geo_at<R>(m.dv, m.dv32, e);
     }
     for (int t = 0; t < b.nT; ++t) {
       // adv_scheme_sg11_run: f_edge = c2e(q), flux = u*f_edge, div_operator
@@ -497,16 +513,12 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
   return r;
 }
 
-// Fit weight deriv2OnVertex: FP32 copy in the precision form (a stencil
-// coefficient, not geometry or a TRiSK weight), FP64 otherwise: the largest
-// stream of a stage (192 B per edge in FP64), loaded non-temporally so that
-// edgesOnEdge/weightsOnEdge, which k_edge_tend reads again, stay cached.
+// Fit weight deriv2OnVertex: the largest stream of a stage (192 B per edge in
+// FP64), loaded non-temporally so that edgesOnEdge/weightsOnEdge, which
+// k_edge_tend reads again, stay cached.
 template <typename R>
 __device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
-  if constexpr (sizeof(R) == 4)
-    return nt_ld(&m.d2v32[i]);  // every FP32 mesh upload carries the copy
-  else
-    return (R)nt_ld(&m.d2v[i]);
+  return geo_at<R, true>(m.d2v, m.d2v32, i);
 }
 
 // PV on the edge (calc_pv_on_edge, operators_mod.F90:273-412) from b.pv_v
@@ -526,8 +538,8 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
     // APVM (:296-316) and conservative APVM (:318-357): v2e(pv) minus a
     // multiple of dpv = v dpv/dt-along + u dpv/dn
     const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-    const V dpv = (v * (p2 - p1)) / (R)m.dv[e] +
-                  (at.ld(b.u, e) * (at.ld(b.pv_c, c2) - at.ld(b.pv_c, c1))) / (R)m.dc[e];
+    const V dpv = (v * (p2 - p1)) / geo_at<R>(m.dv, m.dv32, e) +
+                  (at.ld(b.u, e) * (at.ld(b.pv_c, c2) - at.ld(b.pv_c, c1))) / geo_at<R>(m.dc, m.dc32, e);
     if (b.pv_scheme == 2) {
       at.st(b.dpv, e, dpv);  // eps comes from global sums (k_cvor_*)
       return pve;
@@ -536,11 +548,16 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
   }
   if (b.pv_order >= 3) {
     const R coef2 = b.pv_order == 3 ? R(1) : R(0);
+    // each vertex's stencil is one 48-byte row: three 16-byte loads
+    static_assert(kFit2 % 4 == 0, "fit2Vertices rows are whole int4s");
     int f1[kFit2], f2[kFit2];
+    const int4 *r1 = reinterpret_cast<const int4 *>(m.fit2V) + (size_t)v1 * (kFit2 / 4);
+    const int4 *r2 = reinterpret_cast<const int4 *>(m.fit2V) + (size_t)v2 * (kFit2 / 4);
 #pragma unroll
-    for (int j = 0; j < kFit2; ++j) {
-      f1[j] = m.fit2V[j * m.nV + v1];
-      f2[j] = m.fit2V[j * m.nV + v2];
+    for (int j = 0; j < kFit2 / 4; ++j) {
+      const int4 a1 = r1[j], a2 = r2[j];
+      f1[4 * j] = a1.x, f1[4 * j + 1] = a1.y, f1[4 * j + 2] = a1.z, f1[4 * j + 3] = a1.w;
+      f2[4 * j] = a2.x, f2[4 * j + 1] = a2.y, f2[4 * j + 2] = a2.z, f2[4 * j + 3] = a2.w;
     }
     V q1[kFit2], q2[kFit2];
 #pragma unroll
@@ -554,7 +571,7 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
       d1 += d2v_at<R>(m, (size_t)j * m.nE + e) * (q1[j] - p1);
       d2 += d2v_at<R>(m, (size_t)(kFit2 + j) * m.nE + e) * (q2[j] - p2);
     }
-    const R dv = (R)m.dv[e];
+    const R dv = geo_at<R>(m.dv, m.dv32, e);
     if ((X & kXExt) == 0 || b.pv_order <= 4) {
       pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
     } else {
@@ -581,7 +598,7 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
 #pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * (R)m.woe[s * m.nE + e];
+    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * geo_at<R>(m.woe, m.woe32, (size_t)s * m.nE + e);
   }
   at.st(b.pv_e, e, pv_edge_at<R, L, X>(m, b, at, e, v));
   if (b.d_v) at.st(b.d_v, e, v);
@@ -600,7 +617,7 @@ __device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageB
                                                     val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it) {
   typedef val_t<R, L::W> V;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / (R)m.dc[e];
+  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / geo_at<R>(m.dc, m.dc32, e);
   const V ut = s - dEdx;
   const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
   it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
@@ -627,7 +644,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   V s = V(R(0));
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q)
-    s = s + ((((R)nt_ld(&m.woe[q * m.nE + e]) * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
+    s = s + (((geo_at<R, true>(m.woe, m.woe32, (size_t)q * m.nE + e) * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V tge = upwind_cell_e<R, L, X>(m, b, at, b.tend_gd, 0, e,
                                        R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
@@ -803,7 +820,7 @@ __global__ __launch_bounds__(kBlock) void k_cvor_flux(DMesh m, StageBufs<R> b) {
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     const int ep = m.eoe[q * m.nE + e];
-    const R f = (((R)m.woe[q * m.nE + e] * b.u[ep]) * b.gd_e[ep]) * R(0.5);
+    const R f = ((geo_at<R>(m.woe, m.woe32, (size_t)q * m.nE + e) * b.u[ep]) * b.gd_e[ep]) * R(0.5);
     s = s + f * (p0 + b.pv_e[ep]);
     sd = sd + f * (d0 + b.dpv[ep]);
   }
@@ -822,7 +839,8 @@ __global__ __launch_bounds__(kBlock) void k_cvor_sums(DMesh m, StageBufs<R> b) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int idx = m.eovS[j * m.nV + v], e = unsign(idx);
-      const R f = b.pvf[e] * (R)m.dc[e], fd = b.dpvf[e] * (R)m.dc[e];
+      const R dc = geo_at<R>(m.dc, m.dc32, e);
+      const R f = b.pvf[e] * dc, fd = b.dpvf[e] * dc;
       s += idx < 0 ? -f : f;
       sd += idx < 0 ? -fd : fd;
     }

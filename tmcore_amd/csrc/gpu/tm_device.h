@@ -52,12 +52,18 @@ struct DMesh {
   const double *areaTri;  // [nV]
   const double *fV;       // [nV] 2*Omega*sin(latVertex)
   const int8_t *nFit2V;   // [nV] nFit2Vertices
-  const int *fit2V;       // [12][nV] fit2Vertices slots 1..12
+  const int *fit2V;       // [nV][12] fit2Vertices slots 1..12: vertex-major, since it is
+                          // reached through verticesOnEdge (a gather), not streamed
   const int *fit4V;       // [21][nV] fit4Vertices slots 1..21 (interp_pv_order 5, 6), else null
   const double *d4v;      // [2][21][nE] deriv4OnVertex slots 1..21
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
   const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
   const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))
+  // FP32 copies of the edge geometry and weights that FP32 arithmetic reads
+  // (precision-32 contexts, else null): each element is the round-to-nearest
+  // float of the FP64 one, the value a cast at the point of use gives
+  const float *woe32;     // [10][nE]
+  const float *dv32, *dc32, *aE32;  // [nE]
 };
 
 // Workgroup id -> logical tile so that each XCD (round-robin by workgroup id)

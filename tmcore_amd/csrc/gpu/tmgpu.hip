@@ -782,7 +782,7 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
       const int n = f->nFit2Vertices[ov];
       nFit2V[nv] = (int8_t)n;
       for (int j = 0; j < kFit2; ++j)
-        fit2V[(size_t)j * nV + nv] = j + 1 < n ? md.iV[f->fit2Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
+        fit2V[(size_t)nv * kFit2 + j] = j + 1 < n ? md.iV[f->fit2Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
     }
   }
 
@@ -822,10 +822,15 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   UP(woe, woe, double);
   UP(d2v, d2v, double);
   if (o.precision == 32) {
-    std::vector<float> d2v32(d2v.begin(), d2v.end());
+    std::vector<float> d2v32(d2v.begin(), d2v.end()), woe32(woe.begin(), woe.end());
+    std::vector<float> dv32(dv.begin(), dv.end()), dc32(dc.begin(), dc.end()), aE32(aE.begin(), aE.end());
     UP(d2v32, d2v32, float);
+    UP(woe32, woe32, float);
+    UP(dv32, dv32, float);
+    UP(dc32, dc32, float);
+    UP(aE32, aE32, float);
   } else {
-    d.d2v32 = nullptr;
+    d.d2v32 = d.woe32 = d.dv32 = d.dc32 = d.aE32 = nullptr;
   }
   // cell fits of the upwind correction, new numbering, padded to the device slots
   for (int order = 2; order <= 4; order += 2) {
@@ -971,8 +976,10 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   // Algorithmic bytes (compulsory traffic, actual stencil nnz): constants of
   // one stage + 56 B (FP64) per prognostic element, finalize 24 B per element.
   const double sumEoE = c->md.sumEoE, sumFitE = c->md.sumFitE, sumFitV = c->md.sumFitV, sumEoC = c->md.sumEoC;
-  const double fsz = c->rsz == 4 ? 4.0 : 8.0;  // deriv2OnVertex bytes (FP32 copy in the precision form)
-  const double cst = nE * (8.0 + 8.0 + 8.0 + 8.0) + sumEoE * 12.0 + sumFitE * fsz +
+  // bytes of an edge weight or length read by the stencils (deriv2OnVertex,
+  // weightsOnEdge, dvEdge, dcEdge): their FP32 copies in FP32 contexts
+  const double fsz = c->rsz == 4 ? 4.0 : 8.0;
+  const double cst = nE * (8.0 + 8.0 + 2.0 * fsz) + sumEoE * (4.0 + fsz) + sumFitE * fsz +
                      nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0) + sumFitV * 4.0 + sumEoC * 4.0 + nC * (8.0 + 8.0);
   const double rs = (double)c->rsz;
   const double prog = (double)nE + (double)nC * (1 + c->nT);
@@ -985,9 +992,9 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * 2.0 +
                                 (T > 0 ? 4.0 * sumEoC + nE * (rs + 8.0) : 0.0) + T * nC * 5.0 * rs +
                                 nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
-  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + 8.0 + 2.0 * rs) + 12.0 * sumEoE + fsz * sumFitE +
+  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + fsz + 2.0 * rs) + (4.0 + fsz) * sumEoE + fsz * sumFitE +
                             nV * (rs + 1.0) + 4.0 * sumFitV;
-  c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + 8.0 + 9.0 * rs) + 12.0 * sumEoE + nC * 2.0 * rs;
+  c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + fsz + 9.0 * rs) + (4.0 + fsz) * sumEoE + nC * 2.0 * rs;
   c->kbytes[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
   c->kbytes[TM_K_FINALIZE] = 3.0 * rs * prog;
   return TM_OK;
