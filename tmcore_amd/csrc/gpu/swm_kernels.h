@@ -265,19 +265,26 @@ using Member = Members<1>;
 constexpr int kXAdv = 1, kXExt = 2;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
-// fit stencil of element c (n elements of its kind; f at offset base; padded
-// slots add an exact 0).
+// fit stencil of element c (f at offset base; padded slots add an exact 0).
+// c is itself gathered (an edge's cell or vertex), so its index row is read
+// as whole int4s rather than one slot-major load per slot.
 template <int NS, typename R, typename L>
 __device__ __forceinline__ val_t<R, L::W> fit_sum(const DMesh &m, const R *f, size_t base, L at, const int *fit,
-                                                  const double *w, int c, int side, int e, int n) {
+                                                  const double *w, const float *w32, int c, int side, int e) {
   typedef val_t<R, L::W> V;
+  constexpr int RW = fit_row(NS);
   const V fc = at.ld(f, base + c);
-  int idx[NS];
+  int idx[RW];
+  const int4 *row = reinterpret_cast<const int4 *>(fit) + (size_t)c * (RW / 4);
 #pragma unroll
-  for (int j = 0; j < NS; ++j) idx[j] = fit[j * n + c];
+  for (int j = 0; j < RW / 4; ++j) {
+    const int4 a = row[j];
+    idx[4 * j] = a.x, idx[4 * j + 1] = a.y, idx[4 * j + 2] = a.z, idx[4 * j + 3] = a.w;
+  }
   V s = V(R(0));
 #pragma unroll
-  for (int j = 0; j < NS; ++j) s += (R)w[((size_t)side * NS + j) * m.nE + e] * (at.ld(f, base + idx[j]) - fc);
+  for (int j = 0; j < NS; ++j)
+    s += geo_at<R>(w, w32, ((size_t)side * NS + j) * m.nE + e) * (at.ld(f, base + idx[j]) - fc);
   return s;
 }
 
@@ -290,12 +297,12 @@ __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const St
   if constexpr ((X & kXAdv) == 0) return fe;
   if (b.adv_order < 3) return fe;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c1, 0, e, m.nC);
-  const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, c2, 1, e, m.nC);
+  const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, m.d2c32, c1, 0, e);
+  const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, m.d2c32, c2, 1, e);
   const R dc = geo_at<R>(m.dc, m.dc32, e);
   if (b.adv_order <= 4) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
-  const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c1, 0, e, m.nC);
-  const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, c2, 1, e, m.nC);
+  const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, m.d4c32, c1, 0, e);
+  const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, m.d4c32, c2, 1, e);
   return (fe - ((dc * dc) * (d1 + d2)) / R(12)) +
          (((dc * dc) * (dc * dc)) * ((q1 + q2) - (sgn * (R)b.adv_coef4) * (q2 - q1))) / R(60);
 }
@@ -575,8 +582,8 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
     if ((X & kXExt) == 0 || b.pv_order <= 4) {
       pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
     } else {
-      const V q1 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, v1, 0, e, m.nV);
-      const V q2 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, v2, 1, e, m.nV);
+      const V q1 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, m.d4v32, v1, 0, e);
+      const V q2 = fit_sum<kFit4V, R, L>(m, b.pv_v, 0, at, m.fit4V, m.d4v, m.d4v32, v2, 1, e);
       const R coef4 = b.pv_order == 6 ? R(0) : R(0.25);
       pve = (pve - ((dv * dv) * (d1 + d2)) / R(12)) +
             (((dv * dv) * (dv * dv)) * ((q1 + q2) - (sign1v<R, L::W>(v) * coef4) * (q2 - q1))) / R(60);

@@ -15,6 +15,9 @@ constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding ce
 constexpr int kFit2C = 6;       // cell second-order fit stencil (excluding centre)
 constexpr int kFit4C = 18;      // cell fourth-order fit stencil (excluding centre)
 constexpr int kFit4V = 21;      // vertex fourth-order fit stencil (excluding centre)
+// Row length of a fit-index table (vertex- or cell-major): the stencil's
+// slots rounded up to whole 16-byte int4 loads; extra slots are never summed.
+constexpr int fit_row(int ns) { return (ns + 3) & ~3; }
 constexpr int kNumSums = 13;    // global sums per step (see SumId)
 
 enum SumId {
@@ -42,9 +45,9 @@ struct DMesh {
   const float *d2v32;     // the same in FP32 (precision-32 contexts), else null
   // cell fits of add_upwind_correction_on_cell (adv_order 3..6), else null:
   // unused slots point at the cell itself with weight 0
-  const int *fit2C;       // [6][nC] fit2Cells slots 1..6
+  const int *fit2C;       // [nC][fit_row(6)] fit2Cells slots 1..6
   const double *d2c;      // [2][6][nE] deriv2OnCell slots 1..6
-  const int *fit4C;       // [18][nC] fit4Cells slots 1..18 (adv_order 5, 6)
+  const int *fit4C;       // [nC][fit_row(18)] fit4Cells slots 1..18 (adv_order 5, 6)
   const double *d4c;      // [2][18][nE] deriv4OnCell slots 1..18
   const int *eovS;        // [3][nV] signed edgesOnVertex
   const int *cov;         // [3][nV] cellsOnVertex
@@ -54,7 +57,7 @@ struct DMesh {
   const int8_t *nFit2V;   // [nV] nFit2Vertices
   const int *fit2V;       // [nV][12] fit2Vertices slots 1..12: vertex-major, since it is
                           // reached through verticesOnEdge (a gather), not streamed
-  const int *fit4V;       // [21][nV] fit4Vertices slots 1..21 (interp_pv_order 5, 6), else null
+  const int *fit4V;       // [nV][fit_row(21)] fit4Vertices slots 1..21 (interp_pv_order 5, 6), else null
   const double *d4v;      // [2][21][nE] deriv4OnVertex slots 1..21
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
   const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
@@ -64,6 +67,7 @@ struct DMesh {
   // float of the FP64 one, the value a cast at the point of use gives
   const float *woe32;     // [10][nE]
   const float *dv32, *dc32, *aE32;  // [nE]
+  const float *d2c32, *d4c32, *d4v32;  // FP32 copies of d2c, d4c, d4v where those exist
 };
 
 // Workgroup id -> logical tile so that each XCD (round-robin by workgroup id)

@@ -838,12 +838,13 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     const int ns = order == 2 ? kFit2C : kFit4C;
     const int *nf = order == 2 ? f->nFit2Cells : f->nFit4Cells, *fc = order == 2 ? f->fit2Cells : f->fit4Cells;
     const double *dw = order == 2 ? f->deriv2OnCell : f->deriv4OnCell;
-    std::vector<int> fitc((size_t)ns * nC);
+    const int rw = fit_row(ns);
+    std::vector<int> fitc((size_t)rw * nC);
     std::vector<double> dc((size_t)2 * ns * nE, 0.0);
     for (int ncell = 0; ncell < nC; ++ncell) {
       const int oc = md.pC[ncell], n = nf[oc];
-      for (int j = 0; j < ns; ++j)
-        fitc[(size_t)j * nC + ncell] = j + 1 < n ? md.iC[fc[(size_t)oc * f->maxFit + j + 1] - 1] : ncell;
+      for (int j = 0; j < rw; ++j)
+        fitc[(size_t)ncell * rw + j] = j < ns && j + 1 < n ? md.iC[fc[(size_t)oc * f->maxFit + j + 1] - 1] : ncell;
     }
     for (int ne = 0; ne < nE; ++ne) {
       const int oe = md.pE[ne];
@@ -853,22 +854,28 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
           dc[((size_t)k * ns + j) * nE + ne] = j + 1 < n ? dw[((size_t)oe * 2 + k) * f->maxFit + j + 1] : 0.0;
       }
     }
+    std::vector<float> w32;
+    if (o.precision == 32) w32.assign(dc.begin(), dc.end());
     if (order == 2) {
       UP(fit2C, fitc, int);
       UP(d2c, dc, double);
+      if (o.precision == 32) UP(d2c32, w32, float);
     } else {
       UP(fit4C, fitc, int);
       UP(d4c, dc, double);
+      if (o.precision == 32) UP(d4c32, w32, float);
     }
   }
   // vertex fourth-order fits of the scheme-3 PV (interp_pv_order 5, 6)
   if (o.pv_scheme == 3 && o.interp_pv_order >= 5) {
-    std::vector<int> fitv((size_t)kFit4V * nV);
+    const int rw = fit_row(kFit4V);
+    std::vector<int> fitv((size_t)rw * nV);
     std::vector<double> dv4((size_t)2 * kFit4V * nE, 0.0);
     for (int nv = 0; nv < nV; ++nv) {
       const int ov = md.pV[nv], n = f->nFit4Vertices[ov];
-      for (int j = 0; j < kFit4V; ++j)
-        fitv[(size_t)j * nV + nv] = j + 1 < n ? md.iV[f->fit4Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
+      for (int j = 0; j < rw; ++j)
+        fitv[(size_t)nv * rw + j] =
+            j < kFit4V && j + 1 < n ? md.iV[f->fit4Vertices[(size_t)ov * f->maxFit + j + 1] - 1] : nv;
     }
     for (int ne = 0; ne < nE; ++ne) {
       const int oe = md.pE[ne];
@@ -881,6 +888,10 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     }
     UP(fit4V, fitv, int);
     UP(d4v, dv4, double);
+    if (o.precision == 32) {
+      std::vector<float> w32(dv4.begin(), dv4.end());
+      UP(d4v32, w32, float);
+    }
   }
   UP(eovS, eovS, int);
   UP(cov, cov, int);
