@@ -50,7 +50,7 @@ __device__ __forceinline__ int cl_slot_edge(const DMesh &m, int e, int kap, int 
   const int q = s - 1;
   if (q >= n) return e;
   const int slot = kap > 0 ? q : (q + n1) % n;
-  return m.eoe[slot * m.nE + e];
+  return m.eoe[pair_at(slot, e, m.nE)];
 }
 
 template <typename R>

@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kBlock) void k_op_tangent_wind(DMesh m, const doubl
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nE) return;
   double s = 0.0;
-  for (int q = 0; q < m.nEoE[e]; ++q) s += u[m.eoe[q * m.nE + e]] * m.woe[q * m.nE + e];
+  for (int q = 0; q < m.nEoE[e]; ++q) s += u[m.eoe[pair_at(q, e, m.nE)]] * m.woe[pair_at(q, e, m.nE)];
   out[e] = s;
 }
 
@@ -85,8 +85,8 @@ __global__ __launch_bounds__(kBlock) void k_op_tvf(DMesh m, const double *u, con
   if (e >= m.nE) return;
   double s = 0.0;
   for (int q = 0; q < m.nEoE[e]; ++q) {
-    const int ep = m.eoe[q * m.nE + e];
-    s = s + (((m.woe[q * m.nE + e] * u[ep]) * gde[ep]) * 0.5) * (pve[e] + pve[ep]);
+    const int ep = m.eoe[pair_at(q, e, m.nE)];
+    s = s + (((m.woe[pair_at(q, e, m.nE)] * u[ep]) * gde[ep]) * 0.5) * (pve[e] + pve[ep]);
   }
   out[e] = s;
 }

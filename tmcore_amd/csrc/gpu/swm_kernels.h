@@ -157,6 +157,26 @@ struct val_s<R, 1> {
 template <typename R, int W>
 using val_t = typename val_s<R, W>::type;
 
+// Slot pair p (slots 2p, 2p+1) of element i of a pair-stored table (pair_at).
+template <typename T, bool NT = false>
+__device__ __forceinline__ val_t<T, 2> ld_pair(const T *a, int p, int i, int n) {
+  const val_t<T, 2> *q = reinterpret_cast<const val_t<T, 2> *>(a) + ((size_t)p * n + i);
+  if constexpr (NT) return __builtin_nontemporal_load(q);
+  return *q;
+}
+// The same for an edge weight table read in the arithmetic type R (FP32 copy
+// in FP32 contexts, as geo_at).
+template <typename R, bool NT = false>
+__device__ __forceinline__ void geo_pair(const double *d64, const float *d32, int p, int i, int n, R &x, R &y) {
+  if constexpr (sizeof(R) == 4) {
+    const val_t<float, 2> v = ld_pair<float, NT>(d32, p, i, n);
+    x = v.x, y = v.y;
+  } else {
+    const val_t<double, 2> v = ld_pair<double, NT>(d64, p, i, n);
+    x = v.x, y = v.y;
+  }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ val_t<T, W> tm_sqrt(val_t<T, W> x) {
   if constexpr (W == 1) {
@@ -520,14 +540,6 @@ __device__ __forceinline__ VertSums<val_t<double, L::W>> vertex_e(const DMesh &m
   return r;
 }
 
-// Fit weight deriv2OnVertex: the largest stream of a stage (192 B per edge in
-// FP64), loaded non-temporally so that edgesOnEdge/weightsOnEdge, which
-// k_edge_tend reads again, stay cached.
-template <typename R>
-__device__ __forceinline__ R d2v_at(const DMesh &m, size_t i) {
-  return geo_at<R, true>(m.d2v, m.d2v32, i);
-}
-
 // PV on the edge (calc_pv_on_edge, operators_mod.F90:273-412) from b.pv_v
 // and the tangent wind v: scheme 3 (high-order upwind, :359-412) in full;
 // schemes 1/2 read pv_cell (b.pv_c) and u, and scheme 2 leaves v2e(pv) here
@@ -572,11 +584,20 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
       q1[j] = at.ld(b.pv_v, f1[j]);
       q2[j] = at.ld(b.pv_v, f2[j]);
     }
+    // deriv2OnVertex, the largest stream of a stage (192 B per edge in FP64):
+    // non-temporal, so that edgesOnEdge/weightsOnEdge, which k_edge_tend
+    // reads again, stay cached
+    R w1[kFit2], w2[kFit2];
+#pragma unroll
+    for (int j = 0; j < kFit2 / 2; ++j) {
+      geo_pair<R, true>(m.d2v, m.d2v32, j, e, m.nE, w1[2 * j], w1[2 * j + 1]);
+      geo_pair<R, true>(m.d2v, m.d2v32, kFit2 / 2 + j, e, m.nE, w2[2 * j], w2[2 * j + 1]);
+    }
     V d1 = V(R(0)), d2 = V(R(0));
 #pragma unroll
     for (int j = 0; j < kFit2; ++j) {
-      d1 += d2v_at<R>(m, (size_t)j * m.nE + e) * (q1[j] - p1);
-      d2 += d2v_at<R>(m, (size_t)(kFit2 + j) * m.nE + e) * (q2[j] - p2);
+      d1 += w1[j] * (q1[j] - p1);
+      d2 += w2[j] * (q2[j] - p2);
     }
     const R dv = geo_at<R>(m.dv, m.dv32, e);
     if ((X & kXExt) == 0 || b.pv_order <= 4) {
@@ -600,12 +621,18 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
   {
     int ep[kMaxEoE];
     V uv[kMaxEoE];
+    R w[kMaxEoE];
 #pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) ep[s] = m.eoe[s * m.nE + e];
+    for (int p = 0; p < kMaxEoE / 2; ++p) {
+      const val_t<int, 2> t = ld_pair<int>(m.eoe, p, e, m.nE);
+      ep[2 * p] = t.x, ep[2 * p + 1] = t.y;
+    }
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
 #pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * geo_at<R>(m.woe, m.woe32, (size_t)s * m.nE + e);
+    for (int p = 0; p < kMaxEoE / 2; ++p) geo_pair<R>(m.woe, m.woe32, p, e, m.nE, w[2 * p], w[2 * p + 1]);
+#pragma unroll
+    for (int s = 0; s < kMaxEoE; ++s) v += uv[s] * w[s];
   }
   at.st(b.pv_e, e, pv_edge_at<R, L, X>(m, b, at, e, v));
   if (b.d_v) at.st(b.d_v, e, v);
@@ -640,18 +667,23 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   const V pve = at.ld(b.pv_e, e);
   int ep[kMaxEoE];
   V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
+  R w[kMaxEoE];
 #pragma unroll
-  for (int q = 0; q < kMaxEoE; ++q) ep[q] = nt_ld(&m.eoe[q * m.nE + e]);
+  for (int p = 0; p < kMaxEoE / 2; ++p) {
+    const val_t<int, 2> t = ld_pair<int, true>(m.eoe, p, e, m.nE);
+    ep[2 * p] = t.x, ep[2 * p + 1] = t.y;
+  }
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     uv[q] = at.ld(b.u, ep[q]);
     gv[q] = at.ld(b.gd_e, ep[q]);
     pv[q] = at.ld(b.pv_e, ep[q]);
   }
+#pragma unroll
+  for (int p = 0; p < kMaxEoE / 2; ++p) geo_pair<R, true>(m.woe, m.woe32, p, e, m.nE, w[2 * p], w[2 * p + 1]);
   V s = V(R(0));
 #pragma unroll
-  for (int q = 0; q < kMaxEoE; ++q)
-    s = s + (((geo_at<R, true>(m.woe, m.woe32, (size_t)q * m.nE + e) * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
+  for (int q = 0; q < kMaxEoE; ++q) s = s + (((w[q] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V tge = upwind_cell_e<R, L, X>(m, b, at, b.tend_gd, 0, e,
                                        R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
@@ -826,8 +858,8 @@ __global__ __launch_bounds__(kBlock) void k_cvor_flux(DMesh m, StageBufs<R> b) {
   R s = R(0), sd = R(0);
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
-    const int ep = m.eoe[q * m.nE + e];
-    const R f = ((geo_at<R>(m.woe, m.woe32, (size_t)q * m.nE + e) * b.u[ep]) * b.gd_e[ep]) * R(0.5);
+    const int ep = m.eoe[pair_at(q, e, m.nE)];
+    const R f = ((geo_at<R>(m.woe, m.woe32, pair_at(q, e, m.nE)) * b.u[ep]) * b.gd_e[ep]) * R(0.5);
     s = s + f * (p0 + b.pv_e[ep]);
     sd = sd + f * (d0 + b.dpv[ep]);
   }

@@ -15,6 +15,16 @@ constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding ce
 constexpr int kFit2C = 6;       // cell second-order fit stencil (excluding centre)
 constexpr int kFit4C = 18;      // cell fourth-order fit stencil (excluding centre)
 constexpr int kFit4V = 21;      // vertex fourth-order fit stencil (excluding centre)
+// Streamed slot tables of the edges (edgesOnEdge, weightsOnEdge and their
+// FP32 copy, deriv2OnVertex and its copy) are stored in slot pairs: element
+// (slot, i) of an n-element table sits at pair_at(slot, i, n), so one 8- or
+// 16-byte load per thread reads two slots and a wavefront's load of a pair is
+// still one contiguous block.
+__host__ __device__ inline size_t pair_at(int slot, int i, int n) {
+  return ((size_t)(slot >> 1) * n + i) * 2 + (slot & 1);
+}
+static_assert(kMaxEoE % 2 == 0 && kFit2 % 2 == 0, "slot pairs");
+
 // Row length of a fit-index table (vertex- or cell-major): the stencil's
 // slots rounded up to whole 16-byte int4 loads; extra slots are never summed.
 constexpr int fit_row(int ns) { return (ns + 3) & ~3; }
@@ -39,9 +49,9 @@ struct DMesh {
   const double *dv, *dc, *aE;  // [nE] dvEdge, dcEdge, areaEdge = dv*dc (radius-scaled)
   const int8_t *nEoE;
   const int8_t *kapE;     // [nE] +1 if the normal cellsOnEdge(1)->(2) points east, else -1
-  const int *eoe;         // [10][nE] edgesOnEdge
-  const double *woe;      // [10][nE] weightsOnEdge
-  const double *d2v;      // [2][12][nE] deriv2OnVertex slots 1..12
+  const int *eoe;         // [5][nE][2] edgesOnEdge (slot pairs, pair_at)
+  const double *woe;      // [5][nE][2] weightsOnEdge (slot pairs)
+  const double *d2v;      // [2][6][nE][2] deriv2OnVertex slots 1..12 per side (slot pairs)
   const float *d2v32;     // the same in FP32 (precision-32 contexts), else null
   // cell fits of add_upwind_correction_on_cell (adv_order 3..6), else null:
   // unused slots point at the cell itself with weight 0

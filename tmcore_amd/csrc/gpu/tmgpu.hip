@@ -744,8 +744,8 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     nEoE[ne] = (int8_t)n;
     for (int s = 0; s < kMaxEoE; ++s) {
       // padding slots point at the edge itself with weight 0 (never read)
-      eoe[(size_t)s * nE + ne] = s < n ? md.iE[m->edgesOnEdge[(size_t)oe * ME2 + s] - 1] : ne;
-      woe[(size_t)s * nE + ne] = s < n ? m->weightsOnEdge[(size_t)oe * ME2 + s] : 0.0;
+      eoe[pair_at(s, ne, nE)] = s < n ? md.iE[m->edgesOnEdge[(size_t)oe * ME2 + s] - 1] : ne;
+      woe[pair_at(s, ne, nE)] = s < n ? m->weightsOnEdge[(size_t)oe * ME2 + s] : 0.0;
     }
     if (need_fit)
       for (int k = 0; k < 2; ++k) {
@@ -754,7 +754,7 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
         const int nfit = f->nFit2Vertices[m->verticesOnEdge[2 * (size_t)oe + k] - 1];
         for (int j = 0; j < kFit2; ++j) {
           const int slot = j + 1;
-          d2v[((size_t)k * kFit2 + j) * nE + ne] =
+          d2v[pair_at(k * kFit2 + j, ne, nE)] =
               slot < nfit ? f->deriv2OnVertex[((size_t)oe * 2 + k) * f->maxFit + slot] : 0.0;
         }
       }
