@@ -164,16 +164,31 @@ __device__ __forceinline__ val_t<T, 2> ld_pair(const T *a, int p, int i, int n) 
   if constexpr (NT) return __builtin_nontemporal_load(q);
   return *q;
 }
-// The same for an edge weight table read in the arithmetic type R (FP32 copy
-// in FP32 contexts, as geo_at).
+// Slots 2p, 2p+1 of element i of an edge weight table read in the arithmetic
+// type R: the FP64 table is stored in slot pairs (one 16-byte load), its FP32
+// copy slot-major (two 4-byte loads, which measured faster than one 8-byte
+// pair load).  geo_slot: one slot.
 template <typename R, bool NT = false>
 __device__ __forceinline__ void geo_pair(const double *d64, const float *d32, int p, int i, int n, R &x, R &y) {
   if constexpr (sizeof(R) == 4) {
-    const val_t<float, 2> v = ld_pair<float, NT>(d32, p, i, n);
-    x = v.x, y = v.y;
+    x = geo_at<R, NT>(d64, d32, (size_t)(2 * p) * n + i);
+    y = geo_at<R, NT>(d64, d32, (size_t)(2 * p + 1) * n + i);
   } else {
     const val_t<double, 2> v = ld_pair<double, NT>(d64, p, i, n);
     x = v.x, y = v.y;
+  }
+}
+template <typename R>
+__device__ __forceinline__ R geo_slot(const double *d64, const float *d32, int s, int i, int n) {
+  return geo_at<R>(d64, d32, sizeof(R) == 4 ? (size_t)s * n + i : pair_at(s, i, n));
+}
+// The edgesOnEdge slots of edge e, read in pairs.
+template <bool NT = false>
+__device__ __forceinline__ void eoe_row(const DMesh &m, int e, int (&ep)[kMaxEoE]) {
+#pragma unroll
+  for (int p = 0; p < kMaxEoE / 2; ++p) {
+    const val_t<int, 2> t = ld_pair<int, NT>(m.eoe, p, e, m.nE);
+    ep[2 * p] = t.x, ep[2 * p + 1] = t.y;
   }
 }
 
@@ -622,11 +637,7 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
     int ep[kMaxEoE];
     V uv[kMaxEoE];
     R w[kMaxEoE];
-#pragma unroll
-    for (int p = 0; p < kMaxEoE / 2; ++p) {
-      const val_t<int, 2> t = ld_pair<int>(m.eoe, p, e, m.nE);
-      ep[2 * p] = t.x, ep[2 * p + 1] = t.y;
-    }
+    eoe_row(m, e, ep);
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
 #pragma unroll
@@ -668,11 +679,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   int ep[kMaxEoE];
   V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
   R w[kMaxEoE];
-#pragma unroll
-  for (int p = 0; p < kMaxEoE / 2; ++p) {
-    const val_t<int, 2> t = ld_pair<int, true>(m.eoe, p, e, m.nE);
-    ep[2 * p] = t.x, ep[2 * p + 1] = t.y;
-  }
+  eoe_row<true>(m, e, ep);
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     uv[q] = at.ld(b.u, ep[q]);
@@ -859,7 +866,7 @@ __global__ __launch_bounds__(kBlock) void k_cvor_flux(DMesh m, StageBufs<R> b) {
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     const int ep = m.eoe[pair_at(q, e, m.nE)];
-    const R f = ((geo_at<R>(m.woe, m.woe32, pair_at(q, e, m.nE)) * b.u[ep]) * b.gd_e[ep]) * R(0.5);
+    const R f = ((geo_slot<R>(m.woe, m.woe32, q, e, m.nE) * b.u[ep]) * b.gd_e[ep]) * R(0.5);
     s = s + f * (p0 + b.pv_e[ep]);
     sd = sd + f * (d0 + b.dpv[ep]);
   }

@@ -15,11 +15,12 @@ constexpr int kFit2 = 12;       // vertex second-order fit stencil (excluding ce
 constexpr int kFit2C = 6;       // cell second-order fit stencil (excluding centre)
 constexpr int kFit4C = 18;      // cell fourth-order fit stencil (excluding centre)
 constexpr int kFit4V = 21;      // vertex fourth-order fit stencil (excluding centre)
-// Streamed slot tables of the edges (edgesOnEdge, weightsOnEdge and their
-// FP32 copy, deriv2OnVertex and its copy) are stored in slot pairs: element
+// Streamed slot tables of the edges (edgesOnEdge, and the FP64
+// weightsOnEdge and deriv2OnVertex) are stored in slot pairs: element
 // (slot, i) of an n-element table sits at pair_at(slot, i, n), so one 8- or
 // 16-byte load per thread reads two slots and a wavefront's load of a pair is
-// still one contiguous block.
+// still one contiguous block.  (The FP32 copies of the weights stay
+// slot-major: 8-byte float pairs measured slower than two 4-byte loads.)
 __host__ __device__ inline size_t pair_at(int slot, int i, int n) {
   return ((size_t)(slot >> 1) * n + i) * 2 + (slot & 1);
 }
@@ -52,7 +53,7 @@ struct DMesh {
   const int *eoe;         // [5][nE][2] edgesOnEdge (slot pairs, pair_at)
   const double *woe;      // [5][nE][2] weightsOnEdge (slot pairs)
   const double *d2v;      // [2][6][nE][2] deriv2OnVertex slots 1..12 per side (slot pairs)
-  const float *d2v32;     // the same in FP32 (precision-32 contexts), else null
+  const float *d2v32;     // [24][nE] the same in FP32, slot-major (precision-32 contexts), else null
   // cell fits of add_upwind_correction_on_cell (adv_order 3..6), else null:
   // unused slots point at the cell itself with weight 0
   const int *fit2C;       // [nC][fit_row(6)] fit2Cells slots 1..6
@@ -75,7 +76,7 @@ struct DMesh {
   // FP32 copies of the edge geometry and weights that FP32 arithmetic reads
   // (precision-32 contexts, else null): each element is the round-to-nearest
   // float of the FP64 one, the value a cast at the point of use gives
-  const float *woe32;     // [10][nE]
+  const float *woe32;     // [10][nE] slot-major
   const float *dv32, *dc32, *aE32;  // [nE]
   const float *d2c32, *d4c32, *d4v32;  // FP32 copies of d2c, d4c, d4v where those exist
 };

@@ -822,7 +822,12 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   UP(woe, woe, double);
   UP(d2v, d2v, double);
   if (o.precision == 32) {
-    std::vector<float> d2v32(d2v.begin(), d2v.end()), woe32(woe.begin(), woe.end());
+    // FP32 copies slot-major ([slot][n]): the FP64 tables are slot pairs
+    std::vector<float> d2v32(d2v.size()), woe32(woe.size());
+    for (int ne = 0; ne < nE; ++ne) {
+      for (int s = 0; s < kMaxEoE; ++s) woe32[(size_t)s * nE + ne] = (float)woe[pair_at(s, ne, nE)];
+      for (int s = 0; s < 2 * kFit2; ++s) d2v32[(size_t)s * nE + ne] = (float)d2v[pair_at(s, ne, nE)];
+    }
     std::vector<float> dv32(dv.begin(), dv.end()), dc32(dc.begin(), dc.end()), aE32(aE.begin(), aE.end());
     UP(d2v32, d2v32, float);
     UP(woe32, woe32, float);
