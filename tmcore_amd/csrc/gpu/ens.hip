@@ -48,7 +48,7 @@ struct EnsLevelDev {
   R *gdS[2], *iap_s, *qS[2];
   R *acc_gd, *acc_iap, *acc_q;
   R *t_gd[2], *t_iap[2];
-  R *gd_e, *u, *Fdv, *kee, *udc, *E, *tend_gd, *pv_v, *pv_e;
+  R *gd_e, *u, *fk, *udc, *E, *tend_gd, *pv_v, *pv_e;  // fk: [2 nE][M] (StageBufs::fk)
   double2 *part;  // [kNumSums][M][part_stride]
   int part_stride;
   double *sums;   // [M][kNumSums]
@@ -128,8 +128,7 @@ __device__ __forceinline__ StageBufs<R> ens_bufs(const EnsLevelDev<R> &L, int k,
   b.tcur_iap = L.t_iap[k & 1];
   b.gd_e = L.gd_e;
   b.u = L.u;
-  b.Fdv = L.Fdv;
-  b.kee = L.kee;
+  b.fk = L.fk;
   b.udc = L.udc;
   b.E = L.E;
   b.tend_gd = L.tend_gd;
@@ -529,10 +528,10 @@ int ens_alloc_level(tm_ens *E, int l, EnsLevelDev<R> &L) {
   R **cells[] = {&L.gd_old, &L.ghs, &L.gdS[0], &L.gdS[1], &L.acc_gd, &L.t_gd[0], &L.t_gd[1], &L.E, &L.tend_gd};
   for (R **p : cells)
     if (!(*p = A(nC))) return E->fail(TM_ENOMEM, "hipMalloc ensemble cell state");
-  R **edges[] = {&L.iap_old, &L.iap_s, &L.acc_iap, &L.t_iap[0], &L.t_iap[1], &L.gd_e, &L.u, &L.Fdv, &L.kee,
-                 &L.udc, &L.pv_e};
+  R **edges[] = {&L.iap_old, &L.iap_s, &L.acc_iap, &L.t_iap[0], &L.t_iap[1], &L.gd_e, &L.u, &L.udc, &L.pv_e};
   for (R **p : edges)
     if (!(*p = A(nE))) return E->fail(TM_ENOMEM, "hipMalloc ensemble edge state");
+  if (!(L.fk = A(2 * nE))) return E->fail(TM_ENOMEM, "hipMalloc ensemble edge state");
   if (!(L.pv_v = A(nV))) return E->fail(TM_ENOMEM, "hipMalloc ensemble vertex state");
   const size_t nq = std::max<size_t>((size_t)nT * nC, 1);
   R **tr[] = {&L.q_old, &L.qS[0], &L.qS[1], &L.acc_q};

@@ -70,7 +70,10 @@ struct StageBufs {
   R *acc_gd, *acc_iap, *acc_q;           // t1 + 2 t2 + 2 t3 (+ t4)/6
   const R *tprev_gd, *tprev_iap;         // tendency of stage k-1
   R *tcur_gd, *tcur_iap;                 // tendency of stage k
-  R *gd_e, *u, *Fdv, *kee, *udc, *E, *tend_gd, *pv_v, *pv_e;
+  R *gd_e, *u, *udc, *E, *tend_gd, *pv_v, *pv_e;
+  // [nE][2]: u*gd_e*dvEdge (mass flux) and u^2*areaEdge, interleaved because
+  // the cell sums gather both at the same edge (one load per slot)
+  R *fk;
   const R *ghs;
   int nT;
   int k;          // 1..4: CRK4 stage; 0: diagnostics/invariants only
@@ -266,6 +269,18 @@ struct Solo {
   __device__ __forceinline__ void st_s(R *p, size_t i, R x) const {
     __builtin_nontemporal_store(x, p + i);
   }
+  // two values of element i of an interleaved pair array ([n][2]): one load
+  template <typename R>
+  __device__ __forceinline__ void ld2(const R *p, size_t i, R &a, R &b) const {
+    const val_t<R, 2> v = *reinterpret_cast<const val_t<R, 2> *>(p + 2 * i);
+    a = v.x, b = v.y;
+  }
+  template <typename R>
+  __device__ __forceinline__ void st2(R *p, size_t i, R a, R b) const {
+    val_t<R, 2> v;
+    v.x = a, v.y = b;
+    *reinterpret_cast<val_t<R, 2> *>(p + 2 * i) = v;
+  }
 };
 template <int NW>
 struct Members {
@@ -287,6 +302,15 @@ struct Members {
   template <typename R>
   __device__ __forceinline__ void st_s(R *p, size_t i, val_t<R, NW> x) const {
     st(p, i, x);
+  }
+  // interleaved pair array: elements 2i and 2i+1 of a [2n][M] array
+  template <typename R>
+  __device__ __forceinline__ void ld2(const R *p, size_t i, val_t<R, NW> &a, val_t<R, NW> &b) const {
+    a = ld(p, 2 * i), b = ld(p, 2 * i + 1);
+  }
+  template <typename R>
+  __device__ __forceinline__ void st2(R *p, size_t i, val_t<R, NW> a, val_t<R, NW> b) const {
+    st(p, 2 * i, a), st(p, 2 * i + 1, b);
   }
 };
 using Member = Members<1>;
@@ -355,8 +379,7 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
   const R dv = geo_at<R>(m.dv, m.dv32, e);
   at.st(b.gd_e, e, gde);
   at.st(b.u, e, u);
-  at.st(b.Fdv, e, (u * gde) * dv);
-  at.st(b.kee, e, (u * u) * geo_at<R>(m.aE, m.aE32, e));
+  at.st2(b.fk, e, (u * gde) * dv, (u * u) * geo_at<R>(m.aE, m.aE32, e));
   at.st(b.udc, e, u * geo_at<R>(m.dc, m.dc32, e));
   if constexpr ((X & kXAdv) != 0) {
     if (b.qe)  // tracer edge values with the upwind correction (adv_scheme_sg11_mod.F90:42-43)
@@ -437,8 +460,7 @@ __device__ __forceinline__ CellSums<val_t<double, L::W>> cell_e(const DMesh &m, 
 #pragma unroll
   for (int s = 0; s < kMaxEoC; ++s) {
     const int e = unsign(idx[s]);
-    fv[s] = at.ld(b.Fdv, e);
-    kv[s] = at.ld(b.kee, e);
+    at.ld2(b.fk, e, fv[s], kv[s]);
   }
   V sdiv = V(R(0)), ske = V(R(0));
 #pragma unroll

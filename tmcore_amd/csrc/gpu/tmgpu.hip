@@ -106,7 +106,7 @@ struct tm_ctx {
   double2 *cv_part = nullptr;
   double *cv_eps = nullptr;
   void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
-  void *gd_e = nullptr, *u = nullptr, *Fdv = nullptr, *kee = nullptr, *udc = nullptr;
+  void *gd_e = nullptr, *u = nullptr, *fk = nullptr, *udc = nullptr;  // fk: [nE][2] (StageBufs::fk)
   void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
   void *scratch_e[4] = {}, *scratch_c[2] = {}, *scratch_v[2] = {};
   double2 *part = nullptr;
@@ -247,8 +247,7 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.tcur_iap = P(c->t_iap[k & 1]);
   b.gd_e = P(c->gd_e);
   b.u = P(c->u);
-  b.Fdv = P(c->Fdv);
-  b.kee = P(c->kee);
+  b.fk = P(c->fk);
   b.udc = P(c->udc);
   b.E = P(c->E);
   b.tend_gd = P(c->tend_gd);
@@ -960,10 +959,10 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   for (void **p : rbufs_c)
     if (!(*p = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc cell state");
   void **rbufs_e[] = {&c->iap_old, &c->iap_s, &c->acc_iap, &c->t_iap[0], &c->t_iap[1], &c->gd_e, &c->u,
-                      &c->Fdv, &c->kee, &c->udc, &c->pv_e, &c->scratch_e[0], &c->scratch_e[1],
-                      &c->scratch_e[2], &c->scratch_e[3]};
+                      &c->udc, &c->pv_e, &c->scratch_e[0], &c->scratch_e[1], &c->scratch_e[2], &c->scratch_e[3]};
   for (void **p : rbufs_e)
     if (!(*p = c->alloc_r(nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
+  if (!(c->fk = c->alloc_r(2 * (size_t)nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
   void **rbufs_v[] = {&c->pv_v, &c->scratch_v[0], &c->scratch_v[1]};
   for (void **p : rbufs_v)
     if (!(*p = c->alloc_r(nV))) return c->fail(TM_ENOMEM, "hipMalloc vertex state");
