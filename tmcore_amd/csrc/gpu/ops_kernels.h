@@ -97,7 +97,7 @@ __global__ __launch_bounds__(kBlock) void k_op_upwind_cell(DMesh m, StageBufs<do
                                                            const double *u, const double *fe_in, double *out) {
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nE) return;
-  out[e] = upwind_cell_e<double, Solo, kXAdv>(m, b, Solo{}, f, 0, e, fe_in[e], sign1v<double, 1>(u[e]));
+  out[e] = upwind_cell_e<double, Solo, kXAdv | kXAdv56>(m, b, Solo{}, f, 0, e, fe_in[e], sign1v<double, 1>(u[e]));
 }
 
 // calc_pv_on_vertex: pv = (f + vor) / gd_vertex

@@ -318,10 +318,11 @@ using Member = Members<1>;
 // Compile-time variants of the stage code: X = 0 is the default configuration
 // (adv_order 2, pv_scheme 3 with interp_pv_order <= 4, CRK4) and carries none
 // of the code below, so its kernels keep their register budget; kXAdv adds the
-// upwind cell correction (adv_order >= 3), kXExt the PV schemes 1/2, the
+// upwind cell correction (adv_order >= 3), kXAdv56 (with kXAdv) its
+// fourth-order fits for adv_order 5/6, kXExt the PV schemes 1/2, the
 // order-5/6 PV and the stage-descriptor time integration (CRK3, PC2; CRK4
 // too in that variant), all still checked at run time inside.
-constexpr int kXAdv = 1, kXExt = 2;
+constexpr int kXAdv = 1, kXExt = 2, kXAdv56 = 4;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (f at offset base; padded slots add an exact 0).
@@ -359,12 +360,57 @@ __device__ __forceinline__ val_t<R, L::W> upwind_cell_e(const DMesh &m, const St
   const V d1 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, m.d2c32, c1, 0, e);
   const V d2 = fit_sum<kFit2C, R, L>(m, f, base, at, m.fit2C, m.d2c, m.d2c32, c2, 1, e);
   const R dc = geo_at<R>(m.dc, m.dc32, e);
+  if constexpr ((X & kXAdv56) == 0) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
   if (b.adv_order <= 4) return fe - ((dc * dc) * ((d1 + d2) - (sgn * (R)b.adv_coef2) * (d2 - d1))) / R(12);
   const V q1 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, m.d4c32, c1, 0, e);
   const V q2 = fit_sum<kFit4C, R, L>(m, f, base, at, m.fit4C, m.d4c, m.d4c32, c2, 1, e);
   return (fe - ((dc * dc) * (d1 + d2)) / R(12)) +
          (((dc * dc) * (dc * dc)) * ((q1 + q2) - (sgn * (R)b.adv_coef4) * (q2 - q1))) / R(60);
 }
+
+// The order-3/4 upwind correction's per-edge data (both cells' fit2Cells
+// rows, their deriv2OnCell weights, dcEdge), loaded once per edge and applied
+// to every field the edge corrects: gd and each tracer.  apply() is
+// upwind_cell_e at adv_order 3/4 term for term.
+template <typename R>
+struct UpwindFit2 {
+  int c1, c2;
+  int i1[kFit2C], i2[kFit2C];
+  R w1[kFit2C], w2[kFit2C];
+  R dc;
+  __device__ __forceinline__ void load(const DMesh &m, int e) {
+    constexpr int RW = fit_row(kFit2C);
+    c1 = m.coe[e], c2 = m.coe[m.nE + e];
+    const int4 *r1 = reinterpret_cast<const int4 *>(m.fit2C) + (size_t)c1 * (RW / 4);
+    const int4 *r2 = reinterpret_cast<const int4 *>(m.fit2C) + (size_t)c2 * (RW / 4);
+    int t1[RW], t2[RW];
+#pragma unroll
+    for (int j = 0; j < RW / 4; ++j) {
+      const int4 a = r1[j], b = r2[j];
+      t1[4 * j] = a.x, t1[4 * j + 1] = a.y, t1[4 * j + 2] = a.z, t1[4 * j + 3] = a.w;
+      t2[4 * j] = b.x, t2[4 * j + 1] = b.y, t2[4 * j + 2] = b.z, t2[4 * j + 3] = b.w;
+    }
+#pragma unroll
+    for (int j = 0; j < kFit2C; ++j) {
+      i1[j] = t1[j], i2[j] = t2[j];
+      w1[j] = geo_at<R>(m.d2c, m.d2c32, (size_t)j * m.nE + e);
+      w2[j] = geo_at<R>(m.d2c, m.d2c32, (size_t)(kFit2C + j) * m.nE + e);
+    }
+    dc = geo_at<R>(m.dc, m.dc32, e);
+  }
+  template <typename L>
+  __device__ __forceinline__ val_t<R, L::W> apply(const R *f, size_t base, L at, val_t<R, L::W> fe,
+                                                  val_t<R, L::W> sgn, R coef2) const {
+    typedef val_t<R, L::W> V;
+    const V f1 = at.ld(f, base + c1), f2 = at.ld(f, base + c2);
+    V d1 = V(R(0)), d2 = V(R(0));
+#pragma unroll
+    for (int j = 0; j < kFit2C; ++j) d1 += w1[j] * (at.ld(f, base + i1[j]) - f1);
+#pragma unroll
+    for (int j = 0; j < kFit2C; ++j) d2 += w2[j] * (at.ld(f, base + i2[j]) - f2);
+    return fe - ((dc * dc) * ((d1 + d2) - (sgn * coef2) * (d2 - d1))) / R(12);
+  }
+};
 
 // gd_edge, u and the per-edge products the cell/vertex sums gather; returns
 // the edge part of the energy invariant when b.want_inv.
@@ -373,8 +419,19 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
   typedef val_t<R, L::W> V;
   const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
   const V iap = at.ld(b.iap, e);
-  const V gde = upwind_cell_e<R, L, X>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
-                                       sign1v<R, L::W>(iap));
+  // adv_order 3/4 (kXAdv without kXAdv56): the fit data, loaded once,
+  // serves gd and every tracer of this edge
+  constexpr bool kHoist = (X & kXAdv) != 0 && (X & kXAdv56) == 0;
+  V gde;
+  [[maybe_unused]] UpwindFit2<R> uf;
+  if constexpr (kHoist) {
+    uf.load(m, e);
+    gde = uf.apply(b.gd, 0, at, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)), sign1v<R, L::W>(iap),
+                   (R)b.adv_coef2);
+  } else {
+    gde = upwind_cell_e<R, L, X>(m, b, at, b.gd, 0, e, R(0.5) * (at.ld(b.gd, c1) + at.ld(b.gd, c2)),
+                                 sign1v<R, L::W>(iap));
+  }
   const V u = iap / tm_sqrt<R, L::W>(gde);
   const R dv = geo_at<R>(m.dv, m.dv32, e);
   at.st(b.gd_e, e, gde);
@@ -386,7 +443,12 @@ __device__ __forceinline__ val_t<double, L::W> edge_prep_e(const DMesh &m, const
       for (int t = 0; t < b.nT; ++t) {
         const size_t tb = (size_t)t * m.nC;
         const V qe = R(0.5) * (at.ld(b.q, tb + c1) + at.ld(b.q, tb + c2));
-        at.st(b.qe, (size_t)t * m.nE + e, upwind_cell_e<R, L, X>(m, b, at, b.q, tb, e, qe, sign1v<R, L::W>(u)));
+        V qc;
+        if constexpr (kHoist)
+          qc = uf.apply(b.q, tb, at, qe, sign1v<R, L::W>(u), (R)b.adv_coef2);
+        else
+          qc = upwind_cell_e<R, L, X>(m, b, at, b.q, tb, e, qe, sign1v<R, L::W>(u));
+        at.st(b.qe, (size_t)t * m.nE + e, qc);
       }
   }
   const val_t<double, L::W> id = to_d<R, L::W>(iap);
@@ -1095,7 +1157,7 @@ __global__ __launch_bounds__(kBlock) void k_iap_from_u(DMesh m, StageBufs<R> b, 
   const int e = blockIdx.x * kBlock + threadIdx.x;
   if (e >= m.nE) return;
   const R gde =
-      upwind_cell_e<R, Solo, kXAdv>(m, b, Solo{}, gd, 0, e, R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]), R(1));
+      upwind_cell_e<R, Solo, kXAdv | kXAdv56>(m, b, Solo{}, gd, 0, e, R(0.5) * (gd[m.coe[e]] + gd[m.coe[m.nE + e]]), R(1));
   iap[e] = tm_sqrt(gde) * u[e];
 }
 

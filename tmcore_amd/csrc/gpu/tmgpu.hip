@@ -124,7 +124,7 @@ struct tm_ctx {
   bool have_state = false;
   bool timed = false;
   int serpentine = 1;  // alternate the tile sweep direction between kernels
-  int xmask = 0;       // compile-time variant of the stage kernels (kXAdv, kXExt)
+  int xmask = 0;       // compile-time variant of the stage kernels (kXAdv, kXAdv56, kXExt)
   int nstages = 4;     // stages per step: 4 (CRK4), 3 (CRK3, PC2)
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
@@ -339,7 +339,9 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
     case 0: launch_part_x<R, 0>(c, which, b); break;
     case kXAdv: launch_part_x<R, kXAdv>(c, which, b); break;
     case kXExt: launch_part_x<R, kXExt>(c, which, b); break;
-    default: launch_part_x<R, kXAdv | kXExt>(c, which, b);
+    case kXAdv | kXExt: launch_part_x<R, kXAdv | kXExt>(c, which, b); break;
+    case kXAdv | kXAdv56: launch_part_x<R, kXAdv | kXAdv56>(c, which, b); break;
+    default: launch_part_x<R, kXAdv | kXAdv56 | kXExt>(c, which, b);
   }
 }
 
@@ -971,7 +973,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     if (!(*p = c->alloc_r(std::max<size_t>(nq, 1)))) return c->fail(TM_ENOMEM, "hipMalloc tracer state");
   if (o.adv_order >= 3 && c->nT > 0 && !(c->qe = c->alloc_r((size_t)c->nT * nE)))
     return c->fail(TM_ENOMEM, "hipMalloc tracer edge values");
-  c->xmask = (o.adv_order >= 3 ? kXAdv : 0) |
+  c->xmask = (o.adv_order >= 3 ? kXAdv : 0) | (o.adv_order >= 5 ? kXAdv56 : 0) |
              (o.pv_scheme != 3 || o.interp_pv_order >= 5 || o.time_scheme != 2 ? kXExt : 0);
   c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
