@@ -1045,8 +1045,22 @@ static __global__ __launch_bounds__(kReduceBlock) void k_reduce(ReduceArgs a) {
   const bool is_inv = i >= kMass;
   const int nb = edge ? a.nbE : (vert ? a.nbV : a.nbC);
   double2 acc = make_double2(0.0, 0.0);
-  if ((is_inv && a.have_inv) || (!is_inv && a.mode == 1))
-    for (int j = threadIdx.x; j < nb; j += kReduceBlock) acc = dd_add(acc, a.part[i * a.stride + j]);
+  if ((is_inv && a.have_inv) || (!is_inv && a.mode == 1)) {
+    // partials j = tid, tid + block, ... added in that order; loaded eight
+    // at a time so the loads overlap instead of each waiting for the add
+    constexpr int kB = 8;
+    for (int j0 = threadIdx.x; j0 < nb; j0 += kB * kReduceBlock) {
+      double2 v[kB];
+#pragma unroll
+      for (int k = 0; k < kB; ++k) {
+        const int j = j0 + k * kReduceBlock;
+        v[k] = j < nb ? a.part[i * a.stride + j] : make_double2(0.0, 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k < kB; ++k)
+        if (j0 + k * kReduceBlock < nb) acc = dd_add(acc, v[k]);
+    }
+  }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     double2 o = make_double2(__shfl_xor(acc.x, off, 64), __shfl_xor(acc.y, off, 64));
