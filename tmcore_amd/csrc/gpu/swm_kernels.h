@@ -1135,31 +1135,35 @@ struct FinalBufs {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_finalize(DMesh m, FinalBufs<R> f) {
   __shared__ double sbeta;
+  // What does not need beta goes before the wait for it: each thread's loads,
+  // and the tracer update (beta = 1).
+  const bool edges = (int)blockIdx.x < m.nbE;
+  const int i = edges ? xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x
+                      : xcd_tile(blockIdx.x - m.nbE, m.nbC) * kBlock + threadIdx.x;
+  const bool live = i < (edges ? m.nE : m.nC);
+  R s0 = R(0), a0 = R(0);
+  if (live) {
+    s0 = edges ? f.iap[i] : f.gd[i];
+    a0 = edges ? f.acc_iap[i] : f.acc_gd[i];
+  }
+  if (!edges && live)
+    for (int t = 0; t < f.nT; ++t) {
+      const size_t o = (size_t)t * m.nC + i;
+      f.q[o] = f.q[o] + (R)f.dt * f.acc_q[o];
+    }
   if (threadIdx.x == 0) {
     const double beta = crk4_beta(f.out);
     sbeta = beta;
     if (blockIdx.x == 0) {
-      const int i = *f.out.hist_idx;
-      if (i < f.out.hist_cap) f.out.beta_hist[i] = beta;
-      if (f.out.have_inv) write_invariants(f.out, i - 1);
-      *f.out.hist_idx = i + 1;
+      const int hi = *f.out.hist_idx;
+      if (hi < f.out.hist_cap) f.out.beta_hist[hi] = beta;
+      if (f.out.have_inv) write_invariants(f.out, hi - 1);
+      *f.out.hist_idx = hi + 1;
     }
   }
   __syncthreads();
   const double bdt = sbeta * f.dt;
-  if ((int)blockIdx.x < m.nbE) {
-    const int e = xcd_tile(blockIdx.x, m.nbE) * kBlock + threadIdx.x;
-    if (e < m.nE) f.iap[e] = f.iap[e] + (R)bdt * f.acc_iap[e];
-  } else {
-    const int c = xcd_tile(blockIdx.x - m.nbE, m.nbC) * kBlock + threadIdx.x;
-    if (c < m.nC) {
-      f.gd[c] = f.gd[c] + (R)bdt * f.acc_gd[c];
-      for (int t = 0; t < f.nT; ++t) {
-        const size_t o = (size_t)t * m.nC + c;
-        f.q[o] = f.q[o] + (R)f.dt * f.acc_q[o];
-      }
-    }
-  }
+  if (live) (edges ? f.iap : f.gd)[i] = s0 + (R)bdt * a0;
 }
 
 // iap_u = sqrt(c2e(gd) + upwind correction) * u (tmcore_swm_mod.F90:62-64).
