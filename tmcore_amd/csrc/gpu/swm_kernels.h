@@ -698,7 +698,7 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
       d1 += w1[j] * (q1[j] - p1);
       d2 += w2[j] * (q2[j] - p2);
     }
-    const R dv = geo_at<R>(m.dv, m.dv32, e);
+    const R dv = geo_at<R, true>(m.dv, m.dv32, e);
     if ((X & kXExt) == 0 || b.pv_order <= 4) {
       pve = pve - ((dv * dv) * ((d1 + d2) - (sign1v<R, L::W>(v) * coef2) * (d2 - d1))) / R(12);
     } else {
@@ -745,8 +745,8 @@ template <typename R, typename L>
 __device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
                                                     val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it) {
   typedef val_t<R, L::W> V;
-  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
-  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / geo_at<R>(m.dc, m.dc32, e);
+  const int c1 = nt_ld(&m.coe[e]), c2 = nt_ld(&m.coe[m.nE + e]);
+  const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / geo_at<R, true>(m.dc, m.dc32, e);
   const V ut = s - dEdx;
   const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
   it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
@@ -775,7 +775,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   V s = V(R(0));
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) s = s + (((w[q] * uv[q]) * gv[q]) * R(0.5)) * (pve + pv[q]);
-  const int c1 = m.coe[e], c2 = m.coe[m.nE + e];
+  const int c1 = nt_ld(&m.coe[e]), c2 = nt_ld(&m.coe[m.nE + e]);
   const V tge = upwind_cell_e<R, L, X>(m, b, at, b.tend_gd, 0, e,
                                        R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
                                        sign1v<R, L::W>(at.ld(b.iap, e)));
@@ -786,7 +786,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   if (b.d_tiap) at.st(b.d_tiap, e, it);
   EdgeSums<D> r{D(0.0), D(0.0), D(0.0)};
   if (b.k >= 1) {
-    const double aE = m.aE[e];
+    const double aE = nt_ld(&m.aE[e]);
     if constexpr ((X & kXExt) != 0) {
       (void)rk_update_ext<R, L>(b, at, e, it, b.iap_base, b.iap_next, b.acc_iap, b.tprev_iap, b.tfirst_iap,
                                 b.tcur_iap, aE, r.rp, r.rf, r.r44);
