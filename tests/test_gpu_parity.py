@@ -370,6 +370,31 @@ def test_fp32_form_tracks_fp64():
     g32.close()
 
 
+@pytest.mark.parametrize("adv_order,pv_order,pv_scheme", [(4, 3, 3), (6, 6, 3), (3, 3, 1)])
+def test_fp32_form_tracks_fp64_at_high_orders(adv_order, pv_order, pv_scheme):
+    """The FP32 form on the scheme switches whose stencil weights it reads
+    from FP32 copies (deriv2OnCell, deriv4OnCell, deriv4OnVertex besides the
+    default ones): level 4, 2 tracers, 48 steps against the FP64 form.  A
+    copy laid out in the wrong slots would be off by O(1), not by FP32
+    rounding."""
+    m = mesh(4)
+    f = m.fit(max_fit=31, order4=adv_order >= 5 or pv_order >= 5)
+    gd, u, ghs, q = seeded_state(m, 4, [301, 302])
+    kw = dict(adv_order=adv_order, interp_pv_order=pv_order, pv_scheme=pv_scheme)
+    out = {}
+    for prec in (64, 32):
+        g = SwmGpu(m, f, dt=600.0, ntracers=2, precision=prec, **kw)
+        g.upload(gd, ghs, u=u, tracers=q)
+        inv, _ = g.step(48)
+        out[prec] = (g.download(), inv)
+        g.close()
+    (s64, inv64), (s32, inv32) = out[64], out[32]
+    assert rel_l2(s32["gd"], s64["gd"]) < 1e-5
+    assert rel_l2(s32["iap_u"], s64["iap_u"]) < 1e-4
+    assert rel_l2(s32["tracers"], s64["tracers"]) < 1e-4
+    assert abs(inv32[-1, 0] / inv64[0, 0] - 1) < 1e-6
+
+
 def test_nonfinite_upload_is_rejected():
     m = mesh(2)
     gd, u, ghs, _ = seeded_state(m, 1)
