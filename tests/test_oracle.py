@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle.oracle import Oracle, lib, oracle_poly_fit
-from tmcore_amd.mesh import G, RADIUS, Mesh, seeded_state, steady_geostrophic_state
+from tmcore_amd.mesh import G, OMEGA, RADIUS, Mesh, seeded_state, steady_geostrophic_state
 
 
 @pytest.fixture(scope="module")
@@ -105,6 +105,89 @@ def test_pv_flux_is_energy_neutral(o4, m4):
     aE = m4.dvEdge * m4.dcEdge * RADIUS**2
     terms = aE * u * gde * flx
     assert abs(terms.sum()) < 1e-13 * np.abs(terms).sum()
+
+
+def test_tangent_wind_curl_is_kite_average_of_divergence(o4, m4):
+    """calc_tangent_wind (operators_mod.F90:223-237) through the oracle: the
+    vertex curl of the reconstructed tangential wind W u equals the kite
+    average of div_operator(u) (Thuburn et al. 2009; div_operator returns
+    minus the divergence).  The identity holds to round-off for any u, so it
+    pins the tangent wind's sign and slot pairing and, on the right-hand side,
+    scalar_c2v's kite-to-cell pairing (operators_mod.F90:106-123): a rotated
+    kite slot or a flipped sign leaves an O(1) residual."""
+    rng = np.random.default_rng(11)
+    u = rng.standard_normal(m4.nEdges)
+    lhs = o4.op("curl", o4.op("tangent_wind", u, out_where="e"), out_where="v")
+    rhs = o4.op("c2v", o4.op("div", u, out_where="c"), out_where="v")
+    assert np.abs(lhs - rhs).max() < 1e-13 * np.abs(rhs).max()
+
+
+def test_tangent_wind_reconstructs_solid_body_rotation():
+    """The oracle's tangent wind of a solid-body normal wind approaches the
+    analytic tangential component t.V, t = k x n (from verticesOnEdge(1) to
+    (2)); the rms error falls with resolution.  The mesh is not centroidal,
+    so the error near the pentagons stays at a few percent."""
+    from tests.test_mesh import frames
+    u0, errs = 20.0, []
+    for L in (3, 4, 5):
+        m = Mesh(L)
+        o = Oracle(m, m.fit(), dt=600.0)
+        xe, _, _, n, t = frames(m)
+        V = np.stack([-xe[:, 1], xe[:, 0], 0 * xe[:, 0]], 1) * u0
+        err = o.op("tangent_wind", (V * n).sum(1), out_where="e") - (V * t).sum(1)
+        errs.append(np.sqrt((err**2).mean()) / u0)
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[2] < 0.02
+
+
+def test_c2v_and_v2c_area_integrals_and_adjointness(o4, m4):
+    """scalar_c2v (operators_mod.F90:106-123) keeps the area integral,
+    sum_v A_t c2v(f) = sum_c A_c f, because each cell's kites partition its
+    area; scalar_v2c (:125-141) is its adjoint in the area-weighted inner
+    products.  Both hold to round-off for random fields only when every kite
+    is paired with its own cell (a rotated kite slot moves the integral by
+    ~5e-5 of itself at this level)."""
+    rng = np.random.default_rng(12)
+    A, At = m4.areaCell * RADIUS**2, m4.areaTriangle * RADIUS**2
+    f = rng.uniform(0.0, 1.0, m4.nCells)
+    g = rng.standard_normal(m4.nVertices)
+    cv = o4.op("c2v", f, out_where="v")
+    vc = o4.op("v2c", g, out_where="c")
+    assert abs((At * cv).sum() - (A * f).sum()) < 1e-13 * (A * f).sum()
+    assert abs((At * g * cv).sum() - (A * f * vc).sum()) < 1e-13 * np.abs(At * g * cv).sum()
+
+
+def test_invariants_closed_forms_at_rest_and_in_solid_body_rotation(o4, m4):
+    """Post-step invariants (tmcore_swm_mod.F90:215-255) on states whose
+    integrals are known.  With gd = G0 everywhere:
+    - at rest: mass G0 S, energy G0^2 S (S = 4 pi a^2), potential enstrophy
+      sum gd_v (f/gd_v)^2 A_t = (4 Omega^2 / G0) S/3 exactly (the mesh's
+      icosahedral symmetry makes sum z^2 A_t = S/3), absolute vorticity 0;
+    - in solid-body rotation u0 cos(lat): absolute vorticity (2 Omega +
+      2 u0/a) sin(lat), so the enstrophy tends to (2 Omega + 2 u0/a)^2 S /
+      (3 G0) and the energy to G0 u0^2 (2S/3) + G0^2 S, at second order.
+    A dropped square, a missing thickness weight or a sign on the relative
+    vorticity moves these by O(1)."""
+    G0 = 5000.0 * G
+    S = 4.0 * np.pi * RADIUS**2
+    gd, ghs = np.full(m4.nCells, G0), np.zeros(m4.nCells)
+    inv = o4.invariants(gd, np.zeros(m4.nEdges), ghs)
+    assert inv[0] == pytest.approx(G0 * S, rel=1e-13)
+    assert inv[1] == pytest.approx(G0**2 * S, rel=1e-13)
+    assert inv[2] == pytest.approx(4.0 * OMEGA**2 / G0 * S / 3.0, rel=1e-13)
+    assert abs(inv[3]) < 1e-14 * 2.0 * OMEGA
+    u0 = 20.0
+    errs = []
+    for L, o in ((3, None), (4, o4)):
+        m = m4 if L == 4 else Mesh(L)
+        o = o or Oracle(m, m.fit(), dt=600.0)
+        _, us, _, _ = seeded_state(m, 1)  # solid-body u0 = 20 m/s from a stream function
+        inv = o.invariants(np.full(m.nCells, G0), np.sqrt(G0) * us, np.zeros(m.nCells))
+        ens = (2.0 * OMEGA + 2.0 * u0 / RADIUS) ** 2 / G0 * S / 3.0
+        errs.append(abs(inv[2] / ens - 1.0))
+        assert inv[1] == pytest.approx(G0 * u0**2 * 2.0 * S / 3.0 + G0**2 * S, rel=1e-4)
+        assert abs(inv[3]) < 1e-14 * 2.0 * OMEGA
+    assert errs[1] < 5e-5 and errs[1] < errs[0] / 3.0
 
 
 def test_math_inv_matrix_matches_numpy():
