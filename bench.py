@@ -120,19 +120,23 @@ def closure_bench(days=1, seed=6000):
         return {"skipped": f"no trained weights at {os.path.relpath(CLOSURE_WEIGHTS, ROOT)}"}
     p = Pipeline(9, 6, 300.0, 8, log=log)
     ev = Evaluator(p, CLOSURE_WEIGHTS, "7:150,8:90")
-    speed = {}
-    for name, g in (("L6_plain", ev.plain), ("L6_closure", ev.corr), ("L7_plain", ev.others["L7"][1]),
-                    ("L8_plain", ev.others["L8"][1])):
-        gd, u, ghs, _ = seeded_state(g.mesh, seed)
-        g.upload(gd, ghs, u=u)
-        n = int(round(86400.0 / g.opts.dt))
-        g.step_async(20)
-        g.sync()
-        g.step_async(n)
-        g.sync()
-        ms = g.last_elapsed_ms()
-        speed[name] = {"ms_per_step": round(ms / n, 5), "sim_days_per_hour": round(1.0 / (ms / 3.6e6), 1)}
-    errs, _ = ev.run_seed(seed, days)
+    try:
+        speed = {}
+        for name, g in (("L6_plain", ev.plain), ("L6_closure", ev.corr), ("L7_plain", ev.others["L7"][1]),
+                        ("L8_plain", ev.others["L8"][1])):
+            gd, u, ghs, _ = seeded_state(g.mesh, seed)
+            g.upload(gd, ghs, u=u)
+            n = int(round(86400.0 / g.opts.dt))
+            g.step_async(20)
+            g.sync()
+            g.step_async(n)
+            g.sync()
+            ms = g.last_elapsed_ms()
+            speed[name] = {"ms_per_step": round(ms / n, 5), "sim_days_per_hour": round(1.0 / (ms / 3.6e6), 1)}
+        errs, _ = ev.run_seed(seed, days)
+    finally:
+        ev.close()
+        p.close()
     return {"speed": speed, f"l2_h_rel_day{days}_seed{seed}": {k: v[-1] for k, v in errs.items()},
             "trained_on": ev.meta.get("seeds_all", ev.meta["seeds"]), "deployment_gain": ev.meta.get("gain", 1.0),
             "weights": os.path.relpath(CLOSURE_WEIGHTS, ROOT)}

@@ -26,10 +26,13 @@ typedef struct tm_ctx tm_ctx;
 /* Scheme switches: the reference's &tmcore namelist group
  * (src/core/params_mod.F90:35-58) plus the device options. */
 typedef struct tm_opts {
-  int time_scheme;      /* 2 = CRK4 (time_scheme_mod.F90:109-154); others rejected */
-  int pv_scheme;        /* 3 = high-order upwind PV (operators_mod.F90:359-412)  */
-  int interp_pv_order;  /* 2 or 3 for pv_scheme 3                                */
-  int adv_order;        /* 2 (add_upwind_correction_on_cell is then a no-op)     */
+  int time_scheme;      /* 1 = CRK3, 2 = CRK4, 3 = PC2 (time_scheme_mod.F90:65-184);
+                           tracers need CRK4                                      */
+  int pv_scheme;        /* 1 = APVM, 2 = conservative APVM (global eps),
+                           3 = high-order upwind PV (operators_mod.F90:283-412)  */
+  int interp_pv_order;  /* 2..6 for pv_scheme 3 (5 and 6 need the order-4 fits)  */
+  int adv_order;        /* 2..6: add_upwind_correction_on_cell (operators_mod.F90:
+                           423-472), a no-op at 2; 3..6 need the cell fits      */
   int adv_mono;         /* logical, only read for adv_order 3                    */
   int conserve_energy;  /* logical: CRK4 beta rescale                            */
   double apvm_weight;   /* read by pv_scheme 1                                   */
@@ -53,7 +56,8 @@ void tm_opts_default(tm_opts *o);
  * (reference tmcore_swm_mod.F90:37-41): uploads the mesh (reference layout of
  * tmcore_host.h), derives nSignEdge/tSignEdge/areaEdge/fVertex as
  * mesh_mod.F90:169-224 does, renumbers along a Hilbert curve and allocates the
- * device state.  f may be NULL when pv_scheme 3 is not used. */
+ * device state.  f may be NULL unless pv_scheme 3 with interp_pv_order >= 3
+ * (vertex fits) or adv_order >= 3 (cell fits) is selected. */
 int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts *o);
 int tm_ctx_destroy(tm_ctx *ctx);
 const char *tm_last_error(const tm_ctx *ctx);
@@ -286,7 +290,10 @@ int tm_closure_dataset_add(tm_ctx *ctx, tm_mlp *mlp, const double *t_u, const do
 int tm_closure_dataset_size(tm_ctx *ctx, int *n);
 int tm_closure_dataset_clear(tm_ctx *ctx);
 /* Mean loss of every sample in [first, first + count) under the current
- * parameters, summed per term: loss[0] (velocity) and loss[1] (thickness). */
+ * parameters, summed per term: loss[0] (velocity) and loss[1] (thickness).
+ * tm_closure_eval and tm_closure_train reuse the buffer that holds an
+ * attached closure's correction between steps: after either, the next
+ * tm_step computes the correction afresh from its state. */
 int tm_closure_eval(tm_ctx *ctx, tm_mlp *mlp, int first, int count, double *loss);
 /* epochs passes over samples [0, ntrain), one Adam step (lr, beta 0.9/0.999)
  * per sample in an order shuffled per epoch from seed; loss_hist[2*epochs]

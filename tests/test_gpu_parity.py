@@ -144,6 +144,27 @@ def test_pv_schemes_match_oracle(pv_scheme, order):
     g.close()
 
 
+@pytest.mark.parametrize("pv_scheme", [1, 2])
+def test_apvm_context_without_fit_tables(pv_scheme):
+    """The APVM schemes read no vertex fits: a context created with no fit
+    tables at all (fit = NULL, the default interp_pv_order 3) runs and matches
+    the oracle, which is given no fits either."""
+    m = mesh(4)
+    gd, u, ghs, q = seeded_state(m, 3, [201])
+    o = Oracle(m, None, dt=600.0, pv_scheme=pv_scheme)
+    iap = o.initial_iap(gd, u)
+    g = SwmGpu(m, None, dt=600.0, ntracers=1, pv_scheme=pv_scheme)
+    assert g.fit is None
+    g.upload(gd, ghs, u=u, tracers=q)
+    gd1, iap1, q1, inv, beta = o.run(3, gd, iap, ghs, q)
+    ginv, gbeta = g.step(3)
+    st = g.download()
+    for k, ref in (("gd", gd1), ("iap_u", iap1), ("tracers", q1)):
+        assert rel_l2(st[k], ref) <= 1e-12, k
+    np.testing.assert_allclose(ginv[:, :3], inv[:, :3], rtol=1e-12)
+    g.close()
+
+
 @pytest.mark.parametrize("time_scheme,conserve_energy,pv_scheme", [(1, True, 3), (1, False, 3), (3, True, 3),
                                                                     (3, False, 3), (1, True, 2), (3, True, 1)])
 def test_crk3_pc2_match_oracle(time_scheme, conserve_energy, pv_scheme):

@@ -178,6 +178,13 @@ class Evaluator:
             self.others[f"L{L}"] = (m, SwmGpu(m, m.fit(), dt=float(dt)), CoarseGrainer(m, cm, velocity=False),
                                     CoarseGrainer(p.fm, m, velocity=False))
 
+    def close(self):
+        """Releases every device model and the network (the pipeline stays open)."""
+        self.closure.detach()
+        for g in [self.plain, self.corr] + [o[1] for o in self.others.values()]:
+            g.close()
+        self.mlp.close()
+
     def run_seed(self, seed, days):
         """{run: [error at day 0..days]}, {run: wall seconds of its days}."""
         p, area = self.p, self.p.area

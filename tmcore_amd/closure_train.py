@@ -73,6 +73,11 @@ class Pipeline:
         self.log(f"pipeline: L{fine_level} ({self.fm.nCells} cells, dt {dt / nsub:g} s, FP{fine_precision}) -> "
                  f"L{coarse_level} ({self.cm.nCells} cells, dt {dt:g} s), setup {time.perf_counter() - t0:.1f} s")
 
+    def close(self):
+        """Releases the fine and coarse device models."""
+        self.fine.close()
+        self.coarse.close()
+
     # ---------------------------------------------------------------- truth
     def _advance_fine(self, nsteps):
         while nsteps > 0:

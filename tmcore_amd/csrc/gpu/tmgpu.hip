@@ -164,6 +164,13 @@ struct tm_ctx {
     allocs.push_back(p);
     return p;
   }
+  // frees one allocation made by alloc/alloc_r before the context ends
+  void release(void *p) {
+    if (!p) return;
+    (void)hipFree(p);
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+  }
 };
 
 #define HIPCHK(ctx, call)                                                                    \
@@ -672,7 +679,9 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
                    hipStream_t stream, std::vector<void *> &allocs, std::string &err) {
   int rc = validate_mesh(m, f, o, err);
   if (rc) return rc;
-  const bool need_fit = o.interp_pv_order >= 3;
+  // the vertex fits are read by pv_scheme 3 at interp_pv_order >= 3 only (the
+  // predicate validate_mesh checks them under)
+  const bool need_fit = o.pv_scheme == 3 && o.interp_pv_order >= 3;
   const int nC = m->nCells, nE = m->nEdges, nV = m->nVertices, ME = m->maxEdges, ME2 = m->maxEdges2;
   // Hilbert renumbering: new -> old and old -> new.  Keys are taken on the
   // unit sphere whether or not the caller's coordinates are radius-scaled.
@@ -1241,6 +1250,12 @@ static int stage_impl(tm_ctx *c, tm_stage_out *out) {
     if (!(tq = c->alloc_r((size_t)c->nT * c->nC))) return c->fail(TM_ENOMEM, "hipMalloc tend_q");
     b.d_tq = (R *)tq;
   }
+  // the tracer tendency buffer lives for this call only, on every return path
+  struct Release {
+    tm_ctx *c;
+    void *p;
+    ~Release() { c->release(p); }
+  } release_tq{c, tq};
   int rc = launch_stage<R>(c, b);
   if (rc) return rc;
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -1260,10 +1275,6 @@ static int stage_impl(tm_ctx *c, tm_stage_out *out) {
   };
   for (auto &x : outs)
     if (x.dst && x.src && x.copies > 0 && (rc = download_state(c, x.dst, x.src, *x.perm, x.copies, x.n))) return rc;
-  if (tq) {
-    (void)hipFree(tq);
-    c->allocs.erase(std::find(c->allocs.begin(), c->allocs.end(), tq));
-  }
   return TM_OK;
 }
 
@@ -1423,7 +1434,11 @@ int cl_reserve(tm_ctx *c) {
 
 int cl_loss_reserve(tm_ctx *c, int n) {
   if (n > c->cl_loss_cap) {
-    if (!(c->cl_loss = c->alloc<double>(n))) return c->fail(TM_ENOMEM, "hipMalloc closure loss");
+    double *p = c->alloc<double>(n);
+    if (!p) return c->fail(TM_ENOMEM, "hipMalloc closure loss");
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // no queued work may still use the old buffer
+    c->release(c->cl_loss);
+    c->cl_loss = p;
     c->cl_loss_cap = n;
   }
   HIPCHK(c, hipMemsetAsync(c->cl_loss, 0, (size_t)n * sizeof(double), c->stream));
@@ -1543,10 +1558,15 @@ extern "C" int tm_closure_dataset_add(tm_ctx *c, tm_mlp *mlp, const double *t_u,
     (void)hipFree(smp.tg);
     return c->fail(TM_ENOMEM, "hipMalloc closure sample");
   }
-  c->ds.push_back(smp);  // owned by the context from here on
   if ((rc = cl_features_now(c, mlp, smp.X)) || (rc = upload_perm_f32(c, smp.tu, t_u, c->pE)) ||
-      (rc = upload_perm_f32(c, smp.tg, t_gd, c->pC)))
+      (rc = upload_perm_f32(c, smp.tg, t_gd, c->pC))) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(smp.X);
+    (void)hipFree(smp.tu);
+    (void)hipFree(smp.tg);
     return rc;
+  }
+  c->ds.push_back(smp);  // filled; owned by the context from here on
   return TM_OK;
 }
 
@@ -1576,6 +1596,7 @@ extern "C" int tm_closure_eval(tm_ctx *c, tm_mlp *mlp, int first, int count, dou
   MlpState &ms = mlp_state(mlp);
   HIPCHK(c, hipStreamSynchronize(ms.stream));
   if ((rc = cl_reserve(c)) || (rc = cl_loss_reserve(c, 2))) return rc;
+  c->cl_step = 0;  // cl_Y is overwritten: an attached closure recomputes its correction next step
   for (int i = first; i < first + count; ++i) {
     if ((rc = mlp_forward(ms, c->ds[i].X, c->nE, c->cl_Y, false, c->stream))) return c->fail(rc, ms.err);
     if ((rc = cl_enqueue_loss(c, cl_loss_args(c, mlp, c->ds[i], false, c->cl_loss)))) return rc;
@@ -1596,6 +1617,7 @@ extern "C" int tm_closure_train(tm_ctx *c, tm_mlp *mlp, int ntrain, int epochs, 
   MlpState &ms = mlp_state(mlp);
   HIPCHK(c, hipStreamSynchronize(ms.stream));
   if ((rc = cl_reserve(c)) || (rc = cl_loss_reserve(c, 2 * epochs))) return rc;
+  c->cl_step = 0;  // cl_Y is overwritten: an attached closure recomputes its correction next step
   if ((rc = mlp_reserve(ms, c->nE)) || (rc = mlp_zero_grad(ms, c->stream))) return c->fail(rc, ms.err);
   std::vector<int> order(ntrain);
   std::iota(order.begin(), order.end(), 0);

@@ -115,7 +115,8 @@ class SwmGpu:
             setattr(o, k, v)
         self.opts = o
         self.mesh = mesh
-        if fit is None and interp_pv_order >= 3:
+        needs_fit = (o.pv_scheme == 3 and interp_pv_order >= 3) or o.adv_order >= 3
+        if fit is None and needs_fit:
             fit = mesh.fit(radius)
         self.fit = fit
         self.nC, self.nE, self.nV, self.nT = mesh.nCells, mesh.nEdges, mesh.nVertices, ntracers
