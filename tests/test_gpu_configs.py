@@ -1,0 +1,197 @@
+"""The benchmark configurations at their stated sizes, on the device.
+
+- Plain FP64 at level 8 (655,362 cells, dt 90 s, 4 tracers) for one
+  simulated day (960 steps) against the FP64 oracle: <= 1e-9 relative L2 and
+  the same invariant drift (the time loop tmcore_swm_mod.F90:75-88 with
+  time_integrate_CRK4, time_scheme_mod.F90:109-154).
+- Plain FP64 at level 5 (10,242 cells, dt 600 s, 1 tracer) for fifteen
+  simulated days (2,160 steps) against the oracle, with the per-step
+  invariant log of diag_run (diag_mod.F90:76-91).
+- Precision form at level 10 (10,485,762 cells, dt 22.5 s, 960 steps = six
+  hours): FP32 against the FP64 device run of the same seeded state, and one
+  FP64 level-10 step against the oracle (fields <= 1e-12; beta and the
+  invariants, sums of 4e7 terms that the oracle adds one by one, <= 1e-10).
+- Batched ensemble, 128 x level 6 + 32 x level 7 + 8 x level 8 in one
+  persistent launch: the first ten ticks of the two-day schedule, two members
+  per level against their own single-model runs.
+
+Long tests print their progress (visible with pytest -s).
+"""
+import time
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle
+from tmcore_amd.ensemble import Ensemble
+from tmcore_amd.mesh import RADIUS, Mesh, seeded_state
+from tmcore_amd.swm import SwmGpu
+
+pytestmark = pytest.mark.gpu
+
+OMEGA2 = 2 * 7.292e-5
+
+
+def rel_l2(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def _long_run_vs_oracle(level, dt, steps, seed, tracer_seeds, chunk):
+    """Run `steps` steps on the device and in the oracle from the same seeded
+    state; the oracle advances in chunks so that progress shows."""
+    t0 = time.perf_counter()
+    m = Mesh(level)
+    fit = m.fit()
+    gd, u, ghs, q = seeded_state(m, seed, tracer_seeds)
+    o = Oracle(m, fit, dt=dt)
+    iap = o.initial_iap(gd, u)
+    inv0 = o.invariants(gd, iap, ghs)
+    g = SwmGpu(m, fit, dt=dt, ntracers=len(tracer_seeds))
+    g.upload(gd, ghs, u=u, tracers=q if len(tracer_seeds) else None)
+    ginv, gbeta = g.step(steps)
+    st = g.download()
+    g.close()
+    say(f"level {level}: {steps} device steps done ({time.perf_counter() - t0:.1f} s)")
+    invs, betas = [], []
+    done = 0
+    while done < steps:
+        n = min(chunk, steps - done)
+        gd, iap, q, inv, beta = o.run(n, gd, iap, ghs, q if len(tracer_seeds) else None)
+        invs.append(inv)
+        betas.append(beta)
+        done += n
+        say(f"level {level}: oracle {done}/{steps} steps ({time.perf_counter() - t0:.1f} s)")
+    return (st, ginv, gbeta), (gd, iap, q, np.concatenate(invs), np.concatenate(betas)), inv0
+
+
+def _check_parity(dev, ora, inv0, ntracers, tol):
+    (st, ginv, gbeta), (gd, iap, q, inv, beta) = dev, ora
+    e = {"gd": rel_l2(st["gd"], gd), "iap_u": rel_l2(st["iap_u"], iap)}
+    if ntracers:
+        e["tracers"] = rel_l2(st["tracers"], q)
+    drift_o = inv[:, :3] / inv0[:3] - 1
+    drift_g = ginv[:, :3] / inv0[:3] - 1
+    e["drift"] = float(np.abs(drift_g - drift_o).max())
+    e["abs_vor"] = float(np.abs(ginv[:, 3] - inv[:, 3]).max())
+    e["beta"] = float(np.abs(gbeta - beta).max())
+    say("errors:", e, "final drift (mass, energy, enstrophy):", drift_o[-1])
+    for k in ("gd", "iap_u", "tracers"):
+        if k in e:
+            assert e[k] <= tol, (k, e[k])
+    assert e["drift"] <= 1e-12
+    assert e["abs_vor"] <= 1e-12 * OMEGA2
+    assert e["beta"] <= tol
+
+
+@pytest.mark.timeout(900)
+def test_level8_one_day_matches_oracle():
+    """Level 8, dt 90 s, 960 steps, tracers seeded 201-204: <= 1e-9 after one
+    simulated day, identical invariant drift at every step."""
+    dev, ora, inv0 = _long_run_vs_oracle(8, 90.0, 960, 2, [201, 202, 203, 204], chunk=96)
+    _check_parity(dev, ora, inv0, 4, 1e-9)
+
+
+@pytest.mark.timeout(600)
+def test_level5_fifteen_days_matches_oracle():
+    """Level 5, dt 600 s, 2,160 steps (15 simulated days), tracer seeded 101:
+    the per-step invariant log and the final state against the oracle."""
+    dev, ora, inv0 = _long_run_vs_oracle(5, 600.0, 2160, 1, [101], chunk=720)
+    _check_parity(dev, ora, inv0, 1, 1e-9)
+
+
+@pytest.mark.timeout(900)
+def test_level10_precision_form_against_fp64():
+    """Level 10 (10,485,762 cells), seed 3, dt 22.5 s.  One FP64 step against
+    the oracle (fields <= 1e-12, beta and invariants <= 1e-10); then 960
+    steps (six hours) in FP64 and in FP32.  FP32 against FP64: area-weighted
+    relative L2 of thickness <= 1e-6 and of the normal velocity <= 1e-5
+    (measured 2.1e-7 and 2.8e-6); the FP32 mass and energy drift within 1e-8
+    (compensated FP64 sums over FP32 state); the FP32 enstrophy drift within
+    10% of the FP64 one (measured -1.98e-8 against -2.09e-8)."""
+    t0 = time.perf_counter()
+    m = Mesh(10)
+    fit = m.fit()
+    gd, u, ghs, _ = seeded_state(m, 3)
+    say(f"level 10 mesh + fit + state: {time.perf_counter() - t0:.1f} s")
+    o = Oracle(m, fit, dt=22.5)
+    iap = o.initial_iap(gd, u)
+    gd1, iap1, _, inv1, beta1 = o.run(1, gd, iap, ghs)
+    del o
+    say(f"level 10 oracle step: {time.perf_counter() - t0:.1f} s")
+    aC = m.areaCell * RADIUS**2
+    aE = m.dvEdge * m.dcEdge * RADIUS**2
+    res, drift = {}, {}
+    for prec in (64, 32):
+        g = SwmGpu(m, fit, dt=22.5, precision=prec)
+        g.upload(gd, ghs, u=u)
+        i0 = g.invariants()
+        if prec == 64:
+            ginv, gbeta = g.step(1)
+            st = g.download()
+            assert rel_l2(st["gd"], gd1) <= 1e-12
+            assert rel_l2(st["iap_u"], iap1) <= 1e-12
+            # beta and the invariants are global sums over 31.5M edges and
+            # 10.5M cells: the oracle adds them one by one in FP64 as the
+            # reference does (rounding error up to n eps ~ 5e-9 of the sum),
+            # the device in double-double.  Measured: beta 1.35e-11 apart.
+            assert abs(gbeta[0] - beta1[0]) <= 1e-10
+            np.testing.assert_allclose(ginv[0, :3], inv1[0, :3], rtol=1e-10)
+            g.upload(gd, ghs, u=u)
+        for done in range(0, 960, 240):
+            g.step_async(240)
+            g.sync()
+            say(f"level 10 FP{prec}: {done + 240}/960 steps ({time.perf_counter() - t0:.1f} s)")
+        i1 = g.invariants()
+        res[prec] = g.download()
+        drift[prec] = (i1[:3] - i0[:3]) / i0[:3]
+        g.close()
+    d64, d32 = res[64], res[32]
+    eh = np.sqrt(np.sum(aC * (d32["gd"] - d64["gd"]) ** 2) / np.sum(aC * d64["gd"] ** 2))
+    eu = np.sqrt(np.sum(aE * (d32["u"] - d64["u"]) ** 2) / np.sum(aE * d64["u"] ** 2))
+    say(f"level 10 FP32 vs FP64 after 960 steps: h {eh:.3e}, u {eu:.3e}; drift FP64 {drift[64]}, FP32 {drift[32]}")
+    assert eh <= 1e-6 and eu <= 1e-5
+    assert abs(drift[64][0]) <= 1e-12 and abs(drift[64][1]) <= 1e-12
+    assert abs(drift[32][0]) <= 1e-8 and abs(drift[32][1]) <= 1e-8
+    assert abs(drift[32][2] - drift[64][2]) <= 0.1 * abs(drift[64][2])
+
+
+@pytest.mark.timeout(600)
+def test_ensemble_full_mix_members_match_single_runs():
+    """128 x L6 (dt 300 s) + 32 x L7 (150 s) + 8 x L8 (90 s), FP32, member m
+    seeded 1000 + m in level order.  The two-day schedule steps the levels
+    576 : 1152 : 1920 times; its first ten ticks are 3, 6 and 10 steps.  The
+    first and last member of every level match a single-model FP32 run to
+    <= 1e-5 (only the order of the global beta sums differs)."""
+    t0 = time.perf_counter()
+    spec = [(6, 128, 300.0, 3), (7, 32, 150.0, 6), (8, 8, 90.0, 10)]
+    meshes = [Mesh(L) for L, *_ in spec]
+    fits = [mm.fit() for mm in meshes]
+    ens = Ensemble([(mm, f, M, dt) for mm, f, (L, M, dt, _) in zip(meshes, fits, spec)], precision=32)
+    states, seed = {}, 1000
+    for li, (mm, (L, M, dt, _)) in enumerate(zip(meshes, spec)):
+        for k in range(M):
+            gd, u, ghs, _ = seeded_state(mm, seed)
+            seed += 1
+            ens.upload(li, k, gd, u, ghs)
+            if k in (0, M - 1):
+                states[li, k] = (gd, u, ghs)
+    say(f"ensemble of 168 members uploaded ({time.perf_counter() - t0:.1f} s)")
+    ens.run([s for *_, s in spec])
+    for (li, k), (gd, u, ghs) in states.items():
+        L, M, dt, steps = spec[li]
+        g = SwmGpu(meshes[li], fits[li], dt=dt, precision=32)
+        g.upload(gd, ghs, u=u)
+        inv, _ = g.step(steps)
+        ref = g.download()
+        got = ens.download(li, k)
+        g.close()
+        eg, ei = rel_l2(got["gd"], ref["gd"]), rel_l2(got["iap_u"], ref["iap_u"])
+        say(f"L{L} member {k}: gd {eg:.2e}, iap_u {ei:.2e}")
+        assert eg <= 1e-5 and ei <= 1e-5, (L, k)
+        np.testing.assert_allclose(ens.invariants(li, k)[:3], inv[-1, :3], rtol=1e-5)
+    ens.close()
