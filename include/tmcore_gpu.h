@@ -87,6 +87,18 @@ int tm_sync(tm_ctx *ctx);
 /* The four invariants of the current state (diag_run, diag_mod.F90:76-91). */
 int tm_invariants(tm_ctx *ctx, double out[4]);
 
+/* tm_invariants without device allocation or context scratch: no counterpart
+ * in the reference.  For callers that cannot allocate (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The block
+ * partials, sums and result go through a static device array of 3.25 MiB per
+ * GPU (kept until the process ends), so the context's mesh may have at most
+ * 4,194,304 cells, edges and vertices (16384 workgroups of 256 per element
+ * kind); larger meshes return TM_EINVAL with a message.  The refresh pass
+ * still writes the context's own edge, cell and vertex work arrays, as
+ * tm_invariants does.  Results equal tm_invariants'.  Every call on one GPU
+ * shares that GPU's array: calls that overlap in time on one GPU race. */
+int tm_invariants_noalloc(tm_ctx *ctx, double out[4]);
+
 /* Test path: one spatial_operators call (tmcore_swm_mod.F90:92-125) on the
  * current state plus the tracer tendencies, outputs in reference order; any
  * output pointer may be NULL.  Diagnostics mirror state/tend fields of the

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <random>
 #include <string>
@@ -25,7 +26,43 @@
 
 using namespace tmk;
 
+// ------------------------------------------ device scratch of tm_invariants_noalloc
+// The block partials (double-double, kNumSums x at most kNoAllocMaxBlocks),
+// the kNumSums global sums and the four invariants of tm_invariants_noalloc
+// live in this array, so that call allocates nothing.  Each GPU holds its own
+// copy for the life of the process; host code reaches it only through the
+// per-device address that hipGetSymbolAddress returns (noalloc_scratch_for).
+// Calls on one GPU share it: calls that overlap in time race.
+constexpr int kNoAllocMaxBlocks = 16384;  // workgroups per element kind: <= 4,194,304 elements
+constexpr size_t kNoAllocPartBytes = (size_t)kNumSums * kNoAllocMaxBlocks * sizeof(double2);
+constexpr size_t kNoAllocBytes = kNoAllocPartBytes + 256;  // partials, then sums[13] and inv[4]
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
 namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Address of `scratch` on `device` (which must be current), resolved once per
+// device; nullptr with err set on failure.
+void *noalloc_scratch_for(int device, std::string &err) {
+  if (device < 0 || device >= kMaxDevices) {
+    err = "device ordinal outside the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    g_scratch_ptr[device] = ptr;
+  }
+  return g_scratch_ptr[device];
+}
 
 // ------------------------------------------------------------- Hilbert order
 // Skilling, "Programming the Hilbert curve", AIP Conf. Proc. 707 (2004):
@@ -121,6 +158,7 @@ struct tm_ctx {
   hipGraph_t graph_k[5] = {};
   hipGraphExec_t gexec_k[5] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *noalloc_scratch = nullptr;  // this device's `scratch` (tm_invariants_noalloc)
   bool have_state = false;
   bool timed = false;
   int serpentine = 1;  // alternate the tile sweep direction between kernels
@@ -459,12 +497,27 @@ int enqueue_step(tm_ctx *c, bool infer, int apply_factor) {
   return TM_OK;
 }
 
-// Invariants of state(old) (the post-step refresh of tmcore_swm_mod.F90:78-85).
+// Invariants of state(old) (the post-step refresh of tmcore_swm_mod.F90:78-85):
+// the refresh stage, with its block partials in `part` (kNumSums x
+// c->part_stride), and the reduction into `sums`.
+template <typename R>
+int enqueue_refresh_sums(tm_ctx *c, double2 *part, double *sums) {
+  StageBufs<R> b = stage_bufs<R>(c, 0, true);
+  b.part = part;
+  int rc = launch_stage<R>(c, b, false);
+  if (rc) return rc;
+  ReduceArgs a = reduce_args(c, 0, true);
+  a.part = part;
+  a.sums = sums;
+  hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, a);
+  HIPCHK(c, hipGetLastError());
+  return TM_OK;
+}
+
 template <typename R>
 int enqueue_refresh(tm_ctx *c) {
-  int rc = launch_stage<R>(c, stage_bufs<R>(c, 0, true), false);
+  int rc = enqueue_refresh_sums<R>(c, c->part, c->sums);
   if (rc) return rc;
-  hipLaunchKernelGGL(k_reduce, dim3(kNumSums), dim3(kReduceBlock), 0, c->stream, reduce_args(c, 0, true));
   hipLaunchKernelGGL(k_inv_out, dim3(1), dim3(64), 0, c->stream, step_out(c, true));
   HIPCHK(c, hipGetLastError());
   return TM_OK;
@@ -949,6 +1002,10 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->nT = o.ntracers;
   c->rsz = o.precision == 64 ? 8 : 4;
   HIPCHK(c, hipSetDevice(o.device));
+  {
+    std::string e;
+    if (!(c->noalloc_scratch = noalloc_scratch_for(o.device, e))) return c->fail(TM_EDEVICE, e);
+  }
   HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
   HIPCHK(c, hipEventCreate(&c->ev0));
   HIPCHK(c, hipEventCreate(&c->ev1));
@@ -1230,6 +1287,39 @@ extern "C" int tm_invariants(tm_ctx *c, double out[4]) {
   int rc = c->rsz == 8 ? enqueue_refresh<double>(c) : enqueue_refresh<float>(c);
   if (rc) return rc;
   HIPCHK(c, hipMemcpyAsync(out, c->inv_hist, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
+}
+
+// The four invariants from the sums into inv[4].
+static __global__ void k_inv_write(StepOut o) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) write_invariants(o, 0);
+}
+
+extern "C" int tm_invariants_noalloc(tm_ctx *c, double out[4]) {
+  if (!c || !out) return TM_EINVAL;
+  if (!c->have_state) return c->fail(TM_ESTATE, "no state uploaded");
+  if (!c->noalloc_scratch) return c->fail(TM_ESTATE, "no device scratch for this context");
+  if (c->part_stride > kNoAllocMaxBlocks)
+    return c->fail(TM_EINVAL, "tm_invariants_noalloc: the mesh needs " + std::to_string(c->part_stride) +
+                                  " workgroups per element kind, the static scratch holds " +
+                                  std::to_string(kNoAllocMaxBlocks) +
+                                  " (at most 4,194,304 cells, edges and vertices); use tm_invariants");
+  unsigned char *base = static_cast<unsigned char *>(c->noalloc_scratch);
+  double2 *part = reinterpret_cast<double2 *>(base);
+  double *sums = reinterpret_cast<double *>(base + kNoAllocPartBytes);
+  double *inv = sums + kNumSums;
+  // every partial the reduction reads is written by this call's refresh
+  // pass, so nothing needs clearing
+  int rc = c->rsz == 8 ? enqueue_refresh_sums<double>(c, part, sums) : enqueue_refresh_sums<float>(c, part, sums);
+  if (rc) return rc;
+  StepOut o = step_out(c, true);
+  o.sums = sums;
+  o.inv_hist = inv;
+  o.hist_cap = 1;
+  hipLaunchKernelGGL(k_inv_write, dim3(1), dim3(64), 0, c->stream, o);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(out, inv, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return TM_OK;
 }

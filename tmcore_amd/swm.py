@@ -57,6 +57,8 @@ def gpu_lib() -> C.CDLL:
         L.tm_step_async.argtypes = [_vp, C.c_int]
         L.tm_sync.argtypes = [_vp]
         L.tm_invariants.argtypes = [_vp, _dp]
+        L.tm_invariants_noalloc.argtypes = [_vp, _dp]
+        L.tm_invariants_noalloc.restype = C.c_int
         L.tm_stage.argtypes = [_vp, C.POINTER(StageOut)]
         L.tm_last_elapsed_ms.argtypes = [_vp, _dp]
         L.tm_step_bytes.argtypes = [_vp, _dp]
@@ -202,6 +204,13 @@ class SwmGpu:
     def invariants(self):
         out = np.zeros(4)
         self._check(gpu_lib().tm_invariants(self.h, _p(out)), "tm_invariants")
+        return out
+
+    def invariants_noalloc(self):
+        """tm_invariants_noalloc: the invariants with no device allocation
+        (static per-GPU scratch; meshes up to 4,194,304 elements)."""
+        out = np.zeros(4)
+        self._check(gpu_lib().tm_invariants_noalloc(self.h, _p(out)), "tm_invariants_noalloc")
         return out
 
     def stage(self):
