@@ -60,6 +60,19 @@ void tm_opts_default(tm_opts *o);
  * (vertex fits) or adv_order >= 3 (cell fits) is selected. */
 int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts *o);
 int tm_ctx_destroy(tm_ctx *ctx);
+
+/* Kernel layout of the stage (no counterpart in the reference; results are
+ * bit-identical in every mode): bit 1, the PV kernel, and bit 2, the
+ * tendency kernel, derive each edge's edgesOnEdge slots from the edge rows
+ * of its two cells instead of streaming the edgesOnEdge table.  That needs
+ * the MPAS order of edgesOnEdge (the edges of cellsOnEdge(1) after the edge
+ * counter-clockwise, then those of cellsOnEdge(2)), which tm_ctx_create
+ * checks for every edge, and adv_order 2.  tm_ctx_create picks the faster
+ * mode measured for the context's precision when the order holds, else 0.
+ * Setting a mode the mesh or scheme does not allow returns TM_EINVAL.  For
+ * tests and timing comparisons. */
+int tm_ctx_set_cell_rows(tm_ctx *ctx, int mode);
+int tm_ctx_get_cell_rows(tm_ctx *ctx, int *mode);
 const char *tm_last_error(const tm_ctx *ctx);
 
 /* Prognostic upload (reference state(old), tmcore_swm_mod.F90:62-64):

@@ -425,3 +425,60 @@ def test_nonfinite_upload_is_rejected():
     with pytest.raises(Exception, match="non-finite"):
         g.upload(bad, ghs, u=u)
     g.close()
+
+
+@pytest.mark.parametrize("precision,ntracers,kw", [(64, 2, {}), (32, 1, {}), (64, 0, {"pv_scheme": 2}),
+                                                   (64, 0, {"time_scheme": 1, "pv_scheme": 1})])
+def test_cell_row_kernels_are_bit_identical(precision, ntracers, kw):
+    """edgesOnEdge derived from the cells' edge rows (in the PV kernel, the
+    tendency kernel, both) gives the slots of the table in the same order:
+    the stage diagnostics, ten steps, beta and the invariants are
+    bit-identical in all four modes."""
+    m = mesh(4)
+    gd, u, ghs, q = seeded_state(m, 5, [201, 202][:ntracers])
+    runs = {}
+    for mode in (0, 1, 2, 3):
+        g = SwmGpu(m, m.fit(), dt=600.0, ntracers=ntracers, precision=precision, **kw)
+        assert g.cell_rows != 0  # the generated mesh keeps the MPAS order
+        g.cell_rows = mode
+        assert g.cell_rows == mode
+        g.upload(gd, ghs, u=u, tracers=q if ntracers else None)
+        st = g.stage()
+        inv, beta = g.step(10)
+        runs[mode] = (st, g.download(), inv, beta, g.invariants())
+        g.close()
+    st0, s0, inv0, b0, i0 = runs[0]
+    for mode in (1, 2, 3):
+        st, s, inv, b, i = runs[mode]
+        for k, v in st0.items():
+            np.testing.assert_array_equal(st[k], v, err_msg=f"mode {mode} stage {k}")
+        for k in ("gd", "iap_u", "tracers"):
+            np.testing.assert_array_equal(s[k], s0[k], err_msg=f"mode {mode} {k}")
+        np.testing.assert_array_equal(b, b0)
+        np.testing.assert_array_equal(inv, inv0)
+        np.testing.assert_array_equal(i, i0)
+
+
+def test_mesh_without_mpas_edge_order_keeps_the_table_kernels():
+    """A mesh whose edgesOnEdge rows do not follow the MPAS order (one row's
+    slots rotated, with its weights) cannot use the derived rows: the
+    context keeps the table-driven kernels, refuses the cell-row modes, and
+    matches the oracle on the same reordered mesh."""
+    m = Mesh(3)
+    e = 7
+    n = m.nEdgesOnEdge[e]
+    m.edgesOnEdge[e, :n] = np.roll(m.edgesOnEdge[e, :n], 1)
+    m.weightsOnEdge[e, :n] = np.roll(m.weightsOnEdge[e, :n], 1)
+    gd, u, ghs, q = seeded_state(m, 2, [201])
+    g = SwmGpu(m, m.fit(), dt=600.0, ntracers=1)
+    assert g.cell_rows == 0
+    with pytest.raises(Exception):
+        g.cell_rows = 3
+    o = Oracle(m, m.fit(), dt=600.0)
+    iap = o.initial_iap(gd, u)
+    g.upload(gd, ghs, u=u, tracers=q)
+    gd1, iap1, q1, inv, beta = o.run(2, gd, iap, ghs, q)
+    g.step(2)
+    st = g.download()
+    assert rel_l2(st["gd"], gd1) <= 1e-12 and rel_l2(st["iap_u"], iap1) <= 1e-12
+    g.close()

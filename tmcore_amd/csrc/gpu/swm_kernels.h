@@ -195,6 +195,39 @@ __device__ __forceinline__ void eoe_row(const DMesh &m, int e, int (&ep)[kMaxEoE
   }
 }
 
+// Slot s (0..5) of a cell's edge row held in two int4 registers, s not a
+// compile-time constant: selects on the bits of s, so the row never goes
+// through private memory.
+__device__ __forceinline__ int row_slot(const int4 &a, const int4 &b, int s) {
+  const int lo = (s & 1) ? a.y : a.x, mid = (s & 1) ? a.w : a.z, hi = (s & 1) ? b.y : b.x;
+  return s < 2 ? lo : (s < 4 ? mid : hi);
+}
+
+// The edgesOnEdge slots of edge e derived from the edge rows of its cells c1
+// = cellsOnEdge(1) and c2 = cellsOnEdge(2) (the MPAS ordering checked at
+// upload): the edges of c1 that follow e counter-clockwise, then those of c2;
+// padding slots hold e itself (their weights are 0), as in the table.
+__device__ __forceinline__ void eoe_row_cells(const DMesh &m, int e, int c1, int c2, int (&ep)[kMaxEoE]) {
+  const int4 *r1 = reinterpret_cast<const int4 *>(m.eocR) + (size_t)c1 * 2;
+  const int4 *r2 = reinterpret_cast<const int4 *>(m.eocR) + (size_t)c2 * 2;
+  const int4 a0 = r1[0], a1 = r1[1], b0 = r2[0], b1 = r2[1];
+  const int n1 = a1.z, n2 = b1.z;  // slot 6 of a row: nEdgesOnCell
+  // e's slot in each row (slots past nEdgesOnCell hold 0 and are skipped)
+  const int k1 = unsign(a0.x) == e ? 0 : unsign(a0.y) == e ? 1 : unsign(a0.z) == e ? 2 :
+                 unsign(a0.w) == e ? 3 : unsign(a1.x) == e ? 4 : 5;
+  const int k2 = unsign(b0.x) == e ? 0 : unsign(b0.y) == e ? 1 : unsign(b0.z) == e ? 2 :
+                 unsign(b0.w) == e ? 3 : unsign(b1.x) == e ? 4 : 5;
+#pragma unroll
+  for (int j = 0; j < kMaxEoE; ++j) {
+    int s1 = k1 + 1 + j;
+    s1 -= s1 >= n1 ? n1 : 0;
+    int s2 = k2 + 1 + (j - (n1 - 1));
+    s2 -= s2 >= n2 ? n2 : 0;
+    const int v = j < n1 - 1 ? row_slot(a0, a1, s1) : row_slot(b0, b1, s2);
+    ep[j] = j < n1 + n2 - 2 ? unsign(v) : e;
+  }
+}
+
 template <typename T, int W>
 __device__ __forceinline__ val_t<T, W> tm_sqrt(val_t<T, W> x) {
   if constexpr (W == 1) {
@@ -323,6 +356,12 @@ using Member = Members<1>;
 // order-5/6 PV and the stage-descriptor time integration (CRK3, PC2; CRK4
 // too in that variant), all still checked at run time inside.
 constexpr int kXAdv = 1, kXExt = 2, kXAdv56 = 4;
+// Cell-row variants (meshes whose edgesOnEdge rows follow the MPAS order
+// checked at upload, dm.eocR set): k_edge_pv (kXEoePv) or k_edge_tend
+// (kXEoeTend) derives edgesOnEdge from the edge rows of the edge's two cells
+// instead of streaming the edgesOnEdge table.  Same slots in the same order:
+// results are bit-identical.
+constexpr int kXEoePv = 8, kXEoeTend = 16;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (f at offset base; padded slots add an exact 0).
@@ -721,7 +760,10 @@ __device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b,
     int ep[kMaxEoE];
     V uv[kMaxEoE];
     R w[kMaxEoE];
-    eoe_row(m, e, ep);
+    if constexpr ((X & kXEoePv) != 0)
+      eoe_row_cells(m, e, m.coe[e], m.coe[m.nE + e], ep);
+    else
+      eoe_row(m, e, ep);
 #pragma unroll
     for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
 #pragma unroll
@@ -763,7 +805,10 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
   int ep[kMaxEoE];
   V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
   R w[kMaxEoE];
-  eoe_row<true>(m, e, ep);
+  if constexpr ((X & kXEoeTend) != 0)
+    eoe_row_cells(m, e, nt_ld(&m.coe[e]), nt_ld(&m.coe[m.nE + e]), ep);
+  else
+    eoe_row<true>(m, e, ep);
 #pragma unroll
   for (int q = 0; q < kMaxEoE; ++q) {
     uv[q] = at.ld(b.u, ep[q]);

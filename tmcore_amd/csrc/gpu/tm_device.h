@@ -70,6 +70,8 @@ struct DMesh {
                           // reached through verticesOnEdge (a gather), not streamed
   const int *fit4V;       // [nV][fit_row(21)] fit4Vertices slots 1..21 (interp_pv_order 5, 6), else null
   const double *d4v;      // [2][21][nE] deriv4OnVertex slots 1..21
+  const int *eocR;        // [nC][8] cell-major edge rows: signed edgesOnCell slots 0-5, nEdgesOnCell, 0
+                          // (null when the mesh breaks the MPAS orderings they stand for)
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
   const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
   const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))

@@ -19,6 +19,9 @@ struct MeshDev {
   std::vector<int> iC, iE, iV;  // old -> new
   double totalArea = 0.0, sumAreaTri = 0.0;
   double sumEoE = 0.0, sumFitE = 0.0, sumFitV = 0.0, sumEoC = 0.0;  // stencil nnz
+  // every edgesOnEdge row is derivable from its cells' edgesOnCell rows (the
+  // MPAS order the cell-row kernels rely on); dm.eocR is set
+  bool cell_rows = false;
 };
 
 // Validates the reference-layout mesh (and the fit when o.interp_pv_order

@@ -112,6 +112,14 @@ std::vector<int> hilbert_order(int n, const double *x, const double *y, const do
   return p;
 }
 
+inline int unsign_host(int idx) { return idx < 0 ? ~idx : idx; }
+
+// Which edge kernels derive edgesOnEdge from the cells' rows by default
+// (kXEoePv, kXEoeTend), per state precision: measured at level 8 with four
+// tracers, deriving them speeds k_edge_pv up in both precisions, while the
+// FP64 k_edge_tend is faster streaming the table.
+int cell_rows_default(size_t rsz) { return rsz == 8 ? kXEoePv : kXEoePv | kXEoeTend; }
+
 int round_blocks(int n) {
   int nb = (n + kBlock - 1) / kBlock;
   return ((nb + kXcd - 1) / kXcd) * kXcd;
@@ -132,7 +140,6 @@ struct tm_ctx {
   std::vector<void *> allocs;
   double totalArea = 0.0, sumAreaTri = 0.0;
   double step_bytes[3] = {0, 0, 0};
-  double kbytes[TM_K_COUNT] = {};
   // state (element type R = double or float, stored untyped)
   void *gd_old = nullptr, *iap_old = nullptr, *q_old = nullptr, *ghs = nullptr;
   void *gdS[2] = {nullptr, nullptr}, *iap_s = nullptr, *qS[2] = {nullptr, nullptr};
@@ -386,6 +393,12 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
     case kXExt: launch_part_x<R, kXExt>(c, which, b); break;
     case kXAdv | kXExt: launch_part_x<R, kXAdv | kXExt>(c, which, b); break;
     case kXAdv | kXAdv56: launch_part_x<R, kXAdv | kXAdv56>(c, which, b); break;
+    case kXEoePv: launch_part_x<R, kXEoePv>(c, which, b); break;
+    case kXEoeTend: launch_part_x<R, kXEoeTend>(c, which, b); break;
+    case kXEoePv | kXEoeTend: launch_part_x<R, kXEoePv | kXEoeTend>(c, which, b); break;
+    case kXExt | kXEoePv: launch_part_x<R, kXExt | kXEoePv>(c, which, b); break;
+    case kXExt | kXEoeTend: launch_part_x<R, kXExt | kXEoeTend>(c, which, b); break;
+    case kXExt | kXEoePv | kXEoeTend: launch_part_x<R, kXExt | kXEoePv | kXEoeTend>(c, which, b); break;
     default: launch_part_x<R, kXAdv | kXAdv56 | kXExt>(c, which, b);
   }
 }
@@ -849,6 +862,36 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     }
   }
 
+  // ---- cell-major edge rows.  In the MPAS order, edgesOnEdge(:, e) lists
+  // the edges of cellsOnEdge(1, e) that follow e counter-clockwise, then those
+  // of cellsOnEdge(2, e), so the edge kernels can derive it from the two
+  // cells' rows instead of streaming the table.  Checked for every edge; a
+  // mesh whose rows differ keeps the table (md.cell_rows stays false).
+  std::vector<int> eocR;
+  {
+    bool ok = true;
+    for (int ne = 0; ne < nE && ok; ++ne) {
+      int der[kMaxEoE], nd = 0;
+      for (int side = 0; side < 2 && ok; ++side) {
+        const int c = coe[(size_t)side * nE + ne], n = nEoC[c];
+        int k = -1;
+        for (int q = n - 1; q >= 0; --q)  // the first slot holding e, as the kernels search
+          if (unsign_host(eocS[(size_t)q * nC + c]) == ne) k = q;
+        ok = k >= 0 && nd + n - 1 <= kMaxEoE;
+        for (int j = 1; j < n && ok; ++j) der[nd++] = unsign_host(eocS[(size_t)((k + j) % n) * nC + c]);
+      }
+      for (int q = 0; q < kMaxEoE && ok; ++q) ok = eoe[pair_at(q, ne, nE)] == (q < nd ? der[q] : ne);
+    }
+    md.cell_rows = ok;
+    if (ok) {
+      eocR.assign((size_t)8 * nC, 0);
+      for (int c = 0; c < nC; ++c) {
+        for (int q = 0; q < kMaxEoC; ++q) eocR[(size_t)c * 8 + q] = q < nEoC[c] ? eocS[(size_t)q * nC + c] : 0;
+        eocR[(size_t)c * 8 + 6] = nEoC[c];
+      }
+    }
+  }
+
   DMesh &d = md.dm;
   d.nC = nC;
   d.nE = nE;
@@ -971,6 +1014,10 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
   UP(vocS, vocS, int);
   UP(cocN, cocN, int);
   UP(kiteC, kiteC, double);
+  if (md.cell_rows)
+    UP(eocR, eocR, int);
+  else
+    d.eocR = nullptr;
 #undef UP
 
   md.sumEoE = md.sumFitE = md.sumFitV = md.sumEoC = 0.0;
@@ -1041,6 +1088,9 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
     return c->fail(TM_ENOMEM, "hipMalloc tracer edge values");
   c->xmask = (o.adv_order >= 3 ? kXAdv : 0) | (o.adv_order >= 5 ? kXAdv56 : 0) |
              (o.pv_scheme != 3 || o.interp_pv_order >= 5 || o.time_scheme != 2 ? kXExt : 0);
+  // edgesOnEdge derived from the cells' edge rows where the mesh keeps the
+  // MPAS order (not in the upwind-correction variants)
+  if (c->md.cell_rows && o.adv_order < 3) c->xmask |= cell_rows_default(c->rsz);
   c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
   if (o.pv_scheme == 2 &&
@@ -1069,18 +1119,31 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   c->step_bytes[0] = cst + 7.0 * rs * prog;
   c->step_bytes[1] = 3.0 * rs * prog;
   c->step_bytes[2] = 4.0 * c->step_bytes[0] + c->step_bytes[1];
-  // Compulsory bytes of one launch of each kernel (stage 2/3 pattern).
-  const double T = c->nT;
-  c->kbytes[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs) + nC * rs;
-  c->kbytes[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + nE * rs * 2.0 +
-                                (T > 0 ? 4.0 * sumEoC + nE * (rs + 8.0) : 0.0) + T * nC * 5.0 * rs +
-                                nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs;
-  c->kbytes[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + fsz + 2.0 * rs) + (4.0 + fsz) * sumEoE + fsz * sumFitE +
-                            nV * (rs + 1.0) + 4.0 * sumFitV;
-  c->kbytes[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + fsz + 9.0 * rs) + (4.0 + fsz) * sumEoE + nC * 2.0 * rs;
-  c->kbytes[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
-  c->kbytes[TM_K_FINALIZE] = 3.0 * rs * prog;
   return TM_OK;
+}
+
+// Compulsory bytes of one launch of each kernel (stage 2/3 pattern) in the
+// context's current kernel layout: each array the kernel touches counted
+// once, gathers at the size of the gathered array.
+static void kernel_bytes(const tm_ctx *c, double *kb) {
+  const double nC = c->nC, nE = c->nE, nV = c->nV, T = c->nT;
+  const double sumEoE = c->md.sumEoE, sumFitE = c->md.sumFitE, sumFitV = c->md.sumFitV, sumEoC = c->md.sumEoC;
+  const double fsz = c->rsz == 4 ? 4.0 : 8.0, rs = (double)c->rsz;
+  const double prog = nE + nC * (1 + T);
+  // edgesOnEdge: the table, or (cellsOnEdge plus) the cells' 32-byte edge rows
+  const double eoe_pv = (c->xmask & kXEoePv) ? nE * 8.0 + nC * 32.0 : 4.0 * sumEoE;
+  const double eoe_tend = (c->xmask & kXEoeTend) ? nC * 32.0 : 4.0 * sumEoE;
+  kb[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs) + nC * rs;
+  // cells, vertices, the edge products gathered (fk, u dcEdge) and the
+  // tracers' cellsOnCell, u and dvEdge
+  kb[TM_K_CELL_VERTEX] = nC * (1.0 + 8.0 + 10.0 * rs) + 4.0 * sumEoC + T * nC * 5.0 * rs +
+                         nV * (12.0 + 12.0 + 24.0 + 8.0 + 8.0 + rs) + nE * rs * 3.0 +
+                         (T > 0 ? 4.0 * sumEoC + nE * (rs + 8.0) : 0.0);
+  kb[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + fsz + 2.0 * rs) + eoe_pv + fsz * sumEoE + fsz * sumFitE + nV * (rs + 1.0) +
+                     4.0 * sumFitV;
+  kb[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + fsz + 9.0 * rs) + eoe_tend + fsz * sumEoE + nC * 2.0 * rs;
+  kb[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
+  kb[TM_K_FINALIZE] = 3.0 * rs * prog;
 }
 
 extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, const tm_opts *o) {
@@ -1095,6 +1158,24 @@ extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_d
     return rc;
   }
   *out = c;
+  return TM_OK;
+}
+
+extern "C" int tm_ctx_set_cell_rows(tm_ctx *c, int mode) {
+  if (!c) return TM_EINVAL;
+  if (mode < 0 || mode > 3) return c->fail(TM_EINVAL, "cell-row mode must be 0..3");
+  const int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0);
+  if (want && !c->md.cell_rows) return c->fail(TM_EINVAL, "the mesh's edgesOnEdge rows do not follow the MPAS order");
+  if (want && c->o.adv_order >= 3) return c->fail(TM_EINVAL, "the cell-row kernels do not run with adv_order >= 3");
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  drop_graphs(c);  // captured steps hold the previous variant
+  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend)) | want;
+  return TM_OK;
+}
+
+extern "C" int tm_ctx_get_cell_rows(tm_ctx *c, int *mode) {
+  if (!c || !mode) return TM_EINVAL;
+  *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0);
   return TM_OK;
 }
 
@@ -1211,7 +1292,7 @@ extern "C" int tm_step_bytes(tm_ctx *c, double bytes[3]) {
 
 extern "C" int tm_kernel_bytes(tm_ctx *c, double bytes[TM_K_COUNT]) {
   if (!c || !bytes) return TM_EINVAL;
-  for (int i = 0; i < TM_K_COUNT; ++i) bytes[i] = c->kbytes[i];
+  kernel_bytes(c, bytes);
   return TM_OK;
 }
 
