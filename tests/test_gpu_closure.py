@@ -308,3 +308,26 @@ def test_closure_training_reduces_loss_and_eval_matches_oracle():
     g.close()
     student.close()
     teacher.close()
+
+
+def test_training_is_bitwise_deterministic():
+    """The weight gradients are summed per fixed row range and the partials
+    added in a fixed order (no atomics): two trainings from the same start,
+    samples and shuffle seed give bit-identical parameters and losses."""
+    m = Mesh(3)
+    out = []
+    for _ in range(2):
+        g = SwmGpu(m, m.fit(), dt=600.0)
+        gd, u, ghs, _ = seeded_state(m, 9)
+        g.upload(gd, ghs, u=u)
+        mlp = Mlp(seed=3)
+        clo = Closure(g, mlp)
+        rng = np.random.default_rng(0)
+        for _ in range(3):
+            clo.add_sample(rng.standard_normal(m.nEdges) * 1e-5, rng.standard_normal(m.nCells) * 1e-3, 1e-3)
+        hist = clo.train(3, 4, 1e-3, seed=5)
+        out.append((np.array(mlp.params), np.array(hist)))
+        mlp.close()
+        g.close()
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    np.testing.assert_array_equal(out[0][1], out[1][1])

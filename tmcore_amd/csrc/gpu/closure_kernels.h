@@ -156,10 +156,13 @@ struct ClLossArgs {
   float tgs;          // thickness loss scale
   float *r;           // [nC] normalised thickness residual
   float *dY;          // [nE][2] dL/dY, or null (loss only)
-  double *loss;       // [2] += velocity and thickness loss terms
+  double *part;       // [nbE + nbC] per-block loss partials (velocity, then thickness)
+  double *loss;       // [2] += velocity and thickness loss terms (k_cl_loss_sum)
 };
 
-__device__ __forceinline__ void cl_block_add(double v, double *dst) {
+// The block's sum of v into its own partial slot (no atomics: the loss is
+// summed in a fixed order and is the same from run to run).
+__device__ __forceinline__ void cl_block_part(double v, double *slot) {
   __shared__ double sh[kBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
@@ -170,7 +173,7 @@ __device__ __forceinline__ void cl_block_add(double v, double *dst) {
     double s = 0.0;
 #pragma unroll
     for (int q = 0; q < kBlock / 64; ++q) s += sh[q];
-    atomicAdd(dst, s);
+    *slot = s;
   }
 }
 
@@ -183,7 +186,7 @@ __global__ __launch_bounds__(kBlock) void k_cl_resid(DMesh m, ClLossArgs a) {
     a.r[c] = r;
     l = (double)r * (double)r / (double)m.nC;
   }
-  cl_block_add(l, a.loss + 1);
+  cl_block_part(l, a.part + m.nbE + blockIdx.x);
 }
 
 // Needs k_cl_resid of the same outputs first (reads r).
@@ -202,7 +205,23 @@ __global__ __launch_bounds__(kBlock) void k_cl_dy(DMesh m, ClLossArgs a) {
       a.dY[2 * (size_t)e + 1] = g * (a.r[c2] / (float)m.areaCell[c2] - a.r[c1] / (float)m.areaCell[c1]);
     }
   }
-  cl_block_add(l, a.loss);
+  cl_block_part(l, a.part + blockIdx.x);
+}
+
+// loss[0] += the velocity partials, loss[1] += the thickness partials, each
+// in block order (one wave; lanes take fixed strides, then a fixed tree).
+__global__ void k_cl_loss_sum(DMesh m, const double *part, double *loss) {
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int n = t == 0 ? m.nbE : m.nbC;
+    const double *p = part + (t == 0 ? 0 : m.nbE);
+    double s = 0.0;
+    for (int i = lane; i < n; i += 64) s += p[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane == 0) loss[t] += s;
+  }
 }
 
 }  // namespace tmk

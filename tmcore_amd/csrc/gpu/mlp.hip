@@ -1,18 +1,30 @@
 // BF16 MFMA kernels of the per-edge closure MLP (gfx950).
 //
-// Matrix core: v_mfma_f32_16x16x16_bf16 (__builtin_amdgcn_mfma_f32_16x16x16bf16_1k)
-// on wave64.  Operand layout per lane l (0..63), with q = l / 16, r = l % 16:
-//   A (16x16, rows x k):  A[r][4q + 0..3]
-//   B (16x16, k x cols):  B[4q + 0..3][r]
-//   D (16x16 FP32):       D[4q + 0..3][r]
-// Weights are stored once per update as BF16 "packs" in exactly the B layout
-// (one 8-byte load per lane per MFMA), for both W (forward) and W^T
-// (backward through the layer).  Activations of a 16-row tile live in LDS,
-// row-major, so A fragments are one 8-byte LDS read per lane.
+// Matrix core: v_mfma_f32_16x16x32_bf16 (__builtin_amdgcn_mfma_f32_16x16x32_bf16,
+// gfx950's double-K form) on wave64.  Operands per lane l (0..63), with
+// q = l / 16, r = l % 16:
+//   A (16 x 32, m x k):  A[r][8q + 0..7]
+//   B (32 x 16, k x n):  B[8q + 0..7][r]
+//   D (16 x 16 FP32):    D[4q + 0..3][r]
+// (A and B are filled with the same k mapping, so the sum runs over the same
+// 32 k whatever order the unit takes them in.)
+//
+// Every product is formed transposed, D' = W^T H^T: the weights are the A
+// operand, the activation rows the B operand, so a lane's four results are
+// four consecutive features of one row and go to LDS or HBM as one 8-byte
+// store.  Weights are repacked once per update as BF16 "packs" in the A
+// layout (one 16-byte fragment per lane per MFMA): P(W) for the forward pass
+// (m over a layer's outputs, k over its inputs) and Q(W) for the backward
+// pass (m over inputs, k over outputs).  A workgroup (8 waves, one per CU)
+// copies the four packs it needs (84 KB) into LDS once and then loops over
+// 32-row tiles, one per wave at a time; a wave's activations stay in its own
+// LDS tile between layers.
 //
 // Layers: Z1 = X W1 + b1, H1 = GELU(Z1); Z2 = H1 W2 + b2, H2 = GELU(Z2);
-// Z3 = H2 W3 + b3, H3 = GELU(Z3); Y = H3 W4 + b4.  GELU is the exact form of
-// Hendrycks & Gimpel (2016), x * Phi(x), with Phi from erff.
+// Z3 = H2 W3 + b3, H3 = GELU(Z3); Y = H3 W4 + b4.  GELU is the erf form of
+// Hendrycks & Gimpel (2016), x * Phi(x), with erf from Abramowitz & Stegun
+// (1964) eq. 7.1.26 (|error| <= 1.5e-7, far below the BF16 rounding every
+// activation then takes): branch-free, and GELU' reuses its exponential.
 
 #include <hip/hip_runtime.h>
 
@@ -26,11 +38,17 @@
 
 namespace tmk {
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4_t __attribute__((ext_vector_type(4)));
 
-constexpr int kMlpBlock = 256;  // 4 waves, 16 rows each
-constexpr int kActStride = kMlpH + 8;
+constexpr int kSub = 1;                      // 16-row MFMA column blocks per wave tile
+constexpr int kTileRows = 16 * kSub;         // rows per wave tile
+constexpr int kMlpWaves = 16 / kSub;         // waves per workgroup (one workgroup per CU)
+constexpr int kMlpBlock = 64 * kMlpWaves;
+constexpr int kGradBlock = 512;              // k_mlp_grad: 8 waves, one 16-feature slice each
+constexpr int kActStride = kMlpH + 8;        // bf16 per LDS activation row (conflict-free 16-byte reads)
+constexpr int kIn1 = 64;                     // K of layer 1: 40 inputs padded to two 32-steps
 
 __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
   uint32_t u = __float_as_uint(x);
@@ -38,76 +56,137 @@ __device__ __forceinline__ uint16_t f2bf(float x) {  // round to nearest even
   return (uint16_t)(u >> 16);
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
-
-__device__ __forceinline__ float gelu(float x) { return 0.5f * x * (1.0f + erff(x * 0.70710678118654752f)); }
-__device__ __forceinline__ float gelu_d(float x) {
-  return 0.5f * (1.0f + erff(x * 0.70710678118654752f)) + x * 0.39894228040143268f * expf(-0.5f * x * x);
+typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+// Two floats to BF16 (round to nearest even, as f2bf) in one instruction
+// (v_cvt_pk_bf16_f32), a in the low half.
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+  const bf16x2_t v = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, v);
 }
 
-// pack index: [ks][nt][lane][4]
+// 1 + erf(x / sqrt(2)) = 2 Phi(x) and exp(-x^2 / 2).  A&S 7.1.26 for u >= 0:
+// erf(u) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-u^2), t = 1 /
+// (1 + p u); for x < 0 the complement Q itself is 2 Phi(x), without
+// cancellation.  Explicit fused multiply-adds (the files are built without
+// contraction), the hardware reciprocal and 2^y: the epilogue is the MLP's
+// VALU-bound part.
+struct Phi2 {
+  float two_phi, e;
+};
+__device__ __forceinline__ Phi2 phi2(float x) {
+  const float u = fabsf(x) * 0.70710678118654752f;
+  const float t = __builtin_amdgcn_rcpf(fmaf(0.3275911f, u, 1.0f));
+  const float poly =
+      t * fmaf(t, fmaf(t, fmaf(t, fmaf(t, 1.061405429f, -1.453152027f), 1.421413741f), -0.284496736f), 0.254829592f);
+  const float e = __builtin_amdgcn_exp2f((x * x) * -0.72134752044448170f);  // exp(-x^2 / 2) = exp(-u^2)
+  const float q = poly * e;
+  return Phi2{x >= 0.f ? 2.0f - q : q, e};
+}
+__device__ __forceinline__ float gelu(float x) { return (0.5f * x) * phi2(x).two_phi; }
+// GELU'(x) = Phi(x) + x phi(x), phi(x) = exp(-x^2 / 2) / sqrt(2 pi)
+__device__ __forceinline__ float gelu_d(float x) {
+  const Phi2 p = phi2(x);
+  return fmaf(x * 0.39894228040143268f, p.e, 0.5f * p.two_phi);
+}
+
+// Pack shapes: KS 32-deep k-steps x MT 16-wide m-tiles, 64 lane fragments of
+// 8 bf16 each; element (ks, mt, l, i) holds W[k][m] (P) or W[m][k] (Q) with
+// k = 32 ks + 8 (l / 16) + i and m = 16 mt + l % 16, zero outside the matrix.
 struct PackDesc {
   const float *W;  // row-major [Kin][Nout]
-  int Kin, Nout, KS, NT, transposed;
+  int Kin, Nout, KS, MT, q_form;
   uint16_t *out;
 };
 
 __global__ void k_mlp_pack(PackDesc d) {
-  const int total = d.KS * d.NT * 64 * 4;
+  const int total = d.KS * d.MT * 64 * 8;
   const int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= total) return;
-  const int i = idx & 3, lane = (idx >> 2) & 63, tile = idx >> 8;
-  const int nt = tile % d.NT, ks = tile / d.NT;
-  const int k = ks * 16 + 4 * (lane >> 4) + i, n = nt * 16 + (lane & 15);
+  const int i = idx & 7, lane = (idx >> 3) & 63, frag = idx >> 9;
+  const int mt = frag % d.MT, ks = frag / d.MT;
+  const int k = ks * 32 + 8 * (lane >> 4) + i, m = mt * 16 + (lane & 15);
   float v = 0.f;
-  if (!d.transposed) {
-    if (k < d.Kin && n < d.Nout) v = d.W[(size_t)k * d.Nout + n];
-  } else {
-    if (k < d.Nout && n < d.Kin) v = d.W[(size_t)n * d.Nout + k];
+  if (!d.q_form) {  // P: k over inputs, m over outputs
+    if (k < d.Kin && m < d.Nout) v = d.W[(size_t)k * d.Nout + m];
+  } else {  // Q: m over inputs, k over outputs
+    if (m < d.Kin && k < d.Nout) v = d.W[(size_t)m * d.Nout + k];
   }
   d.out[idx] = f2bf(v);
 }
 
-// acc[nt] = A(16 x 16*KS, LDS row-major) x B(pack)
-template <int KS, int NT>
-__device__ __forceinline__ void wave_mma(const uint16_t (*A)[kActStride], const bf16x4_t *pack, f32x4_t (&acc)[NT],
-                                         int lane) {
+// Fragment counts (16-byte lane fragments = KS * MT * 64) of the packs.
+constexpr int kP1 = 2 * 8 * 64, kP2 = 4 * 8 * 64, kP4 = 4 * 1 * 64;  // P(W1), P(W2) = P(W3), P(W4)
+constexpr int kQ1 = 4 * 3 * 64, kQ2 = 4 * 8 * 64, kQ4 = 1 * 8 * 64;  // Q(W1), Q(W2) = Q(W3), Q(W4)
+constexpr int kPackFrags = kP1 + 2 * kP2 + kP4;                         // per pass (P or Q): 5376 = 84 KB
+static_assert(kQ1 + 2 * kQ2 + kQ4 == kPackFrags, "P and Q packs take the same LDS");
+
+struct MlpPtrs {
+  const bf16x8_t *p1, *p2, *p3, *p4, *q1, *q2, *q3, *q4;
+  const float *params;
+};
+
+// Copies a pass's four packs (laid out back to back, kPackFrags fragments)
+// into LDS.
+__device__ __forceinline__ void stage_packs(bf16x8_t *lds, const bf16x8_t *g) {
+  for (int i = threadIdx.x; i < kPackFrags; i += kMlpBlock) lds[i] = g[i];
+}
+
+constexpr int kNBias = 3 * kMlpH + kMlpOut;  // b1, b2, b3, b4 back to back in LDS
+
+// acc[s][mt] = pack(ks, mt) x act^T over the wave's 16-row blocks s.
+template <int KS, int MT>
+__device__ __forceinline__ void wave_mma(const bf16x8_t *pack, const uint16_t (*act)[kActStride],
+                                         f32x4_t (&acc)[kSub][MT], int lane) {
 #pragma unroll
-  for (int nt = 0; nt < NT; ++nt) acc[nt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  for (int s = 0; s < kSub; ++s)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[s][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks) {
-    const uint16_t *ap = &A[lane & 15][ks * 16 + 4 * (lane >> 4)];
-    const bf16x4_t a = *reinterpret_cast<const bf16x4_t *>(ap);
+    const int col = ks * 32 + 8 * (lane >> 4);
+    bf16x8_t b[kSub];
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const bf16x4_t b = pack[(ks * NT + nt) * 64 + lane];
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
+    for (int s = 0; s < kSub; ++s) b[s] = *reinterpret_cast<const bf16x8_t *>(&act[16 * s + (lane & 15)][col]);
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const bf16x8_t a = pack[(ks * MT + mt) * 64 + lane];
+#pragma unroll
+      for (int s = 0; s < kSub; ++s) acc[s][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[s], acc[s][mt], 0, 0, 0);
     }
   }
 }
 
-struct MlpPtrs {
-  const bf16x4_t *f1, *f2, *f3, *f4, *t1, *t2, *t3, *t4;
-  const float *params;
-};
-
-// Hidden layer epilogue: bias + GELU, H (and GELU') to global when training,
-// H as the next A operand in LDS.
-__device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[8], const float *bias, uint16_t (*act)[kActStride],
-                                                int lane, int row0, int n, bool train, uint16_t *H, uint16_t *D) {
+// Hidden-layer epilogue: bias + GELU into the wave's LDS tile (the next
+// layer's B operand); when training, H and GELU'(Z) to HBM (BF16 [n][128]).
+__device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[kSub][8], const float *bias,
+                                                uint16_t (*act)[kActStride], int lane, int row0, int n, bool train,
+                                                uint16_t *H, uint16_t *D) {
+  const int q = lane >> 4, r = lane & 15;
 #pragma unroll
-  for (int nt = 0; nt < 8; ++nt) {
-    const int col = nt * 16 + (lane & 15);
-    const float b = bias[col];
+  for (int mt = 0; mt < 8; ++mt) {
+    const int f = mt * 16 + 4 * q;
+    const f32x4_t b = *reinterpret_cast<const f32x4_t *>(bias + f);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = 4 * (lane >> 4) + i;
-      const float z = acc[nt][i] + b;
-      const float h = gelu(z);
-      const uint16_t hb = f2bf(h);
-      act[r][col] = hb;
-      if (train && row0 + r < n) {
-        H[(size_t)(row0 + r) * kMlpH + col] = hb;
-        D[(size_t)(row0 + r) * kMlpH + col] = f2bf(gelu_d(z));
+    for (int s = 0; s < kSub; ++s) {
+      float g[4], gd[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float z = acc[s][mt][i] + b[i];
+        if (train) {
+          const Phi2 ph = phi2(z);
+          g[i] = (0.5f * z) * ph.two_phi;
+          gd[i] = fmaf(z * 0.39894228040143268f, ph.e, 0.5f * ph.two_phi);
+        } else {
+          g[i] = gelu(z);
+        }
+      }
+      const uint2 h = make_uint2(pk_bf16(g[0], g[1]), pk_bf16(g[2], g[3]));
+      const int row = s * 16 + r;
+      *reinterpret_cast<uint2 *>(&act[row][f]) = h;
+      if (train && row0 + row < n) {
+        *reinterpret_cast<uint2 *>(&H[(size_t)(row0 + row) * kMlpH + f]) = h;
+        *reinterpret_cast<uint2 *>(&D[(size_t)(row0 + row) * kMlpH + f]) =
+            make_uint2(pk_bf16(gd[0], gd[1]), pk_bf16(gd[2], gd[3]));
       }
     }
   }
@@ -115,144 +194,293 @@ __device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[8], const f
 
 __global__ __launch_bounds__(kMlpBlock) void k_mlp_fwd(MlpPtrs p, const float *__restrict__ X, int n,
                                                        float *__restrict__ Y, int train, uint16_t *H, uint16_t *D) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[kMlpBlock / 64][16][kActStride];
+  __shared__ bf16x8_t packs[kPackFrags];
+  __shared__ __attribute__((aligned(16))) uint16_t act[kMlpWaves][kTileRows][kActStride];
+  __shared__ __attribute__((aligned(16))) float bias[kNBias];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row0 = (blockIdx.x * (kMlpBlock / 64) + wave) * 16;
+  stage_packs(packs, p.p1);
+  for (int i = threadIdx.x; i < kNBias; i += kMlpBlock) {
+    const int l = i / kMlpH;
+    bias[i] = p.params[(l == 0 ? kOB1 : l == 1 ? kOB2 : l == 2 ? kOB3 : kOB4) + i % kMlpH];
+  }
+  __syncthreads();
+  const bf16x8_t *P1 = packs, *P2 = P1 + kP1, *P3 = P2 + kP2, *P4 = P3 + kP2;
   uint16_t(*A)[kActStride] = act[wave];
-  for (int i = lane; i < 16 * kMlpInP; i += 64) {
-    const int r = i / kMlpInP, k = i % kMlpInP;
-    const float v = (row0 + r < n && k < kMlpIn) ? X[(size_t)(row0 + r) * kMlpIn + k] : 0.f;
-    A[r][k] = f2bf(v);
-  }
-  __syncthreads();
   const size_t plane = (size_t)n * kMlpH;
-  f32x4_t acc[8];
-  wave_mma<3, 8>(A, p.f1, acc, lane);
-  hidden_epilogue(acc, p.params + kOB1, A, lane, row0, n, train, H, D);
-  __syncthreads();
-  wave_mma<8, 8>(A, p.f2, acc, lane);
-  __syncthreads();
-  hidden_epilogue(acc, p.params + kOB2, A, lane, row0, n, train, H + plane, D + plane);
-  __syncthreads();
-  wave_mma<8, 8>(A, p.f3, acc, lane);
-  __syncthreads();
-  hidden_epilogue(acc, p.params + kOB3, A, lane, row0, n, train, H + 2 * plane, D + 2 * plane);
-  __syncthreads();
-  f32x4_t out[1];
-  wave_mma<8, 1>(A, p.f4, out, lane);
-  const int col = lane & 15;
-  if (col < kMlpOut) {
-    const float b = p.params[kOB4 + col];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = 4 * (lane >> 4) + i;
-      if (row0 + r < n) Y[(size_t)(row0 + r) * kMlpOut + col] = out[0][i] + b;
+  const int ntiles = (n + kTileRows - 1) / kTileRows;
+  for (int tile = blockIdx.x * kMlpWaves + wave; tile < ntiles; tile += gridDim.x * kMlpWaves) {
+    const int row0 = tile * kTileRows;
+    // X (FP32, 40 per row) -> BF16 tile columns 0..39, zeros to 63
+    for (int i = lane; i < kTileRows * (kMlpIn / 4); i += 64) {
+      const int r = i / (kMlpIn / 4), c = 4 * (i % (kMlpIn / 4));
+      f32x4_t x = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      if (row0 + r < n) x = *reinterpret_cast<const f32x4_t *>(&X[(size_t)(row0 + r) * kMlpIn + c]);
+      *reinterpret_cast<uint2 *>(&A[r][c]) = make_uint2(pk_bf16(x[0], x[1]), pk_bf16(x[2], x[3]));
     }
-  }
-}
-
-// dZ of a hidden layer = (upstream A x W^T) * GELU'(Z); kept in LDS and saved.
-__device__ __forceinline__ void dz_epilogue(const f32x4_t (&acc)[8], const uint16_t *D, uint16_t (*act)[kActStride],
-                                            int lane, int row0, int n, uint16_t *dZ) {
+    for (int i = lane; i < kTileRows * 3; i += 64)
+      *reinterpret_cast<f32x4_t *>(&A[i / 3][kMlpIn + 8 * (i % 3)]) = f32x4_t{0.f, 0.f, 0.f, 0.f};
+    f32x4_t acc[kSub][8];
+    wave_mma<2, 8>(P1, A, acc, lane);
+    hidden_epilogue(acc, bias, A, lane, row0, n, train, H, D);
+    wave_mma<4, 8>(P2, A, acc, lane);
+    hidden_epilogue(acc, bias + kMlpH, A, lane, row0, n, train, H + plane, D + plane);
+    wave_mma<4, 8>(P3, A, acc, lane);
+    hidden_epilogue(acc, bias + 2 * kMlpH, A, lane, row0, n, train, H + 2 * plane, D + 2 * plane);
+    f32x4_t out[kSub][1];
+    wave_mma<4, 1>(P4, A, out, lane);
+    if (lane < 16) {  // outputs 0 and 1 are features 4q + 0, 1 of q = 0
 #pragma unroll
-  for (int nt = 0; nt < 8; ++nt) {
-    const int col = nt * 16 + (lane & 15);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = 4 * (lane >> 4) + i;
-      float dz = 0.f;
-      if (row0 + r < n) dz = acc[nt][i] * bf2f(D[(size_t)(row0 + r) * kMlpH + col]);
-      const uint16_t b = f2bf(dz);
-      act[r][col] = b;
-      if (row0 + r < n) dZ[(size_t)(row0 + r) * kMlpH + col] = b;
-    }
-  }
-}
-
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_bwd(MlpPtrs p, const float *__restrict__ dY, int n,
-                                                       const uint16_t *D, uint16_t *dZ, uint16_t *dYb,
-                                                       float *__restrict__ dX) {
-  __shared__ __attribute__((aligned(16))) uint16_t act[kMlpBlock / 64][16][kActStride];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int row0 = (blockIdx.x * (kMlpBlock / 64) + wave) * 16;
-  uint16_t(*A)[kActStride] = act[wave];
-  for (int i = lane; i < 16 * 16; i += 64) {
-    const int r = i / 16, k = i % 16;
-    const float v = (row0 + r < n && k < kMlpOut) ? dY[(size_t)(row0 + r) * kMlpOut + k] : 0.f;
-    const uint16_t b = f2bf(v);
-    A[r][k] = b;
-    if (row0 + r < n) dYb[(size_t)(row0 + r) * kMlpOutP + k] = b;
-  }
-  __syncthreads();
-  const size_t plane = (size_t)n * kMlpH;
-  f32x4_t acc[8];
-  wave_mma<1, 8>(A, p.t4, acc, lane);  // dH3 = dY W4^T
-  __syncthreads();
-  dz_epilogue(acc, D + 2 * plane, A, lane, row0, n, dZ + 2 * plane);
-  __syncthreads();
-  wave_mma<8, 8>(A, p.t3, acc, lane);  // dH2 = dZ3 W3^T
-  __syncthreads();
-  dz_epilogue(acc, D + plane, A, lane, row0, n, dZ + plane);
-  __syncthreads();
-  wave_mma<8, 8>(A, p.t2, acc, lane);  // dH1 = dZ2 W2^T
-  __syncthreads();
-  dz_epilogue(acc, D, A, lane, row0, n, dZ);
-  __syncthreads();
-  if (dX) {
-    f32x4_t ax[3];
-    wave_mma<8, 3>(A, p.t1, ax, lane);  // dX = dZ1 W1^T
-#pragma unroll
-    for (int nt = 0; nt < 3; ++nt) {
-      const int col = nt * 16 + (lane & 15);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = 4 * (lane >> 4) + i;
-        if (col < kMlpIn && row0 + r < n) dX[(size_t)(row0 + r) * kMlpIn + col] = ax[nt][i];
+      for (int s = 0; s < kSub; ++s) {
+        const int row = row0 + s * 16 + lane;
+        if (row < n) {
+          float2 y;
+          y.x = out[s][0][0] + bias[3 * kMlpH];
+          y.y = out[s][0][1] + bias[3 * kMlpH + 1];
+          *reinterpret_cast<float2 *>(&Y[(size_t)row * kMlpOut]) = y;
+        }
       }
     }
   }
 }
 
-// dW[k][j] += sum_rows A[row][k] dZ[row][j]  (MFMA over the row dimension).
-// One wave per 16x16 output tile and row split; A is BF16 [n][lda] or, for
-// layer 1, FP32 X [n][40].
-__global__ __launch_bounds__(64) void k_mlp_wgrad(const uint16_t *A, const float *Xf, int lda, int Kin,
-                                                  const uint16_t *dZ, int ldz, int Nout, int n, int rows_per_split,
-                                                  float *dW) {
-  const int ntk = (Kin + 15) / 16;
-  const int tk = blockIdx.x % ntk, tj = blockIdx.x / ntk;
-  const int lane = threadIdx.x;
-  const int r_begin = blockIdx.y * rows_per_split, r_end = min(n, r_begin + rows_per_split);
-  const int kin = tk * 16 + (lane & 15), col = tj * 16 + (lane & 15);
-  f32x4_t acc{0.f, 0.f, 0.f, 0.f};
-  for (int r0 = r_begin; r0 < r_end; r0 += 16) {
-    bf16x4_t a, b;
+// dZ of a hidden layer = (upstream x W^T) * GELU'(Z) (GELU' saved by the
+// training forward), BF16, into the wave's LDS tile and to HBM for the
+// weight gradients.
+__device__ __forceinline__ void dz_epilogue(const f32x4_t (&acc)[kSub][8], const uint16_t *D, uint16_t (*act)[kActStride],
+                                            int lane, int row0, int n, uint16_t *dZ) {
+  const int q = lane >> 4, r = lane & 15;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int row = r0 + 4 * (lane >> 4) + i;
-      const bool ok = row < r_end;
-      uint16_t av = 0, bv = 0;
-      if (ok && kin < Kin) av = Xf ? f2bf(Xf[(size_t)row * kMlpIn + kin]) : A[(size_t)row * lda + kin];
-      if (ok && col < Nout) bv = dZ[(size_t)row * ldz + col];
-      a[i] = (short)av;
-      b[i] = (short)bv;
+  for (int mt = 0; mt < 8; ++mt) {
+    const int f = mt * 16 + 4 * q;
+#pragma unroll
+    for (int s = 0; s < kSub; ++s) {
+      const int row = s * 16 + r;
+      const bool live = row0 + row < n;
+      u16x4_t d = {0, 0, 0, 0};
+      if (live) d = *reinterpret_cast<const u16x4_t *>(&D[(size_t)(row0 + row) * kMlpH + f]);
+      float v[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) v[i] = acc[s][mt][i] * bf2f(d[i]);  // 0 on dead rows (D = 0)
+      const uint2 z = make_uint2(pk_bf16(v[0], v[1]), pk_bf16(v[2], v[3]));
+      *reinterpret_cast<uint2 *>(&act[row][f]) = z;
+      if (live) *reinterpret_cast<uint2 *>(&dZ[(size_t)(row0 + row) * kMlpH + f]) = z;
     }
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int m = tk * 16 + 4 * (lane >> 4) + i;
-    if (m < Kin && col < Nout) atomicAdd(&dW[(size_t)m * Nout + col], acc[i]);
   }
 }
 
-// db[j] += sum_rows dZ[row][j] (BF16 dZ), FP32 atomics per block.
-__global__ void k_mlp_bgrad(const uint16_t *dZ, int ldz, int N, int n, float *db) {
-  const int j = threadIdx.x;
-  if (j >= N) return;
+__global__ __launch_bounds__(kMlpBlock) void k_mlp_bwd(MlpPtrs p, const float *__restrict__ dY, int n,
+                                                       const uint16_t *D, uint16_t *dZ, float *__restrict__ dX) {
+  __shared__ bf16x8_t packs[kPackFrags];
+  __shared__ __attribute__((aligned(16))) uint16_t act[kMlpWaves][kTileRows][kActStride];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  stage_packs(packs, p.q1);
+  __syncthreads();
+  const bf16x8_t *Q1 = packs, *Q2 = Q1 + kQ1, *Q3 = Q2 + kQ2, *Q4 = Q3 + kQ2;
+  uint16_t(*A)[kActStride] = act[wave];
+  const size_t plane = (size_t)n * kMlpH;
+  const int ntiles = (n + kTileRows - 1) / kTileRows;
+  for (int tile = blockIdx.x * kMlpWaves + wave; tile < ntiles; tile += gridDim.x * kMlpWaves) {
+    const int row0 = tile * kTileRows;
+    // dY (FP32, 2 per row) -> BF16 tile columns 0, 1, zeros to 31
+    for (int i = lane; i < kTileRows * 4; i += 64) {
+      const int r = i / 4, c = 8 * (i % 4);
+      u16x4_t lo = {0, 0, 0, 0};
+      if (c == 0 && row0 + r < n) {
+        const float2 y = *reinterpret_cast<const float2 *>(&dY[(size_t)(row0 + r) * kMlpOut]);
+        lo[0] = f2bf(y.x);
+        lo[1] = f2bf(y.y);
+      }
+      *reinterpret_cast<u16x4_t *>(&A[r][c]) = lo;
+      *reinterpret_cast<u16x4_t *>(&A[r][c + 4]) = u16x4_t{0, 0, 0, 0};
+    }
+    f32x4_t acc[kSub][8];
+    wave_mma<1, 8>(Q4, A, acc, lane);  // dH3 = dY W4^T
+    dz_epilogue(acc, D + 2 * plane, A, lane, row0, n, dZ + 2 * plane);
+    wave_mma<4, 8>(Q3, A, acc, lane);  // dH2 = dZ3 W3^T
+    dz_epilogue(acc, D + plane, A, lane, row0, n, dZ + plane);
+    wave_mma<4, 8>(Q2, A, acc, lane);  // dH1 = dZ2 W2^T
+    dz_epilogue(acc, D, A, lane, row0, n, dZ);
+    if (dX) {
+      f32x4_t ax[kSub][3];
+      wave_mma<4, 3>(Q1, A, ax, lane);  // dX = dZ1 W1^T
+      const int q = lane >> 4, r = lane & 15;
+#pragma unroll
+      for (int mt = 0; mt < 3; ++mt) {
+        const int f = mt * 16 + 4 * q;
+#pragma unroll
+        for (int s = 0; s < kSub; ++s) {
+          const int row = row0 + s * 16 + r;
+          if (f < kMlpIn && row < n) *reinterpret_cast<f32x4_t *>(&dX[(size_t)row * kMlpIn + f]) = ax[s][mt];
+        }
+      }
+    }
+  }
+}
+
+// Weight and bias gradients, deterministic: workgroup g sums its own fixed
+// row range [g rps, (g + 1) rps) into part[g][kMlpNParams], and k_mlp_gsum
+// adds the partials in workgroup order.  dW[k][j] = sum_rows A[row][k]
+// dZ[row][j] is an MFMA with the rows as its k: 64-row chunks of every layer's
+// input (X, H1, H2, H3) and output gradient (dZ1, dZ2, dZ3, dY) are staged in
+// LDS transposed, feature-major, so both operands are 16-byte fragment reads.
+// Wave w owns, for W2 and W3, the 16 inputs 16w..16w+15 (8 output tiles
+// each), for W1 the outputs 16w..16w+15 (3 input tiles) and for W4 the
+// inputs 16w..: 20 tiles, accumulated over the whole range in registers.
+constexpr int kGradRows = 64;              // rows per staged chunk
+constexpr int kTStride = kGradRows + 8;    // bf16 per transposed LDS row
+
+struct GradArgs {
+  const float *X;         // [n][40]
+  const uint16_t *H;      // [3][n][128]
+  const uint16_t *dZ;     // [3][n][128]
+  const float *dY;        // [n][2]
+  int n, rps;
+  float *part;            // [gridDim.x][kMlpNParams]
+};
+
+// Rows [row0, row0 + 64) of a BF16 [n][128] array into T[128][kTStride]
+// (feature-major); two rows per thread and item so that each LDS store is a
+// 4-byte pair; rows past n are zero.
+__device__ __forceinline__ void stage_t128(uint16_t (*T)[kTStride], const uint16_t *src, int row0, int n) {
+  for (int it = threadIdx.x; it < (kGradRows / 2) * 16; it += kGradBlock) {
+    const int pr = it % (kGradRows / 2), fc = it / (kGradRows / 2);
+    const int ra = row0 + 2 * pr, rb = ra + 1;
+    u16x4_t a0 = {0, 0, 0, 0}, a1 = a0, b0 = a0, b1 = a0;
+    if (ra < n) {
+      a0 = *reinterpret_cast<const u16x4_t *>(&src[(size_t)ra * kMlpH + 8 * fc]);
+      a1 = *reinterpret_cast<const u16x4_t *>(&src[(size_t)ra * kMlpH + 8 * fc + 4]);
+    }
+    if (rb < n) {
+      b0 = *reinterpret_cast<const u16x4_t *>(&src[(size_t)rb * kMlpH + 8 * fc]);
+      b1 = *reinterpret_cast<const u16x4_t *>(&src[(size_t)rb * kMlpH + 8 * fc + 4]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      *reinterpret_cast<uint32_t *>(&T[8 * fc + i][2 * pr]) = (uint32_t)a0[i] | ((uint32_t)b0[i] << 16);
+      *reinterpret_cast<uint32_t *>(&T[8 * fc + 4 + i][2 * pr]) = (uint32_t)a1[i] | ((uint32_t)b1[i] << 16);
+    }
+  }
+}
+
+// acc[t] += A(16 x 64 rows, LDS feature-major at row a0) x B(64 rows x 16, at
+// rows b0[t]) over the chunk's two 32-row k-steps.
+template <int NT>
+__device__ __forceinline__ void grad_mma(const uint16_t (*TA)[kTStride], int a0, const uint16_t (*TB)[kTStride],
+                                         const int (&b0)[NT], f32x4_t (&acc)[NT], int lane) {
+#pragma unroll
+  for (int ks = 0; ks < kGradRows / 32; ++ks) {
+    const int col = ks * 32 + 8 * (lane >> 4);
+    const bf16x8_t a = *reinterpret_cast<const bf16x8_t *>(&TA[a0 + (lane & 15)][col]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8_t b = *reinterpret_cast<const bf16x8_t *>(&TB[b0[t] + (lane & 15)][col]);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[t], 0, 0, 0);
+    }
+  }
+}
+
+// A 16 x 16 gradient tile (rows: inputs m0.., columns: outputs j0..) into
+// the flat parameter layout W[m][j] at woff.
+__device__ __forceinline__ void store_tile(float *part, int woff, int Kin, int Nout, int m0, int j0, const f32x4_t &t,
+                                           int lane) {
+  const int j = j0 + (lane & 15);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int m = m0 + 4 * (lane >> 4) + i;
+    if (m < Kin && j < Nout) part[woff + m * Nout + j] = t[i];
+  }
+}
+
+__global__ __launch_bounds__(kGradBlock) void k_mlp_grad(GradArgs g) {
+  __shared__ __attribute__((aligned(16))) uint16_t TX[48][kTStride];      // X^T (40 inputs, padded)
+  __shared__ __attribute__((aligned(16))) uint16_t TH[3][kMlpH][kTStride];  // H1^T, H2^T, H3^T
+  __shared__ __attribute__((aligned(16))) uint16_t TZ[3][kMlpH][kTStride];  // dZ1^T, dZ2^T, dZ3^T
+  __shared__ __attribute__((aligned(16))) uint16_t TY[16][kTStride];      // dY^T (2 outputs, padded)
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int r_begin = blockIdx.x * g.rps, r_end = min(g.n, r_begin + g.rps);
+  const size_t plane = (size_t)g.n * kMlpH;
+  f32x4_t a2[8], a3[8], a1[3], a4[1];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) a2[t] = a3[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < 3; ++t) a1[t] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  a4[0] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+  // bias gradients: thread f < 386 sums one output feature over the rows
+  const int f = threadIdx.x;
+  float bsum = 0.f;
+  for (int row0 = r_begin; row0 < r_end; row0 += kGradRows) {
+    __syncthreads();  // the previous chunk's reads are done
+    for (int l = 0; l < 3; ++l) {
+      stage_t128(TH[l], g.H + l * plane, row0, r_end);
+      stage_t128(TZ[l], g.dZ + l * plane, row0, r_end);
+    }
+    for (int it = threadIdx.x; it < kGradRows * 12; it += kGradBlock) {  // X: 64 rows x 12 float4 (40 + 8 zeros)
+      const int r = it % kGradRows, c = 4 * (it / kGradRows);
+      f32x4_t x = f32x4_t{0.f, 0.f, 0.f, 0.f};
+      if (c < kMlpIn && row0 + r < r_end) x = *reinterpret_cast<const f32x4_t *>(&g.X[(size_t)(row0 + r) * kMlpIn + c]);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) TX[c + i][r] = f2bf(x[i]);
+    }
+    for (int it = threadIdx.x; it < kGradRows * 16; it += kGradBlock) {  // dY: 64 rows x 16 (2 + 14 zeros)
+      const int r = it % kGradRows, c = it / kGradRows;
+      float y = 0.f;
+      if (c < kMlpOut && row0 + r < r_end) y = g.dY[(size_t)(row0 + r) * kMlpOut + c];
+      TY[c][r] = f2bf(y);
+    }
+    __syncthreads();
+    {
+      int b[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) b[t] = 16 * t;
+      grad_mma<8>(TH[0], 16 * w, TZ[1], b, a2, lane);  // dW2 = H1^T dZ2, inputs 16w..
+      grad_mma<8>(TH[1], 16 * w, TZ[2], b, a3, lane);  // dW3 = H2^T dZ3
+      const int bx[1] = {0};
+      grad_mma<1>(TH[2], 16 * w, TY, bx, a4, lane);  // dW4 = H3^T dY
+    }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {  // dW1 = X^T dZ1, outputs 16w.. (input tiles t)
+      const int b1[1] = {16 * w};
+      f32x4_t one[1] = {a1[t]};
+      grad_mma<1>(TX, 16 * t, TZ[0], b1, one, lane);
+      a1[t] = one[0];
+    }
+    // bias sums in row order
+    if (f < 3 * kMlpH + kMlpOut) {
+      const uint16_t *z = f < 3 * kMlpH ? TZ[f / kMlpH][f % kMlpH] : TY[f - 3 * kMlpH];
+      for (int r = 0; r < kGradRows; ++r) bsum += bf2f(z[r]);
+    }
+  }
+  float *part = g.part + (size_t)blockIdx.x * kMlpNParams;
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    store_tile(part, kOW2, kMlpH, kMlpH, 16 * w, 16 * t, a2[t], lane);
+    store_tile(part, kOW3, kMlpH, kMlpH, 16 * w, 16 * t, a3[t], lane);
+  }
+#pragma unroll
+  for (int t = 0; t < 3; ++t) store_tile(part, kOW1, kMlpIn, kMlpH, 16 * t, 16 * w, a1[t], lane);
+  store_tile(part, kOW4, kMlpH, kMlpOut, 16 * w, 0, a4[0], lane);
+  if (f < 3 * kMlpH + kMlpOut) {
+    const int boff = f < kMlpH ? kOB1 + f : f < 2 * kMlpH ? kOB2 + f - kMlpH : f < 3 * kMlpH ? kOB3 + f - 2 * kMlpH
+                                                                                               : kOB4 + f - 3 * kMlpH;
+    part[boff] = bsum;
+  }
+}
+
+// grad[i] += sum over workgroups g = 0, 1, ... of part[g][i], in that order
+// (the partials are loaded eight at a time ahead of the dependent adds).
+__global__ void k_mlp_gsum(const float *part, int nparts, float *grad) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kMlpNParams) return;
   float s = 0.f;
-  for (int row = blockIdx.x; row < n; row += gridDim.x) s += bf2f(dZ[(size_t)row * ldz + j]);
-  atomicAdd(&db[j], s);
+  for (int g0 = 0; g0 < nparts; g0 += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = g0 + k < nparts ? part[(size_t)(g0 + k) * kMlpNParams + i] : 0.f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      if (g0 + k < nparts) s += v[k];
+  }
+  grad[i] += s;
 }
 
 __global__ void k_mlp_adam(float *p, float *g, float *m, float *v, int np, float lr, float b1, float b2, float eps,
@@ -276,27 +504,32 @@ __global__ void k_fill0(float *p, int n) {
 // ------------------------------------------------------------------ host
 namespace {
 struct PackShape {
-  int W_off, Kin, Nout, KS, NT, transposed;
+  int W_off, Kin, Nout, KS, MT, q_form;
 };
+// P(W1..W4) then Q(W1..W4), back to back (each pass's four packs contiguous,
+// as k_mlp_fwd / k_mlp_bwd copy them into LDS in one sweep)
 const PackShape kPacks[8] = {
-    {kOW1, kMlpIn, kMlpH, 3, 8, 0},  {kOW2, kMlpH, kMlpH, 8, 8, 0},  {kOW3, kMlpH, kMlpH, 8, 8, 0},
-    {kOW4, kMlpH, kMlpOut, 8, 1, 0}, {kOW1, kMlpIn, kMlpH, 8, 3, 1}, {kOW2, kMlpH, kMlpH, 8, 8, 1},
-    {kOW3, kMlpH, kMlpH, 8, 8, 1},   {kOW4, kMlpH, kMlpOut, 1, 8, 1}};
+    {kOW1, kMlpIn, kMlpH, 2, 8, 0},  {kOW2, kMlpH, kMlpH, 4, 8, 0},  {kOW3, kMlpH, kMlpH, 4, 8, 0},
+    {kOW4, kMlpH, kMlpOut, 4, 1, 0}, {kOW1, kMlpIn, kMlpH, 4, 3, 1}, {kOW2, kMlpH, kMlpH, 4, 8, 1},
+    {kOW3, kMlpH, kMlpH, 4, 8, 1},   {kOW4, kMlpH, kMlpOut, 1, 8, 1}};
 
 MlpPtrs ptrs(const MlpState &s) {
   MlpPtrs p;
-  const bf16x4_t *b = reinterpret_cast<const bf16x4_t *>(s.pack);
-  p.f1 = b + s.pack_off[0] / 4;
-  p.f2 = b + s.pack_off[1] / 4;
-  p.f3 = b + s.pack_off[2] / 4;
-  p.f4 = b + s.pack_off[3] / 4;
-  p.t1 = b + s.pack_off[4] / 4;
-  p.t2 = b + s.pack_off[5] / 4;
-  p.t3 = b + s.pack_off[6] / 4;
-  p.t4 = b + s.pack_off[7] / 4;
+  const bf16x8_t *b = reinterpret_cast<const bf16x8_t *>(s.pack);
+  const bf16x8_t **dst[8] = {&p.p1, &p.p2, &p.p3, &p.p4, &p.q1, &p.q2, &p.q3, &p.q4};
+  for (int i = 0; i < 8; ++i) *dst[i] = b + s.pack_off[i] / 8;
   p.params = s.params;
   return p;
 }
+
+// Workgroups of the persistent MLP kernels: one per CU, fewer for small batches.
+int mlp_blocks(const MlpState &s, int rows) {
+  const int tiles = (rows + kTileRows - 1) / kTileRows;
+  return std::max(1, std::min(s.ncu, (tiles + kMlpWaves - 1) / kMlpWaves));
+}
+
+// Row ranges of the gradient partials: one per CU, at least 64 rows each.
+int grad_parts(const MlpState &s, int rows) { return std::max(1, std::min(s.ncu, (rows + 255) / 256)); }
 
 int fail(MlpState &s, int code, const char *m) {
   s.err = m;
@@ -329,7 +562,8 @@ int mlp_reserve(MlpState &s, int rows) {
   if (!(s.X = (float *)a((size_t)cap * kMlpIn * 4)) || !(s.Y = (float *)a((size_t)cap * kMlpOut * 4)) ||
       !(s.dY = (float *)a((size_t)cap * kMlpOut * 4)) || !(s.dX = (float *)a((size_t)cap * kMlpIn * 4)) ||
       !(s.H = (uint16_t *)a(3 * plane * 2)) || !(s.D = (uint16_t *)a(3 * plane * 2)) ||
-      !(s.dZ = (uint16_t *)a(3 * plane * 2)) || !(s.dYb = (uint16_t *)a((size_t)cap * kMlpOutP * 2)))
+      !(s.dZ = (uint16_t *)a(3 * plane * 2)) ||
+      !(s.gpart = (float *)a((size_t)grad_parts(s, cap) * kMlpNParams * 4)))
     return fail(s, TM_ENOMEM, "hipMalloc MLP batch buffers");
   s.cap = cap;
   return TM_OK;
@@ -338,8 +572,8 @@ int mlp_reserve(MlpState &s, int rows) {
 int mlp_pack(MlpState &s, hipStream_t st) {
   for (int i = 0; i < 8; ++i) {
     const PackShape &sh = kPacks[i];
-    PackDesc d{s.params + sh.W_off, sh.Kin, sh.Nout, sh.KS, sh.NT, sh.transposed, s.pack + s.pack_off[i]};
-    const int total = sh.KS * sh.NT * 256;
+    PackDesc d{s.params + sh.W_off, sh.Kin, sh.Nout, sh.KS, sh.MT, sh.q_form, s.pack + s.pack_off[i]};
+    const int total = sh.KS * sh.MT * 512;
     hipLaunchKernelGGL(k_mlp_pack, dim3((total + 255) / 256), dim3(256), 0, st, d);
   }
   return hipGetLastError() == hipSuccess ? TM_OK : fail(s, TM_EDEVICE, "k_mlp_pack launch");
@@ -348,39 +582,22 @@ int mlp_pack(MlpState &s, hipStream_t st) {
 int mlp_forward(MlpState &s, const float *X, int n, float *Y, bool train, hipStream_t st) {
   if (n <= 0) return TM_OK;
   if (train && n > s.cap) return fail(s, TM_EINVAL, "batch exceeds the reserved MLP batch buffers");
-  const int rows_per_block = 16 * (kMlpBlock / 64);
-  hipLaunchKernelGGL(k_mlp_fwd, dim3((n + rows_per_block - 1) / rows_per_block), dim3(kMlpBlock), 0, st, ptrs(s),
-                     X, n, Y, train ? 1 : 0, s.H, s.D);
+  hipLaunchKernelGGL(k_mlp_fwd, dim3(mlp_blocks(s, n)), dim3(kMlpBlock), 0, st, ptrs(s), X, n, Y, train ? 1 : 0,
+                     s.H, s.D);
   return hipGetLastError() == hipSuccess ? TM_OK : fail(s, TM_EDEVICE, "k_mlp_fwd launch");
 }
 
 int mlp_backward(MlpState &s, const float *X, const float *dY, int n, float *dX, hipStream_t st) {
   if (n <= 0) return TM_OK;
   if (n > s.cap) return fail(s, TM_EINVAL, "batch exceeds the reserved MLP batch buffers");
-  const int rows_per_block = 16 * (kMlpBlock / 64);
-  hipLaunchKernelGGL(k_mlp_bwd, dim3((n + rows_per_block - 1) / rows_per_block), dim3(kMlpBlock), 0, st, ptrs(s),
-                     dY, n, (const uint16_t *)s.D, s.dZ, s.dYb, dX);
-  const size_t plane = (size_t)n * kMlpH;
-  const int splits = std::max(1, std::min(256, (n + 255) / 256));
-  const int rps = ((n + splits - 1) / splits + 15) / 16 * 16;
-  // dW4 = H3^T dY, dW3 = H2^T dZ3, dW2 = H1^T dZ2, dW1 = X^T dZ1
-  struct L {
-    const uint16_t *A;
-    const float *Xf;
-    int Kin;
-    const uint16_t *dZ;
-    int ldz, Nout, woff, boff;
-  } layers[4] = {{s.H + 2 * plane, nullptr, kMlpH, s.dYb, kMlpOutP, kMlpOut, kOW4, kOB4},
-                 {s.H + plane, nullptr, kMlpH, s.dZ + 2 * plane, kMlpH, kMlpH, kOW3, kOB3},
-                 {s.H, nullptr, kMlpH, s.dZ + plane, kMlpH, kMlpH, kOW2, kOB2},
-                 {nullptr, X, kMlpIn, s.dZ, kMlpH, kMlpH, kOW1, kOB1}};
-  for (const L &l : layers) {
-    const int tiles = ((l.Kin + 15) / 16) * ((l.Nout + 15) / 16);
-    hipLaunchKernelGGL(k_mlp_wgrad, dim3(tiles, splits), dim3(64), 0, st, l.A, l.Xf, kMlpH, l.Kin, l.dZ, l.ldz,
-                       l.Nout, n, rps, s.grad + l.woff);
-    hipLaunchKernelGGL(k_mlp_bgrad, dim3(std::min(n, 1024)), dim3(kMlpH), 0, st, l.dZ, l.ldz, l.Nout, n,
-                       s.grad + l.boff);
-  }
+  hipLaunchKernelGGL(k_mlp_bwd, dim3(mlp_blocks(s, n)), dim3(kMlpBlock), 0, st, ptrs(s), dY, n, (const uint16_t *)s.D,
+                     s.dZ, dX);
+  const int parts = grad_parts(s, n);
+  const int rps = ((n + parts - 1) / parts + kGradRows - 1) / kGradRows * kGradRows;
+  GradArgs g{X, s.H, s.dZ, dY, n, rps, s.gpart};
+  hipLaunchKernelGGL(k_mlp_grad, dim3((n + rps - 1) / rps), dim3(kGradBlock), 0, st, g);
+  hipLaunchKernelGGL(k_mlp_gsum, dim3((kMlpNParams + 255) / 256), dim3(256), 0, st, (const float *)s.gpart,
+                     (n + rps - 1) / rps, s.grad);
   return hipGetLastError() == hipSuccess ? TM_OK : fail(s, TM_EDEVICE, "MLP backward launch");
 }
 
@@ -425,12 +642,13 @@ extern "C" int tm_mlp_create(tm_mlp **out, int device, uint64_t seed) {
     tm_mlp_destroy(p);
     return rc;
   };
-  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess)
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess ||
+      hipDeviceGetAttribute(&s.ncu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || s.ncu < 1)
     return bail(TM_EDEVICE);
   size_t off = 0;
   for (int i = 0; i < 8; ++i) {
     s.pack_off[i] = off;
-    off += (size_t)kPacks[i].KS * kPacks[i].NT * 256;
+    off += (size_t)kPacks[i].KS * kPacks[i].MT * 512;
   }
   if (!(s.params = (float *)salloc(s, kMlpNParams * 4)) || !(s.grad = (float *)salloc(s, kMlpNParams * 4)) ||
       !(s.adam_m = (float *)salloc(s, kMlpNParams * 4)) || !(s.adam_v = (float *)salloc(s, kMlpNParams * 4)) ||

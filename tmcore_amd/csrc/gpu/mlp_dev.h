@@ -12,10 +12,8 @@
 namespace tmk {
 
 constexpr int kMlpIn = 40;
-constexpr int kMlpInP = 48;  // K of layer 1 padded to the MFMA k-step
 constexpr int kMlpH = 128;
 constexpr int kMlpOut = 2;
-constexpr int kMlpOutP = 16;
 // flat FP32 parameter vector: W1[40][128] b1 W2[128][128] b2 W3 b3 W4[128][2] b4
 constexpr int kOW1 = 0;
 constexpr int kOB1 = kOW1 + kMlpIn * kMlpH;
@@ -35,13 +33,14 @@ struct MlpState {
   float *grad = nullptr;          // [kMlpNParams]
   float *adam_m = nullptr, *adam_v = nullptr;
   int64_t adam_t = 0;
-  uint16_t *pack = nullptr;       // BF16 MFMA operand packs (forward and transposed)
-  size_t pack_off[8] = {};        // offsets (in uint16) of f1..f4, t1..t4
+  uint16_t *pack = nullptr;       // BF16 MFMA operand packs, forward P(W1..W4) then backward Q(W1..W4)
+  size_t pack_off[8] = {};        // offsets (in uint16) of the eight packs
+  int ncu = 1;                    // compute units of the device (workgroups of the persistent kernels)
   // per-batch buffers for up to cap rows
   int cap = 0;
   float *X = nullptr, *Y = nullptr, *dY = nullptr, *dX = nullptr;
   uint16_t *H = nullptr, *D = nullptr, *dZ = nullptr;  // [3][cap][128] each
-  uint16_t *dYb = nullptr;                             // [cap][16]
+  float *gpart = nullptr;  // [parts][kMlpNParams] per-row-range gradient partials
   // input normalisation and output scales, host copy and device copy
   // norm[kNormShift + j], norm[kNormScale + j], norm[kNormOut + k]
   float norm_h[2 * kMlpIn + kMlpOut] = {};

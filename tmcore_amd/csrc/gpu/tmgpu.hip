@@ -185,6 +185,7 @@ struct tm_ctx {
   float *cl_X = nullptr, *cl_Y = nullptr, *cl_r = nullptr, *cl_dY = nullptr;
   void *cl_sq = nullptr, *cl_vor = nullptr, *cl_div = nullptr;
   double *cl_loss = nullptr;
+  double *cl_lpart = nullptr;  // [nbE + nbC] per-block loss partials
   int cl_loss_cap = 0;
   struct ClSample {
     float *X, *tu, *tg;
@@ -1598,7 +1599,8 @@ int cl_reserve(tm_ctx *c) {
   if (c->cl_X) return TM_OK;
   if (!(c->cl_X = c->alloc<float>((size_t)c->nE * kMlpIn)) || !(c->cl_Y = c->alloc<float>((size_t)c->nE * kMlpOut)) ||
       !(c->cl_dY = c->alloc<float>((size_t)c->nE * kMlpOut)) || !(c->cl_r = c->alloc<float>(c->nC)) ||
-      !(c->cl_sq = c->alloc_r(c->nE)) || !(c->cl_vor = c->alloc_r(c->nV)) || !(c->cl_div = c->alloc_r(c->nC)))
+      !(c->cl_sq = c->alloc_r(c->nE)) || !(c->cl_vor = c->alloc_r(c->nV)) || !(c->cl_div = c->alloc_r(c->nC)) ||
+      !(c->cl_lpart = c->alloc<double>((size_t)c->dm.nbE + c->dm.nbC)))
     return c->fail(TM_ENOMEM, "hipMalloc closure buffers");
   return TM_OK;
 }
@@ -1643,6 +1645,7 @@ ClLossArgs cl_loss_args(tm_ctx *c, tm_mlp *mlp, const tm_ctx::ClSample &smp, boo
   a.tgs = smp.tgs;
   a.r = c->cl_r;
   a.dY = want_dy ? c->cl_dY : nullptr;
+  a.part = c->cl_lpart;
   a.loss = loss;
   return a;
 }
@@ -1650,6 +1653,7 @@ ClLossArgs cl_loss_args(tm_ctx *c, tm_mlp *mlp, const tm_ctx::ClSample &smp, boo
 int cl_enqueue_loss(tm_ctx *c, const ClLossArgs &a) {
   hipLaunchKernelGGL(k_cl_resid, dim3(c->dm.nbC), dim3(kBlock), 0, c->stream, c->dm, a);
   hipLaunchKernelGGL(k_cl_dy, dim3(c->dm.nbE), dim3(kBlock), 0, c->stream, c->dm, a);
+  hipLaunchKernelGGL(k_cl_loss_sum, dim3(1), dim3(64), 0, c->stream, c->dm, (const double *)a.part, a.loss);
   HIPCHK(c, hipGetLastError());
   return TM_OK;
 }
