@@ -1,6 +1,7 @@
 """Batched-ensemble benchmark (one persistent launch for every member).
 
     python -m tmcore_amd.ensemble_bench [--days 2]
+    python -m tmcore_amd.ensemble_bench --group 6:128:300 [--steps 20] [--tuning 64,1,4]
 
 128 members on level 6 (dt 300 s), 32 on level 7 (dt 150 s) and 8 on level 8
 (dt 90 s), FP32 state with FP64 invariants, member m seeded 1000 + m, advanced
@@ -19,11 +20,31 @@ from .ensemble import Ensemble
 from .mesh import Mesh, seeded_state
 
 SPEC = [(6, 128, 300.0), (7, 32, 150.0), (8, 8, 90.0)]
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def step_bytes(mesh, fit, members, ntracers=0, rs=4.0, fsz=4.0):
+    """Compulsory HBM bytes of one step of a level group: per stage, each mesh
+    constant once (shared by the members) plus 7 state values per prognostic
+    element and member (stage in, base, accumulator read and write, previous
+    and current tendency, next state); the finalize 3 per element and member.
+    The single-model model of tm_step_bytes with FP32 state and the FP32 copies
+    of the edge weights and lengths."""
+    nC, nE, nV = mesh.nCells, mesh.nEdges, mesh.nVertices
+    nf = fit.nFit2Vertices.astype(np.int64)
+    v1, v2 = mesh.verticesOnEdge[:, 0] - 1, mesh.verticesOnEdge[:, 1] - 1
+    sum_eoe, sum_eoc = int(mesh.nEdgesOnEdge.sum()), int(mesh.nEdgesOnCell.sum())
+    sum_fit_e, sum_fit_v = int((nf[v1] - 1).sum() + (nf[v2] - 1).sum()), int((nf - 1).sum())
+    cst = (nE * (8 + 8 + 2 * fsz) + sum_eoe * (4 + fsz) + sum_fit_e * fsz + nV * (12 + 12 + 24 + 8 + 8) +
+           sum_fit_v * 4 + sum_eoc * 4 + nC * (8 + 8))
+    prog = (nE + nC * (1 + ntracers)) * members
+    return 4 * (cst + 7 * rs * prog) + 3 * rs * prog
 
 
 def config4(days=2.0, spec=SPEC):
     meshes = [Mesh(L) for L, _, _ in spec]
-    ens = Ensemble([(m, m.fit(), M, dt) for m, (L, M, dt) in zip(meshes, spec)], precision=32)
+    fits = [m.fit() for m in meshes]
+    ens = Ensemble([(m, f, M, dt) for m, f, (L, M, dt) in zip(meshes, fits, spec)], precision=32)
     seed = 1000
     for li, (m, (L, M, dt)) in enumerate(zip(meshes, spec)):
         for k in range(M):
@@ -40,22 +61,67 @@ def config4(days=2.0, spec=SPEC):
            "autotune": {"chunk": chunk, "blocks_per_cu": bpc, "members_per_thread": width,
                         "ms_per_tick": round(tick_ms, 4)},
            "finite_invariants": bool(np.isfinite(inv).all())}
+    # roofline: compulsory bytes of the whole run over its time, and of each
+    # level group's step over that group's step latency alone
+    sb = [step_bytes(m, f, M) for m, f, (L, M, dt) in zip(meshes, fits, spec)]
+    run_bytes = sum(b * n for b, n in zip(sb, steps))
+    achieved = run_bytes / (ms * 1e-3) / 1e9
     # per-size step latency: each level group alone, 20 steps
-    lat = {}
+    lat, frac = {}, {}
     for li, (L, M, dt) in enumerate(spec):
         ns = [0] * len(spec)
         ns[li] = 20
-        lat[f"L{L}x{M}"] = round(ens.run(ns) / 20, 4)
+        t = ens.run(ns) / 20
+        lat[f"L{L}x{M}"] = round(t, 4)
+        frac[f"L{L}x{M}"] = round(sb[li] / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     out["step_latency_ms_alone"] = lat
+    out["roofline"] = {"bound": "hbm", "kernel": "k_ens_run", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_run": run_bytes,
+                       "model": "per step and level: mesh constants once per stage + 7 FP32 state values per "
+                                "prognostic element and member per stage + 3 in the finalize",
+                       "frac_alone": frac}
     ens.close()
     return out
+
+
+def group_steps(spec, steps, tuning=None):
+    """One level group alone (spec = (level, members, dt)) for `steps` steps
+    after two warm-up steps, at the given (chunk, blocks_per_cu,
+    members_per_thread) or the autotuner's choice: ms per step and the
+    compulsory-bytes roofline of the step."""
+    L, M, dt = spec
+    m = Mesh(L)
+    f = m.fit()
+    ens = Ensemble([(m, f, M, dt)], precision=32)
+    for k in range(M):
+        gd, u, ghs, _ = seeded_state(m, 1000 + k)
+        ens.upload(0, k, gd, u, ghs)
+    tuned = None
+    if tuning:
+        ens.set_tuning(*tuning)
+    else:
+        tuned, _ = ens.autotune(ticks=2)
+    ens.run([2])
+    ms = ens.run([steps]) / steps
+    ens.close()
+    b = step_bytes(m, f, M)
+    return {"group": f"L{L}x{M}", "ms_per_step": round(ms, 4), "bytes_per_step": b, "tuning": tuning or tuned,
+            "GBps": round(b / (ms * 1e-3) / 1e9, 1), "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--days", type=float, default=2.0)
+    ap.add_argument("--group", default=None, help="level:members:dt of one group run alone (e.g. 6:128:300)")
+    ap.add_argument("--steps", type=int, default=20, help="steps of --group")
+    ap.add_argument("--tuning", default=None, help="chunk,blocks_per_cu,members_per_thread (default: autotune)")
     args = ap.parse_args()
-    print(json.dumps(config4(args.days)), flush=True)
+    if args.group:
+        L, M, dt = args.group.split(":")
+        tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
+        print(json.dumps(group_steps((int(L), int(M), float(dt)), args.steps, tuning)), flush=True)
+    else:
+        print(json.dumps(config4(args.days)), flush=True)
 
 
 if __name__ == "__main__":
