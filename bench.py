@@ -30,6 +30,18 @@ def log(*a):
 
 
 def cpu_model():
+    """The host CPU as lscpu names it (model name, sockets, cores per socket),
+    else /proc/cpuinfo's model name."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        f = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        f = {k.strip(): v.strip() for k, v in f.items()}
+        if "Model name" in f:
+            return (f"{f['Model name']} ({f.get('Socket(s)', '?')} socket(s) x {f.get('Core(s) per socket', '?')} "
+                    f"cores x {f.get('Thread(s) per core', '?')} threads; {f.get('CPU(s)', '?')} CPUs)")
+    except (OSError, subprocess.SubprocessError, ValueError):
+        pass
     try:
         for line in open("/proc/cpuinfo"):
             if line.startswith("model name"):
@@ -102,6 +114,59 @@ def ensemble_bench(days=2.0):
     return config4(days)
 
 
+def precision_judge(mesh, fit, seed=2, dt=90.0, days=1, judge_level=9, judge_dt=37.5):
+    """FP64 and FP32 forms on the benchmark mesh against an FP64 judge on the
+    next finer mesh: both forms run `days` simulated days from the seeded
+    state; the judge runs the same initial recipe on level `judge_level` and
+    is coarse-grained to the benchmark mesh by area-weighted cell means.
+    Returns each form's relative L2 thickness error against the judge, its
+    sim-days/hour, and the FP32 speed-up at that (equal) error."""
+    import numpy as np
+    from tmcore_amd.coarse import CoarseGrainer
+    from tmcore_amd.closure_train import l2_thickness
+    from tmcore_amd.mesh import RADIUS, Mesh, seeded_state
+    from tmcore_amd.swm import SwmGpu
+    fm = Mesh(judge_level)
+    gd, u, ghs, _ = seeded_state(fm, seed)
+    g = SwmGpu(fm, fm.fit(), dt=judge_dt, precision=64)
+    g.upload(gd, ghs, u=u)
+    n = int(round(days * 86400.0 / judge_dt))
+    for done in range(0, n, 1152):
+        g.step_async(min(1152, n - done))
+        g.sync()
+    judge = CoarseGrainer(fm, mesh, velocity=False).cell_mean(g.download()["gd"])
+    g.close()
+    area = mesh.areaCell * RADIUS ** 2
+    gd, u, ghs, _ = seeded_state(mesh, seed)
+    out = {"judge": f"level {judge_level} ({fm.nCells} cells), FP64, dt {judge_dt:g} s, coarse-grained to level "
+                    f"{mesh.level} by area-weighted cell means", "days": days}
+    n = int(round(days * 86400.0 / dt))
+    for prec in (64, 32):
+        g = SwmGpu(mesh, fit, dt=dt, precision=prec)
+        g.upload(gd, ghs, u=u)
+        g.step_async(n)
+        g.sync()
+        ms = g.last_elapsed_ms()
+        out[f"fp{prec}"] = {"l2_h_rel_vs_judge": l2_thickness(g.download()["gd"], judge, area),
+                            "sim_days_per_hour": round(days / (ms / 3.6e6), 1)}
+        g.close()
+    out["fp32_speedup_at_equal_error"] = round(out["fp32"]["sim_days_per_hour"] / out["fp64"]["sim_days_per_hour"], 3)
+    return out
+
+
+def equal_error_cost(err, curve):
+    """Sim-days/hour the plain method needs for error `err`, interpolated
+    log-log between the two points of curve [(sim_days_per_hour, error), ...]
+    (plain runs at increasing resolution) that bracket it; None outside."""
+    import math
+    pts = sorted(curve, key=lambda p: -p[1])  # decreasing error
+    for (s0, e0), (s1, e1) in zip(pts, pts[1:]):
+        if e1 <= err <= e0 and e0 > e1:
+            w = (math.log(e0) - math.log(err)) / (math.log(e0) - math.log(e1))
+            return math.exp(math.log(s0) + w * (math.log(s1) - math.log(s0)))
+    return None
+
+
 CLOSURE_WEIGHTS = os.path.join(ROOT, "tmcore_amd", "data", "closure_L6.npz")
 
 
@@ -137,7 +202,17 @@ def closure_bench(days=1, seed=6000):
     finally:
         ev.close()
         p.close()
-    return {"speed": speed, f"l2_h_rel_day{days}_seed{seed}": {k: v[-1] for k, v in errs.items()},
+    final = {k: v[-1] for k, v in errs.items()}
+    # the closure's error placed on the plain method's error-versus-cost curve
+    curve = [(speed[f"L{L}_plain"]["sim_days_per_hour"], final[f"L{L}_plain"]) for L in (6, 7, 8)]
+    e_c, s_c = final["L6_closure"], speed["L6_closure"]["sim_days_per_hour"]
+    s_p = equal_error_cost(e_c, curve)
+    eq = {"error": e_c, "closure_sim_days_per_hour": s_c,
+          "plain_sim_days_per_hour_at_equal_error": round(s_p, 1) if s_p else None,
+          "closure_speedup_at_equal_error": round(s_c / s_p, 3) if s_p else None,
+          "plain_curve": [{"level": L, "sim_days_per_hour": c[0], "error": c[1]} for L, c in zip((6, 7, 8), curve)],
+          "interpolation": "log(sim-days/hour) linear in log(error) between the plain runs that bracket it"}
+    return {"speed": speed, f"l2_h_rel_day{days}_seed{seed}": final, "equal_error": eq,
             "trained_on": ev.meta.get("seeds_all", ev.meta["seeds"]), "deployment_gain": ev.meta.get("gain", 1.0),
             "weights": os.path.relpath(CLOSURE_WEIGHTS, ROOT)}
 
@@ -156,6 +231,7 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-days", type=float, default=2.0)
     ap.add_argument("--no-closure", action="store_true")
+    ap.add_argument("--no-judge", action="store_true", help="skip the FP64 level-9 judge of the FP64/FP32 forms")
     ap.add_argument("--closure-days", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -223,6 +299,10 @@ def main():
                          "l2_height_rel_err_vs_fp64": float(norm(h32 - h64) / norm(h64)),
                          "after_steps": args.warmup + args.steps}
         log(f"FP32: {forms['fp32']}")
+
+    if not args.no_judge and not args.no_fp32:
+        forms["equal_error_vs_fp64_judge"] = precision_judge(mesh, fit, dt=args.dt)
+        log(f"judge: {forms['equal_error_vs_fp64_judge']}")
 
     if not args.no_ensemble:
         forms["ensemble_fp32"] = ensemble_bench(args.ensemble_days)

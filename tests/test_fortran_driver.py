@@ -189,3 +189,53 @@ def test_fortran_driver_writes_history(tmp_path):
         for k, v in (("u", "u"), ("zonal", "zonalWind"), ("meridional", "merdionalWind"), ("h", "h"),
                      ("pv", "pv"), ("vor", "vor"), ("div", "div")):
             np.testing.assert_array_equal(nc.variables[v][2], f[k], err_msg=v)
+
+
+@pytest.mark.gpu
+def test_fortran_operator_wrappers_match_oracle(tmp_path):
+    """The operators of operators_mod.F90 called from Fortran under their
+    reference names and dummy arguments (tm_swm_gpu_mod: div_operator ...
+    calc_u_tend_on_edge, through iso_c_binding) on random arrays, against the
+    oracle's operators; adv_order 3 so that add_upwind_correction_on_cell
+    acts."""
+    exe = os.path.join(_ffi.LIB_DIR, "test_tmcore_ops")
+    if not os.path.exists(exe):
+        pytest.skip("Fortran operator test not built (amdflang absent when building)")
+    level = 4
+    nml = tmp_path / "namelist.ops"
+    nml.write_text(NAMELIST.format(level=level, dt=600, hours=1, extra="", tmcore="adv_order = 3\nadv_mono = .false.\n",
+                                   io=""))
+    m = Mesh(level)
+    nC, nE, nV = m.nCells, m.nEdges, m.nVertices
+    rng = np.random.default_rng(21)
+    fc, fe, fv, ue, ve = (rng.standard_normal(n) for n in (nC, nE, nV, nE, nE))
+    gde = 1.0 + rng.random(nE)
+    tgv, pvv, pvc = rng.standard_normal(nV), rng.standard_normal(nV), rng.standard_normal(nC)
+    kec, gdc, ghs = rng.standard_normal(nC), 1.0 + rng.random(nC), rng.standard_normal(nC)
+    pvf, tge = rng.standard_normal(nE), rng.standard_normal(nE)
+    fin, fout = tmp_path / "ops_in.bin", tmp_path / "ops_out.bin"
+    np.concatenate([fc, fe, fv, ue, ve, gde, tgv, pvv, pvc, kec, gdc, ghs, pvf, tge]).astype("=f8").tofile(fin)
+    r = subprocess.run([exe, str(nml), str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = np.fromfile(fout, dtype="=f8")
+    sizes = [nC, nV, nE, nV, nC, nE, nC, nE, nE, nE, nV, nE, nE, nE]
+    assert out.size == sum(sizes)
+    got = np.split(out, np.cumsum(sizes)[:-1])
+    o = Oracle(m, m.fit(), dt=600.0, adv_order=3, adv_mono=False)
+    from oracle.oracle import _p, lib
+    upw = fe.copy()
+    lib().om_upwind_cell(o.h, _p(fc), _p(ue), _p(upw))
+    ut, it = np.zeros(nE), np.zeros(nE)
+    lib().om_u_tend(o.h, *[_p(a) for a in (ue, kec, gdc, gde, ghs, pvf, tge)], _p(ut), _p(it))
+    ref = [o.op("div", fe, out_where="c"), o.op("curl", fe, out_where="v"), o.op("c2e", fc, out_where="e"),
+           o.op("c2v", fc, out_where="v"), o.op("v2c", fv, out_where="c"), o.op("v2e", fv, out_where="e"),
+           o.op("ke", ue, out_where="c"), o.op("tangent_wind", ue, out_where="e"),
+           o.op("tangent_vor_flux", ue, gde, fe, out_where="e"), upw,
+           o.op("pv_vertex", fv, np.abs(pvv) + 1, out_where="v"),
+           o.op("pv_edge", ue, ve, gde, tgv, pvv, pvc, out_where="e"), ut, it]
+    names = ["div", "curl", "c2e", "c2v", "v2c", "v2e", "ke", "tangent_wind", "tangent_vor_flux", "upwind",
+             "pv_vertex", "pv_edge", "u_tend", "iap_u_tend"]
+    assert np.abs(upw - fe).max() > 0  # the correction acts at adv_order 3
+    for name, g, rf in zip(names, got, ref):
+        err = np.linalg.norm(g - rf) / np.linalg.norm(rf)
+        assert err <= 1e-13, (name, err)

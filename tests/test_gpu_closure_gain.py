@@ -2,9 +2,10 @@
 method on held-out initial states: level-6 runs (dt 300 s) from the level-9
 truth coarse-grained to level 6, relative L2 thickness error against the
 coarse-grained truth.  At the full correction (gain 1) the network's
-short-range skill shows at 24 h; at the deployment gain stored with the
-weights the corrected run is closer to the truth than the plain run at every
-day of a 3-day forecast.  Mass is kept to round-off."""
+short-range skill shows at 24 h (seed 6000); at the deployment gain stored
+with the weights the corrected run is closer to the truth than the plain run
+at every day of a 3-day forecast, for held-out seeds 6000 and 6001.  Mass is
+kept to round-off."""
 import os
 
 import numpy as np
@@ -21,8 +22,9 @@ WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__)
 
 
 @pytest.mark.skipif(not os.path.exists(WEIGHTS), reason="no trained closure weights")
-def test_trained_closure_beats_plain_on_held_out_state():
-    seed, days = 6000, 3
+@pytest.mark.parametrize("seed", [6000, 6001])
+def test_trained_closure_beats_plain_on_held_out_state(seed):
+    days = 3
     mlp = Mlp(seed=0)
     meta = load(WEIGHTS, mlp)
     H = int(meta["horizon"])
@@ -30,7 +32,8 @@ def test_trained_closure_beats_plain_on_held_out_state():
     truth, ghs_c, _ = p.truth_daily(seed, days)
     m0 = np.sum(p.area * truth[0][0])
     errs = {}
-    for name, gain in (("plain", None), ("full", 1.0), ("deployed", float(meta.get("gain", 1.0)))):
+    runs = (("plain", None), ("full", 1.0), ("deployed", float(meta.get("gain", 1.0))))
+    for name, gain in runs if seed == 6000 else (runs[0], runs[2]):
         g = SwmGpu(p.cm, p.cm.fit(), dt=300.0)
         if gain is not None:
             clo = Closure(g, mlp)
@@ -45,9 +48,11 @@ def test_trained_closure_beats_plain_on_held_out_state():
             e.append(l2_thickness(st["gd"], truth[d + 1][0], p.area))
             assert abs(np.sum(p.area * st["gd"]) / m0 - 1) <= 1e-12
         errs[name] = e
-        print(f"{name} (gain {gain}): daily relative L2 thickness error " + " ".join(f"{x:.4e}" for x in e),
+        print(f"seed {seed} {name} (gain {gain}): daily relative L2 thickness error " + " ".join(f"{x:.4e}" for x in e),
               flush=True)
         g.close()
-    assert errs["full"][0] < 0.97 * errs["plain"][0], errs
+    if "full" in errs:
+        assert errs["full"][0] < 0.97 * errs["plain"][0], errs
     assert all(c < q for c, q in zip(errs["deployed"], errs["plain"])), errs
+    p.close()
     mlp.close()

@@ -63,9 +63,12 @@ def build_fortran():
         return
     bdir = os.path.join(fdir, "build")
     os.makedirs(bdir, exist_ok=True)
+    link = ["-L" + LIB, "-ltmgpu", "-ltmhost", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
     _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs], "-o",
-          os.path.join(LIB, "test_tmcore_sw"), "-L" + LIB, "-ltmgpu", "-ltmhost",
-          "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"], cwd=bdir)
+          os.path.join(LIB, "test_tmcore_sw"), *link], cwd=bdir)
+    # the operator wrappers under their reference names, for the parity test
+    _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs[:-1]],
+          os.path.join(fdir, "test_tmcore_ops.F90"), "-o", os.path.join(LIB, "test_tmcore_ops"), *link], cwd=bdir)
 
 
 def build_all():
