@@ -62,17 +62,20 @@ int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_desc *f, con
 int tm_ctx_destroy(tm_ctx *ctx);
 
 /* Kernel layout of the stage (no counterpart in the reference; results are
- * bit-identical in every mode): bit 1, the PV kernel, and bit 2, the
- * tendency kernel, derive each edge's edgesOnEdge slots from the edge rows
+ * bit-identical in every mode).  Bit 1 (the PV kernel) and bit 2 (the
+ * tendency kernel) derive each edge's edgesOnEdge slots from the edge rows
  * of its two cells instead of streaming the edgesOnEdge table.  That needs
  * the MPAS order of edgesOnEdge (the edges of cellsOnEdge(1) after the edge
  * counter-clockwise, then those of cellsOnEdge(2)), which tm_ctx_create
- * checks for every edge, and adv_order 2.  tm_ctx_create picks the faster
- * mode measured for the context's precision when the order holds, else 0.
- * Setting a mode the mesh or scheme does not allow returns TM_EINVAL.  For
- * tests and timing comparisons. */
-int tm_ctx_set_cell_rows(tm_ctx *ctx, int mode);
-int tm_ctx_get_cell_rows(tm_ctx *ctx, int *mode);
+ * checks for every edge.  Bit 4: the tendency kernel of RK stages 1-3 also
+ * forms the next stage's edge values (gd_edge, u and their products), so
+ * stages 2-4 run no edge-prep pass; needs time_scheme 2 (CRK4) with
+ * pv_scheme 3 and interp_pv_order <= 4.  Every bit needs adv_order 2.
+ * tm_ctx_create picks the fastest mode measured for the context's precision
+ * that the mesh and options allow.  Setting a mode they do not allow
+ * returns TM_EINVAL.  For tests and timing comparisons. */
+int tm_ctx_set_layout(tm_ctx *ctx, int mode);
+int tm_ctx_get_layout(tm_ctx *ctx, int *mode);
 const char *tm_last_error(const tm_ctx *ctx);
 
 /* Prognostic upload (reference state(old), tmcore_swm_mod.F90:62-64):

@@ -429,26 +429,34 @@ def test_nonfinite_upload_is_rejected():
 
 @pytest.mark.parametrize("precision,ntracers,kw", [(64, 2, {}), (32, 1, {}), (64, 0, {"pv_scheme": 2}),
                                                    (64, 0, {"time_scheme": 1, "pv_scheme": 1})])
-def test_cell_row_kernels_are_bit_identical(precision, ntracers, kw):
+def test_kernel_layouts_are_bit_identical(precision, ntracers, kw):
     """edgesOnEdge derived from the cells' edge rows (in the PV kernel, the
-    tendency kernel, both) gives the slots of the table in the same order:
-    the stage diagnostics, ten steps, beta and the invariants are
-    bit-identical in all four modes."""
+    tendency kernel, both) gives the slots of the table in the same order,
+    and the next stage's edge values formed in the tendency kernel are the
+    edge-prep pass's: the stage diagnostics, ten steps, beta and the
+    invariants are bit-identical in every mode the scheme allows.  The
+    schemes outside plain CRK4 refuse bit 4."""
     m = mesh(4)
     gd, u, ghs, q = seeded_state(m, 5, [201, 202][:ntracers])
+    crk4 = not kw
+    modes = range(8) if crk4 else range(4)
     runs = {}
-    for mode in (0, 1, 2, 3):
+    for mode in modes:
         g = SwmGpu(m, m.fit(), dt=600.0, ntracers=ntracers, precision=precision, **kw)
-        assert g.cell_rows != 0  # the generated mesh keeps the MPAS order
-        g.cell_rows = mode
-        assert g.cell_rows == mode
+        assert g.layout & 3  # the generated mesh keeps the MPAS order
+        assert bool(g.layout & 4) == crk4
+        if not crk4:
+            with pytest.raises(Exception, match="CRK4"):
+                g.layout = mode | 4
+        g.layout = mode
+        assert g.layout == mode
         g.upload(gd, ghs, u=u, tracers=q if ntracers else None)
         st = g.stage()
         inv, beta = g.step(10)
         runs[mode] = (st, g.download(), inv, beta, g.invariants())
         g.close()
     st0, s0, inv0, b0, i0 = runs[0]
-    for mode in (1, 2, 3):
+    for mode in modes[1:]:
         st, s, inv, b, i = runs[mode]
         for k, v in st0.items():
             np.testing.assert_array_equal(st[k], v, err_msg=f"mode {mode} stage {k}")
@@ -457,6 +465,17 @@ def test_cell_row_kernels_are_bit_identical(precision, ntracers, kw):
         np.testing.assert_array_equal(b, b0)
         np.testing.assert_array_equal(inv, inv0)
         np.testing.assert_array_equal(i, i0)
+
+
+def test_upwind_correction_refuses_the_layout_modes():
+    """adv_order 3 keeps the table kernels and the edge-prep pass."""
+    m = mesh(3)
+    g = SwmGpu(m, m.fit(max_fit=31), dt=600.0, ntracers=1, adv_order=3)
+    assert g.layout == 0
+    for mode in (1, 2, 4):
+        with pytest.raises(Exception, match="adv_order"):
+            g.layout = mode
+    g.close()
 
 
 def test_mesh_without_mpas_edge_order_keeps_the_table_kernels():
@@ -471,9 +490,9 @@ def test_mesh_without_mpas_edge_order_keeps_the_table_kernels():
     m.weightsOnEdge[e, :n] = np.roll(m.weightsOnEdge[e, :n], 1)
     gd, u, ghs, q = seeded_state(m, 2, [201])
     g = SwmGpu(m, m.fit(), dt=600.0, ntracers=1)
-    assert g.cell_rows == 0
-    with pytest.raises(Exception):
-        g.cell_rows = 3
+    assert g.layout == 4  # the edge values from the tendency kernel need no MPAS order
+    with pytest.raises(Exception, match="MPAS"):
+        g.layout = 3
     o = Oracle(m, m.fit(), dt=600.0)
     iap = o.initial_iap(gd, u)
     g.upload(gd, ghs, u=u, tracers=q)

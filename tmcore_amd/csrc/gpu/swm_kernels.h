@@ -71,6 +71,8 @@ struct StageBufs {
   const R *tprev_gd, *tprev_iap;         // tendency of stage k-1
   R *tcur_gd, *tcur_iap;                 // tendency of stage k
   R *gd_e, *u, *udc, *E, *tend_gd, *pv_v, *pv_e;
+  R *gd_e_n, *u_n;  // kXTendPrep: the next stage's gd_edge and u
+  int prep_next;    // kXTendPrep: this stage's k_edge_tend forms the next stage's edge values
   // [nE][2]: u*gd_e*dvEdge (mass flux) and u^2*areaEdge, interleaved because
   // the cell sums gather both at the same edge (one load per slot)
   R *fk;
@@ -362,6 +364,12 @@ constexpr int kXAdv = 1, kXExt = 2, kXAdv56 = 4;
 // instead of streaming the edgesOnEdge table.  Same slots in the same order:
 // results are bit-identical.
 constexpr int kXEoePv = 8, kXEoeTend = 16;
+// kXTendPrep (CRK4 without the upwind correction): k_edge_tend of stages 1-3
+// also forms the next stage's edge values from iap_next and the cells' next
+// state, as k_edge_prep would (same expressions on the same values), so
+// stages 2-4 have no edge-prep pass.  gd_edge and u ping-pong between two
+// buffers (the tendency reads its neighbours' current values).
+constexpr int kXTendPrep = 32;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (f at offset base; padded slots add an exact 0).
@@ -836,7 +844,22 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
       (void)rk_update_ext<R, L>(b, at, e, it, b.iap_base, b.iap_next, b.acc_iap, b.tprev_iap, b.tfirst_iap,
                                 b.tcur_iap, aE, r.rp, r.rf, r.r44);
     } else {
-      if (b.k < 4) at.st(b.iap_next, e, at.ld_s(b.iap_base, e) + (R)b.cdt * it);
+      if (b.k < 4) {
+        const V xn = at.ld_s(b.iap_base, e) + (R)b.cdt * it;
+        at.st(b.iap_next, e, xn);
+        if constexpr ((X & kXTendPrep) != 0) {
+          if (b.prep_next) {
+            // edge_prep_e of the next stage: its gd is gd_next (written by
+            // this stage's k_cell_vertex), its iap_u is xn
+            const V gde = R(0.5) * (at.ld(b.gd_next, c1) + at.ld(b.gd_next, c2));
+            const V un = xn / tm_sqrt<R, L::W>(gde);
+            at.st(b.gd_e_n, e, gde);
+            at.st(b.u_n, e, un);
+            at.st2(b.fk, e, (un * gde) * geo_at<R>(m.dv, m.dv32, e), (un * un) * geo_at<R>(m.aE, m.aE32, e));
+            at.st(b.udc, e, un * geo_at<R>(m.dc, m.dc32, e));
+          }
+        }
+      }
       const V acc = rk_accumulate<R, L>(b, b.acc_iap, at, e, it);
       if (b.k >= 2) r.rp = (to_d<R, L::W>(at.ld_s(b.tprev_iap, e)) * to_d<R, L::W>(it)) * aE;
       if (b.k == 4) r.r44 = (to_d<R, L::W>(acc) * to_d<R, L::W>(acc)) * aE;

@@ -151,6 +151,7 @@ struct tm_ctx {
   double *cv_eps = nullptr;
   void *t_gd[2] = {nullptr, nullptr}, *t_iap[2] = {nullptr, nullptr};
   void *gd_e = nullptr, *u = nullptr, *fk = nullptr, *udc = nullptr;  // fk: [nE][2] (StageBufs::fk)
+  void *gd_e2 = nullptr, *u2 = nullptr;  // second gd_edge/u set (kXTendPrep, stages 2 and 4)
   void *E = nullptr, *tend_gd = nullptr, *pv_v = nullptr, *pv_e = nullptr;
   void *scratch_e[4] = {}, *scratch_c[2] = {}, *scratch_v[2] = {};
   double2 *part = nullptr;
@@ -169,7 +170,7 @@ struct tm_ctx {
   bool have_state = false;
   bool timed = false;
   int serpentine = 1;  // alternate the tile sweep direction between kernels
-  int xmask = 0;       // compile-time variant of the stage kernels (kXAdv, kXAdv56, kXExt)
+  int xmask = 0;       // compile-time variant of the stage kernels (kX* bits of swm_kernels.h)
   int nstages = 4;     // stages per step: 4 (CRK4), 3 (CRK3, PC2)
   // learned closure: attached MLP, its per-step buffers and the training set
   tm_mlp *cl = nullptr;
@@ -298,8 +299,15 @@ StageBufs<R> stage_bufs(tm_ctx *c, int k, bool want_inv) {
   b.tprev_iap = P(c->t_iap[(k + 1) & 1]);
   b.tcur_gd = P(c->t_gd[k & 1]);
   b.tcur_iap = P(c->t_iap[k & 1]);
-  b.gd_e = P(c->gd_e);
-  b.u = P(c->u);
+  // gd_edge and u: set 0 for stages 0, 1 and 3, set 1 for stages 2 and 4 when
+  // k_edge_tend forms the next stage's edge values (kXTendPrep); else set 0
+  const bool tp = (c->xmask & kXTendPrep) != 0;
+  const int es = tp && (k == 2 || k == 4) ? 1 : 0;
+  b.gd_e = P(es ? c->gd_e2 : c->gd_e);
+  b.u = P(es ? c->u2 : c->u);
+  b.gd_e_n = P(es ? c->gd_e : c->gd_e2);
+  b.u_n = P(es ? c->u : c->u2);
+  b.prep_next = tp && k >= 1 && k <= 3 ? 1 : 0;
   b.fk = P(c->fk);
   b.udc = P(c->udc);
   b.E = P(c->E);
@@ -395,6 +403,10 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
     case kXAdv | kXExt: launch_part_x<R, kXAdv | kXExt>(c, which, b); break;
     case kXAdv | kXAdv56: launch_part_x<R, kXAdv | kXAdv56>(c, which, b); break;
     case kXEoePv: launch_part_x<R, kXEoePv>(c, which, b); break;
+    case kXTendPrep: launch_part_x<R, kXTendPrep>(c, which, b); break;
+    case kXTendPrep | kXEoePv: launch_part_x<R, kXTendPrep | kXEoePv>(c, which, b); break;
+    case kXTendPrep | kXEoeTend: launch_part_x<R, kXTendPrep | kXEoeTend>(c, which, b); break;
+    case kXTendPrep | kXEoePv | kXEoeTend: launch_part_x<R, kXTendPrep | kXEoePv | kXEoeTend>(c, which, b); break;
     case kXEoeTend: launch_part_x<R, kXEoeTend>(c, which, b); break;
     case kXEoePv | kXEoeTend: launch_part_x<R, kXEoePv | kXEoeTend>(c, which, b); break;
     case kXExt | kXEoePv: launch_part_x<R, kXExt | kXEoePv>(c, which, b); break;
@@ -409,7 +421,9 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
 template <typename R>
 int launch_stage(tm_ctx *c, const StageBufs<R> &b0, bool tail = true) {
   StageBufs<R> b = b0;
-  for (int which = 0; which < (tail ? 4 : 2); ++which) {
+  // with kXTendPrep the previous stage's k_edge_tend formed this stage's edge values
+  const int first = (c->xmask & kXTendPrep) && b.k >= 2 ? 1 : 0;
+  for (int which = first; which < (tail ? 4 : 2); ++which) {
     b.rev = (which & 1) ? c->serpentine : 0;
     launch_part<R>(c, which, b);
   }
@@ -1079,6 +1093,7 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   for (void **p : rbufs_e)
     if (!(*p = c->alloc_r(nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
   if (!(c->fk = c->alloc_r(2 * (size_t)nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
+  if (!(c->gd_e2 = c->alloc_r(nE)) || !(c->u2 = c->alloc_r(nE))) return c->fail(TM_ENOMEM, "hipMalloc edge state");
   void **rbufs_v[] = {&c->pv_v, &c->scratch_v[0], &c->scratch_v[1]};
   for (void **p : rbufs_v)
     if (!(*p = c->alloc_r(nV))) return c->fail(TM_ENOMEM, "hipMalloc vertex state");
@@ -1092,6 +1107,8 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   // edgesOnEdge derived from the cells' edge rows where the mesh keeps the
   // MPAS order (not in the upwind-correction variants)
   if (c->md.cell_rows && o.adv_order < 3) c->xmask |= cell_rows_default(c->rsz);
+  // next-stage edge values from the tendency kernel (plain CRK4, no upwind correction)
+  if ((c->xmask & (kXAdv | kXExt)) == 0) c->xmask |= kXTendPrep;
   c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
   if (o.pv_scheme == 2 &&
@@ -1123,9 +1140,10 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   return TM_OK;
 }
 
-// Compulsory bytes of one launch of each kernel (stage 2/3 pattern) in the
-// context's current kernel layout: each array the kernel touches counted
-// once, gathers at the size of the gathered array.
+// Compulsory bytes of one launch of each kernel (stage 2/3 pattern; the
+// tendency kernel's averaged over the step) in the context's current kernel
+// layout: each array the kernel touches counted once, gathers at the size of
+// the gathered array.
 static void kernel_bytes(const tm_ctx *c, double *kb) {
   const double nC = c->nC, nE = c->nE, nV = c->nV, T = c->nT;
   const double sumEoE = c->md.sumEoE, sumFitE = c->md.sumFitE, sumFitV = c->md.sumFitV, sumEoC = c->md.sumEoC;
@@ -1143,6 +1161,9 @@ static void kernel_bytes(const tm_ctx *c, double *kb) {
   kb[TM_K_EDGE_PV] = nE * (1.0 + 8.0 + fsz + 2.0 * rs) + eoe_pv + fsz * sumEoE + fsz * sumFitE + nV * (rs + 1.0) +
                      4.0 * sumFitV;
   kb[TM_K_EDGE_TEND] = nE * (1.0 + 8.0 + fsz + 9.0 * rs) + eoe_tend + fsz * sumEoE + nC * 2.0 * rs;
+  // stages 1-3 of 4 also gather gd_next, read dvEdge, and write gd_edge, u,
+  // fk and u dcEdge: averaged over the step's four launches, as the timings are
+  if (c->xmask & kXTendPrep) kb[TM_K_EDGE_TEND] += 0.75 * (nC * rs + nE * (fsz + 5.0 * rs));
   kb[TM_K_REDUCE] = (double)kNumSums * c->part_stride * 16.0;
   kb[TM_K_FINALIZE] = 3.0 * rs * prog;
 }
@@ -1162,21 +1183,24 @@ extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_d
   return TM_OK;
 }
 
-extern "C" int tm_ctx_set_cell_rows(tm_ctx *c, int mode) {
+extern "C" int tm_ctx_set_layout(tm_ctx *c, int mode) {
   if (!c) return TM_EINVAL;
-  if (mode < 0 || mode > 3) return c->fail(TM_EINVAL, "cell-row mode must be 0..3");
-  const int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0);
-  if (want && !c->md.cell_rows) return c->fail(TM_EINVAL, "the mesh's edgesOnEdge rows do not follow the MPAS order");
-  if (want && c->o.adv_order >= 3) return c->fail(TM_EINVAL, "the cell-row kernels do not run with adv_order >= 3");
+  if (mode < 0 || mode > 7) return c->fail(TM_EINVAL, "layout mode must be 0..7");
+  const int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0) | ((mode & 4) ? kXTendPrep : 0);
+  if ((want & (kXEoePv | kXEoeTend)) && !c->md.cell_rows)
+    return c->fail(TM_EINVAL, "the mesh's edgesOnEdge rows do not follow the MPAS order");
+  if (want && (c->xmask & kXAdv)) return c->fail(TM_EINVAL, "the layout modes do not run with adv_order >= 3");
+  if ((want & kXTendPrep) && (c->xmask & kXExt))
+    return c->fail(TM_EINVAL, "edge values from the tendency kernel need CRK4 with pv_scheme 3 order <= 4");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);  // captured steps hold the previous variant
-  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend)) | want;
+  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend | kXTendPrep)) | want;
   return TM_OK;
 }
 
-extern "C" int tm_ctx_get_cell_rows(tm_ctx *c, int *mode) {
+extern "C" int tm_ctx_get_layout(tm_ctx *c, int *mode) {
   if (!c || !mode) return TM_EINVAL;
-  *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0);
+  *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0) | ((c->xmask & kXTendPrep) ? 4 : 0);
   return TM_OK;
 }
 
@@ -1319,7 +1343,7 @@ static int profile_one_step(tm_ctx *c, std::vector<hipEvent_t> &ev, double *ms, 
   for (int k = 1; k <= c->nstages; ++k) {
     StageBufs<R> b = stage_bufs<R>(c, k, k == 1);
     const int ids[4] = {TM_K_EDGE_PREP, TM_K_CELL_VERTEX, TM_K_EDGE_PV, TM_K_EDGE_TEND};
-    for (int which = 0; which < 4; ++which) {
+    for (int which = (c->xmask & kXTendPrep) && k >= 2 ? 1 : 0; which < 4; ++which) {
       if ((rc = mark(ids[which]))) return rc;
       b.rev = (which & 1) ? c->serpentine : 0;
       launch_part<R>(c, which, b);

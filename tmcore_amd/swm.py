@@ -57,10 +57,10 @@ def gpu_lib() -> C.CDLL:
         L.tm_step_async.argtypes = [_vp, C.c_int]
         L.tm_sync.argtypes = [_vp]
         L.tm_invariants.argtypes = [_vp, _dp]
-        L.tm_ctx_set_cell_rows.argtypes = [_vp, C.c_int]
-        L.tm_ctx_get_cell_rows.argtypes = [_vp, C.POINTER(C.c_int)]
-        L.tm_ctx_set_cell_rows.restype = C.c_int
-        L.tm_ctx_get_cell_rows.restype = C.c_int
+        L.tm_ctx_set_layout.argtypes = [_vp, C.c_int]
+        L.tm_ctx_get_layout.argtypes = [_vp, C.POINTER(C.c_int)]
+        L.tm_ctx_set_layout.restype = C.c_int
+        L.tm_ctx_get_layout.restype = C.c_int
         L.tm_invariants_noalloc.argtypes = [_vp, _dp]
         L.tm_invariants_noalloc.restype = C.c_int
         L.tm_stage.argtypes = [_vp, C.POINTER(StageOut)]
@@ -211,16 +211,17 @@ class SwmGpu:
         return out
 
     @property
-    def cell_rows(self) -> int:
-        """Kernel layout (tm_ctx_set_cell_rows): bit 1 edge values formed in
-        the cell/vertex pass, bit 2 edgesOnEdge derived from the cells' rows."""
+    def layout(self) -> int:
+        """Kernel layout (tm_ctx_set_layout): bit 1 and bit 2, edgesOnEdge
+        derived from the cells' edge rows in the PV and tendency kernels;
+        bit 4, the tendency kernel forms the next RK stage's edge values."""
         m = C.c_int()
-        self._check(gpu_lib().tm_ctx_get_cell_rows(self.h, C.byref(m)), "tm_ctx_get_cell_rows")
+        self._check(gpu_lib().tm_ctx_get_layout(self.h, C.byref(m)), "tm_ctx_get_layout")
         return m.value
 
-    @cell_rows.setter
-    def cell_rows(self, mode: int):
-        self._check(gpu_lib().tm_ctx_set_cell_rows(self.h, int(mode)), "tm_ctx_set_cell_rows")
+    @layout.setter
+    def layout(self, mode: int):
+        self._check(gpu_lib().tm_ctx_set_layout(self.h, int(mode)), "tm_ctx_set_layout")
 
     def invariants_noalloc(self):
         """tm_invariants_noalloc: the invariants with no device allocation
