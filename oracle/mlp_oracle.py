@@ -6,13 +6,15 @@ has no counterpart in the reference; what it follows is the definition in
 include/tmcore_gpu.h:
   * the MLP 40 -> 128 -> 128 -> 128 -> 2 with exact GELU x*Phi(x)
     (Hendrycks & Gimpel 2016), forward and backward;
-  * optionally the BF16 rounding points of the device path (weights,
+  * optionally the rounding points of the device path (weights,
     activations, dY, dZ rounded to nearest-even BF16, products accumulated
-    exactly here, in FP32 on the device);
+    exactly here, in FP32 on the device) and its forward GELU, the chord
+    interpolant of GELU on 1024 equal intervals of [-6, 6] (gelu_chord);
   * the 40 inputs of an edge from the stage fields, the state correction and
     the training loss with its gradient with respect to the outputs.
 Pinned by: torch float64 autograd (forward, backward), torch's bfloat16
-conversion (rounding), central finite differences (loss gradient), the
+conversion (rounding), the interpolation error bound h^2/8 max|GELU''| and
+exactness at the nodes (gelu_chord), central finite differences (loss gradient), the
 edge-orientation invariance of the inputs and exact mass conservation of the
 correction (tests/test_closure_oracle.py).
 """
@@ -52,19 +54,38 @@ def gelu(z):
     return 0.5 * z * (1.0 + erf(z / np.sqrt(2.0)))
 
 
+GELU_N, GELU_LO, GELU_HI = 1024, -6.0, 6.0
+
+
+def gelu_chord(z):
+    """The device's forward GELU: on interval j = clamp(trunc((z + 6) / h),
+    0, N - 1) of [-6, 6] (h = 12 / N), the line through GELU at its two
+    ends, continued beyond the end intervals."""
+    h = (GELU_HI - GELU_LO) / GELU_N
+    x0 = GELU_LO + np.arange(GELU_N) * h
+    g0, g1 = gelu(x0), gelu(x0 + h)
+    b = (g1 - g0) / h
+    a = g0 - b * x0
+    z = np.asarray(z, np.float64)
+    t = np.trunc((z - GELU_LO) / h)
+    j = np.clip(np.nan_to_num(t, nan=0.0), 0, GELU_N - 1).astype(np.int64)
+    return a[j] + b[j] * z
+
+
 def gelu_d(z):
     return 0.5 * (1.0 + erf(z / np.sqrt(2.0))) + z * np.exp(-0.5 * z * z) / np.sqrt(2.0 * np.pi)
 
 
 def forward(params, X, bf=False):
-    """Y[n,2] and the cache for backward.  bf=True rounds where the device does."""
+    """Y[n,2] and the cache for backward.  bf=True rounds where the device
+    does and takes the device's GELU (gelu_chord)."""
     P = unpack(params)
     r = bf16 if bf else (lambda a: np.asarray(a, np.float64))
     A = r(X)
     cache = {"A": [A], "Z": []}
     for k in (1, 2, 3):
         Z = A @ r(P[f"W{k}"]) + P[f"b{k}"]
-        H = gelu(Z)
+        H = gelu_chord(Z) if bf else gelu(Z)
         cache["Z"].append(Z)
         A = r(H)
         cache["A"].append(A)

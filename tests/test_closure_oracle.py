@@ -61,6 +61,26 @@ def test_gelu_special_values():
     np.testing.assert_allclose(mo.gelu_d(z), (mo.gelu(z + h) - mo.gelu(z - h)) / (2 * h), atol=1e-8)
 
 
+def test_gelu_chord_is_the_interpolant_within_its_error_bound():
+    """The device's forward GELU (gelu_chord): exact at the 1025 nodes of
+    [-6, 6], within the linear-interpolation bound h^2/8 max|GELU''| of GELU
+    between them (GELU'' = phi(z)(2 - z^2) peaks at z = 0 with 2 phi(0)), the
+    bound attained next to z = 0; beyond +-6 the end lines' slopes differ from
+    GELU's by at most 6 phi(5.99) + Phi(-6) < 4e-8, so the error grows at most
+    that fast."""
+    h = 12.0 / mo.GELU_N
+    nodes = -6.0 + h * np.arange(mo.GELU_N + 1)
+    assert np.abs(mo.gelu_chord(nodes) - mo.gelu(nodes)).max() <= 1e-15
+    z = np.linspace(-6, 6, 1_000_001)
+    err = np.abs(mo.gelu_chord(z) - mo.gelu(z))
+    bound = h * h / 8 * 2 / np.sqrt(2 * np.pi)
+    assert err.max() <= bound * (1 + 1e-9)
+    assert err.max() >= 0.99 * bound and abs(z[err.argmax()]) < h
+    far = np.concatenate([np.linspace(-50, -6, 1001), np.linspace(6, 50, 1001)])
+    assert np.all(np.abs(mo.gelu_chord(far) - mo.gelu(far)) <= 1e-8 + 4e-8 * (np.abs(far) - 6))
+    assert np.isnan(mo.gelu_chord(np.array([np.nan]))[0])
+
+
 def test_bf16_rounding_matches_torch():
     rng = np.random.default_rng(3)
     x = np.concatenate([rng.standard_normal(10000) * 10.0 ** rng.integers(-8, 8, 10000),

@@ -44,7 +44,10 @@ typedef unsigned short u16x4_t __attribute__((ext_vector_type(4)));
 
 constexpr int kSub = 1;                      // 16-row MFMA column blocks per wave tile
 constexpr int kTileRows = 16 * kSub;         // rows per wave tile
-constexpr int kMlpWaves = 16 / kSub;         // waves per workgroup (one workgroup per CU)
+// waves per workgroup (one workgroup per CU): 15, so that the packs, the
+// waves' activation tiles, the biases and the GELU table fit the 160 KB LDS
+constexpr int kMlpWaves = 15;
+static_assert(kSub == 1, "the LDS budget above is for 16-row tiles");
 constexpr int kMlpBlock = 64 * kMlpWaves;
 constexpr int kGradBlock = 512;              // k_mlp_grad: 8 waves, one 16-feature slice each
 constexpr int kActStride = kMlpH + 8;        // bf16 per LDS activation row (conflict-free 16-byte reads)
@@ -68,8 +71,8 @@ __device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
 // erf(u) = 1 - t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) exp(-u^2), t = 1 /
 // (1 + p u); for x < 0 the complement Q itself is 2 Phi(x), without
 // cancellation.  Explicit fused multiply-adds (the files are built without
-// contraction), the hardware reciprocal and 2^y: the epilogue is the MLP's
-// VALU-bound part.
+// contraction), the hardware reciprocal and 2^y.  Used for the nodes of the
+// forward pass's GELU table and for GELU' of the training forward.
 struct Phi2 {
   float two_phi, e;
 };
@@ -83,6 +86,30 @@ __device__ __forceinline__ Phi2 phi2(float x) {
   return Phi2{x >= 0.f ? 2.0f - q : q, e};
 }
 __device__ __forceinline__ float gelu(float x) { return (0.5f * x) * phi2(x).two_phi; }
+
+// GELU of the forward pass: the chord interpolant of GELU on kGeluN equal
+// intervals of [-6, 6], each interval's line a + b z kept in LDS.  Its error
+// is at most h^2 / 8 max|GELU''| = h^2 / 8 * 2 phi(0) = 1.37e-5 (h = 12 /
+// 1024), below the BF16 rounding that follows; outside [-6, 6] the end
+// intervals' lines continue (GELU(z) - z resp. GELU(z) are below 1e-8 in
+// magnitude at |z| = 6).  Three VALU operations, a clamp and one LDS read
+// instead of the reciprocal, 2^y and polynomial of phi2.
+constexpr int kGeluN = 1024;
+constexpr float kGeluLo = -6.0f, kGeluH = 12.0f / kGeluN, kGeluInvH = kGeluN / 12.0f;
+__device__ __forceinline__ void gelu_table(float2 *tab, int tid, int nthreads) {
+  for (int i = tid; i < kGeluN; i += nthreads) {
+    const float x0 = fmaf((float)i, kGeluH, kGeluLo), x1 = x0 + kGeluH;
+    const float g0 = gelu(x0), b = (gelu(x1) - g0) * kGeluInvH;
+    tab[i] = make_float2(fmaf(-b, x0, g0), b);
+  }
+}
+__device__ __forceinline__ float gelu_tab(const float2 *tab, float z) {
+  // truncation toward zero: z < -6 gives index <= 0, NaN gives 0 (and a NaN result)
+  int j = (int)fmaf(z, kGeluInvH, -kGeluLo * kGeluInvH);
+  j = min(max(j, 0), kGeluN - 1);
+  const float2 ab = tab[j];
+  return fmaf(ab.y, z, ab.x);
+}
 // GELU'(x) = Phi(x) + x phi(x), phi(x) = exp(-x^2 / 2) / sqrt(2 pi)
 __device__ __forceinline__ float gelu_d(float x) {
   const Phi2 p = phi2(x);
@@ -156,11 +183,12 @@ __device__ __forceinline__ void wave_mma(const bf16x8_t *pack, const uint16_t (*
   }
 }
 
-// Hidden-layer epilogue: bias + GELU into the wave's LDS tile (the next
-// layer's B operand); when training, H and GELU'(Z) to HBM (BF16 [n][128]).
+// Hidden-layer epilogue: bias + GELU (the LDS table) into the wave's LDS
+// tile (the next layer's B operand); when training, H and GELU'(Z) (A&S) to
+// HBM (BF16 [n][128]).
 __device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[kSub][8], const float *bias,
                                                 uint16_t (*act)[kActStride], int lane, int row0, int n, bool train,
-                                                uint16_t *H, uint16_t *D) {
+                                                uint16_t *H, uint16_t *D, const float2 *gtab) {
   const int q = lane >> 4, r = lane & 15;
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
@@ -172,13 +200,8 @@ __device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[kSub][8], c
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float z = acc[s][mt][i] + b[i];
-        if (train) {
-          const Phi2 ph = phi2(z);
-          g[i] = (0.5f * z) * ph.two_phi;
-          gd[i] = fmaf(z * 0.39894228040143268f, ph.e, 0.5f * ph.two_phi);
-        } else {
-          g[i] = gelu(z);
-        }
+        g[i] = gelu_tab(gtab, z);
+        if (train) gd[i] = gelu_d(z);
       }
       const uint2 h = make_uint2(pk_bf16(g[0], g[1]), pk_bf16(g[2], g[3]));
       const int row = s * 16 + r;
@@ -197,8 +220,10 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_fwd(MlpPtrs p, const float *_
   __shared__ bf16x8_t packs[kPackFrags];
   __shared__ __attribute__((aligned(16))) uint16_t act[kMlpWaves][kTileRows][kActStride];
   __shared__ __attribute__((aligned(16))) float bias[kNBias];
+  __shared__ float2 gtab[kGeluN];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   stage_packs(packs, p.p1);
+  gelu_table(gtab, threadIdx.x, kMlpBlock);
   for (int i = threadIdx.x; i < kNBias; i += kMlpBlock) {
     const int l = i / kMlpH;
     bias[i] = p.params[(l == 0 ? kOB1 : l == 1 ? kOB2 : l == 2 ? kOB3 : kOB4) + i % kMlpH];
@@ -221,11 +246,11 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_fwd(MlpPtrs p, const float *_
       *reinterpret_cast<f32x4_t *>(&A[i / 3][kMlpIn + 8 * (i % 3)]) = f32x4_t{0.f, 0.f, 0.f, 0.f};
     f32x4_t acc[kSub][8];
     wave_mma<2, 8>(P1, A, acc, lane);
-    hidden_epilogue(acc, bias, A, lane, row0, n, train, H, D);
+    hidden_epilogue(acc, bias, A, lane, row0, n, train, H, D, gtab);
     wave_mma<4, 8>(P2, A, acc, lane);
-    hidden_epilogue(acc, bias + kMlpH, A, lane, row0, n, train, H + plane, D + plane);
+    hidden_epilogue(acc, bias + kMlpH, A, lane, row0, n, train, H + plane, D + plane, gtab);
     wave_mma<4, 8>(P3, A, acc, lane);
-    hidden_epilogue(acc, bias + 2 * kMlpH, A, lane, row0, n, train, H + 2 * plane, D + 2 * plane);
+    hidden_epilogue(acc, bias + 2 * kMlpH, A, lane, row0, n, train, H + 2 * plane, D + 2 * plane, gtab);
     f32x4_t out[kSub][1];
     wave_mma<4, 1>(P4, A, out, lane);
     if (lane < 16) {  // outputs 0 and 1 are features 4q + 0, 1 of q = 0
