@@ -199,7 +199,10 @@ int tm_hbm_copy_probe(int device, double bytes, int iters, double *gbps);
  * Batched ensemble (no reference counterpart): many members on meshes of
  * uneven size, each level group advanced with its own dt, all of them in one
  * persistent kernel launch (replayed from a hipGraph) per tm_ens_run.
- * Members of a level share its mesh constants; state is stored member-fastest.
+ * Members of a level share its mesh constants.  A level's members are cut
+ * into slices of consecutive members (tm_ens_set_slices), each slice's state
+ * stored member-fastest; the slices of a level are spread over the GPU's
+ * XCDs, so that each XCD's L2 holds the member rows of its own slice.
  * Every member runs the same CRK4 + beta scheme as tm_step (precision from
  * tm_opts, FP64 double-double global sums and invariants per member).
  * ------------------------------------------------------------------------ */
@@ -224,9 +227,14 @@ int tm_ens_download(tm_ens *e, int level, int member, double *gd, double *iap_u,
  * invariants of every member after the run come back through
  * tm_ens_invariants. */
 int tm_ens_run(tm_ens *e, const int *nsteps, double *ms);
+/* Member slices of level l: 1..8, dividing its member count; set before the
+ * first tm_ens_upload (it re-lays out the level's storage).  tm_ens_create
+ * picks the most slices, up to eight, that keep 16 members in a slice. */
+int tm_ens_set_slices(tm_ens *e, int level, int slices);
+int tm_ens_get_slices(tm_ens *e, int level, int *slices);
 int tm_ens_invariants(tm_ens *e, int level, int member, double out[4]);
 /* Tuning: elements per work chunk, workgroups per CU of the persistent grid
- * and members per thread (1, 2 or 4; it must divide every level's member
+ * and members per thread (1, 2 or 4; it must divide every slice's member
  * count and span at most 16 bytes).  tm_ens_autotune times candidates over
  * `ticks` ticks on a saved copy of the state (restored afterwards), keeps the
  * fastest and returns it in best = {chunk, blocks_per_cu, members_per_thread}. */

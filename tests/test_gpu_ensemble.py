@@ -18,11 +18,13 @@ def rel_l2(a, b):
     return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
 
 
-def _run_case(precision, tol, ntracers=0, tuning=None, specs=((3, 5, 600.0, 6), (4, 3, 300.0, 4))):
+def _run_case(precision, tol, ntracers=0, tuning=None, specs=((3, 5, 600.0, 6), (4, 3, 300.0, 4)), slices=None):
     # specs: (level, members, dt, steps): uneven sizes and counts
     meshes = {L: Mesh(L) for L, *_ in specs}
     ens = Ensemble([(meshes[L], meshes[L].fit(), M, dt) for L, M, dt, _ in specs], precision=precision,
-                   ntracers=ntracers)
+                   ntracers=ntracers, slices=slices)
+    if slices is not None:
+        assert ens.slices == list(slices)
     if tuning:
         ens.set_tuning(*tuning)
     states = {}
@@ -68,6 +70,33 @@ def test_ensemble_members_per_thread(precision, width):
     print(f"ensemble precision={precision} members_per_thread={width}", flush=True)
     _run_case(precision, 1e-12 if precision == 64 else 1e-5, ntracers=1, tuning=(128, 1, width),
               specs=((3, 8, 600.0, 6), (4, 4, 300.0, 4)))
+
+
+@pytest.mark.parametrize("precision,slices", [(32, (8, 2)), (32, (4, 8)), (64, (2, 4))])
+def test_ensemble_member_slices_match_single_runs(precision, slices):
+    """Members cut into slices (each slice its own member-fastest arrays,
+    the slices dealt to the XCDs) still match their single-model runs."""
+    print(f"ensemble precision={precision} slices={slices}", flush=True)
+    _run_case(precision, 1e-12 if precision == 64 else 1e-5, ntracers=1, slices=slices,
+              specs=((3, 16, 600.0, 6), (4, 8, 300.0, 4)))
+
+
+def test_ensemble_slice_rules():
+    """Slices divide the level's members, number 1..8, and are fixed once
+    state is uploaded; the default keeps 16 members in a slice."""
+    m = Mesh(2)
+    ens = Ensemble([(m, m.fit(), 16, 600.0), (m, m.fit(), 6, 600.0), (m, m.fit(), 128, 600.0)], precision=32)
+    assert ens.slices == [1, 1, 8]
+    for bad in (0, 3, 9, 16):
+        with pytest.raises(Exception):
+            ens.set_slices(0, bad)
+    ens.set_slices(0, 2)
+    assert ens.slices == [2, 1, 8]
+    gd, u, ghs, _ = seeded_state(m, 3)
+    ens.upload(0, 0, gd, u, ghs)
+    with pytest.raises(Exception, match="before the first upload"):
+        ens.set_slices(0, 4)
+    ens.close()
 
 
 def test_ensemble_rejects_bad_member_width():

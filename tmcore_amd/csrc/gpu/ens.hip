@@ -1,8 +1,13 @@
 // Batched ensemble of shallow-water members on meshes of uneven size,
 // advanced by one persistent kernel per run.
 //
-// Layout: every state and intermediate array of a level group is
-// [element][member] (member fastest).  A workgroup takes a chunk of elements;
+// Layout: the members of a level group are cut into slices of consecutive
+// members (tm_ens_set_slices); every state and intermediate array of a slice
+// is [element][member] (member fastest).  The slices are the kernel's
+// sub-levels: each XCD takes its contiguous eighth of every level's chunk
+// list (slice after slice), so with eight slices an XCD advances one slice
+// over the whole mesh and its gathered member rows stay in that XCD's L2.
+// A workgroup takes a chunk of elements;
 // each thread carries W consecutive members of one element as one vector
 // value (W = 4 for FP32 and 2 for FP64 when M allows), so an element's
 // constants (indices, TRiSK and fit weights, geometry) are loaded once per W
@@ -33,7 +38,9 @@
 namespace tmk {
 
 constexpr int kEnsBlock = 256;
-constexpr int kMaxLevels = 4;
+constexpr int kMaxLevels = 4;   // level groups
+constexpr int kMaxSlices = 8;   // member slices per level group
+constexpr int kMaxSub = kMaxLevels * kMaxSlices;
 
 template <typename R>
 struct EnsLevelDev {
@@ -63,7 +70,10 @@ struct EnsCtl {
   unsigned long long wait_limit;  // wall-clock ticks (100 MHz) a barrier may wait
   int T;                          // ticks of this run
   int chunk;                      // elements per chunk
-  int nlev;
+  int nlev;                       // sub-levels (slices of all level groups)
+  int ngroups;                    // level groups
+  int first[kMaxLevels];          // first sub-level of each group
+  int nsl[kMaxLevels];            // slices of each group
 };
 
 // Grid barrier with a bounded wait.  Returns false when the run is aborted.
@@ -314,25 +324,31 @@ __device__ __forceinline__ int phase_chunks(const EnsLevelDev<R> &L, int phase, 
   return (n + chunk - 1) / chunk;
 }
 
-// Deals the chunks of all active levels to the workgroups, each XCD taking a
-// contiguous eighth of the list.
+// Deals the chunks of all active sub-levels to the workgroups.  Each level
+// group's chunk list (its slices one after another) is cut into kXcd
+// contiguous parts, XCD x taking part x of every group; the XCD's workgroups
+// stride over its parts.  f(sub-level, chunk).
 template <typename R, typename F>
 __device__ __forceinline__ void for_chunks(const EnsLevelDev<R> *lv, const EnsCtl &c, const bool *act, int phase,
                                            F f) {
-  int cnt[kMaxLevels], total = 0;
-  for (int l = 0; l < c.nlev; ++l) {
-    cnt[l] = act[l] ? phase_chunks(lv[l], phase, c.chunk) : 0;
-    total += cnt[l];
-  }
   const int G = gridDim.x, xcd = blockIdx.x % kXcd, G8 = G / kXcd, bi = blockIdx.x / kXcd;
-  const int lo = (int)((long long)total * xcd / kXcd), hi = (int)((long long)total * (xcd + 1) / kXcd);
-  for (int g = lo + bi; g < hi; g += G8) {
-    int l = 0, r = g;
-    while (r >= cnt[l]) {
-      r -= cnt[l];
-      ++l;
+  int lo[kMaxLevels], cnt[kMaxLevels], per[kMaxLevels], total = 0;
+  for (int g = 0; g < c.ngroups; ++g) {
+    const int s0 = c.first[g];
+    per[g] = act[s0] ? phase_chunks(lv[s0], phase, c.chunk) : 0;
+    const long long n = (long long)per[g] * c.nsl[g];
+    lo[g] = (int)(n * xcd / kXcd);
+    cnt[g] = (int)(n * (xcd + 1) / kXcd) - lo[g];
+    total += cnt[g];
+  }
+  for (int i = bi; i < total; i += G8) {
+    int g = 0, r = i;
+    while (r >= cnt[g]) {
+      r -= cnt[g];
+      ++g;
     }
-    f(l, r);
+    r += lo[g];
+    f(c.first[g] + r / per[g], r % per[g]);
   }
 }
 
@@ -340,7 +356,7 @@ __device__ __forceinline__ void for_chunks(const EnsLevelDev<R> *lv, const EnsCt
 template <typename R>
 __device__ void ens_reduce(const EnsLevelDev<R> *lv, const EnsCtl &c, const bool *act, bool inv_only) {
   __shared__ double2 sh[kEnsBlock / 64];
-  int total = 0, cnt[kMaxLevels];
+  int total = 0, cnt[kMaxSub];
   for (int l = 0; l < c.nlev; ++l) {
     cnt[l] = act[l] ? lv[l].M * kNumSums : 0;
     total += cnt[l];
@@ -394,7 +410,7 @@ __device__ void ens_energy_out(const EnsLevelDev<R> *lv, const EnsCtl &c) {
 template <typename R, int W>
 __global__ __launch_bounds__(kEnsBlock) void k_ens_run(const EnsLevelDev<R> *__restrict__ lv, EnsCtl c) {
   __shared__ double lds[2 * kEnsBlock * W];
-  bool act[kMaxLevels];
+  bool act[kMaxSub];
   for (int t = 0; t < c.T; ++t) {
     for (int l = 0; l < c.nlev; ++l) act[l] = level_active(lv[l].nsteps, c.T, t);
     for (int k = 1; k <= 4; ++k) {
@@ -415,7 +431,7 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_run(const EnsLevelDev<R> *__r
     if (!grid_sync(c)) return;
   }
   // refresh: invariants of the final state of every member
-  bool all[kMaxLevels];
+  bool all[kMaxSub];
   for (int l = 0; l < c.nlev; ++l) all[l] = true;
   for_chunks<R>(lv, c, all, 0, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 0, 0, true, ch, c.chunk, lds); });
   if (!grid_sync(c)) return;
@@ -472,12 +488,17 @@ struct tm_ens {
   std::string err;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  int nlev = 0;
+  int nlev = 0;  // level groups
   size_t rsz = 8;
   std::vector<MeshDev> md;
   std::vector<int> M;
   std::vector<double> dt;
+  std::vector<int> S;      // member slices per level group
+  std::vector<int> first;  // first sub-level (slice) of each group
+  int nsub = 0;
+  bool uploaded = false;   // slices are fixed once state is on the device
   std::vector<void *> allocs;
+  std::vector<void *> level_allocs;  // the sub-levels' storage (replaced by tm_ens_set_slices)
   std::vector<EnsLevelDev<double>> lv64;
   std::vector<EnsLevelDev<float>> lv32;
   void *d_lv = nullptr;
@@ -494,11 +515,17 @@ struct tm_ens {
     err = m;
     return code;
   }
-  void *alloc(size_t bytes) {
+  void *alloc(size_t bytes, bool level = false) {
     void *p = nullptr;
     if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) return nullptr;
-    allocs.push_back(p);
+    (level ? level_allocs : allocs).push_back(p);
     return p;
+  }
+  // sub-level and member within it of member m of group l
+  int sub_of(int l, int m, int *j) const {
+    const int ms = M[l] / S[l];
+    *j = m % ms;
+    return first[l] + m / ms;
   }
 };
 
@@ -510,10 +537,21 @@ struct tm_ens {
 
 namespace {
 
+// Default slices of a level group of M members: the most (up to eight, one
+// per XCD) that leave at least 16 members in a slice.  Each slice re-reads
+// the mesh constants, so slices pay only where the member rows are wide
+// (measured on the 128/32/8-member level 6/7/8 mix: 8 slices of the 128
+// members +13%, slices of 8 or 32 members neutral or slower).
+int default_slices(int M) {
+  for (int S : {8, 4, 2})
+    if (M % S == 0 && M / S >= 16) return S;
+  return 1;
+}
+
 template <typename R>
 int ens_alloc_level(tm_ens *E, int l, EnsLevelDev<R> &L) {
   const MeshDev &md = E->md[l];
-  const int M = E->M[l], nT = E->o.ntracers;
+  const int M = E->M[l] / E->S[l], nT = E->o.ntracers;
   const size_t nC = md.dm.nC, nE = md.dm.nE, nV = md.dm.nV, rs = sizeof(R);
   L.m = md.dm;
   L.M = M;
@@ -524,7 +562,7 @@ int ens_alloc_level(tm_ens *E, int l, EnsLevelDev<R> &L) {
   L.conserve_energy = E->o.conserve_energy;
   L.inv_total_area = 1.0 / md.totalArea;
   L.inv_sum_area_tri = 1.0 / md.sumAreaTri;
-  auto A = [&](size_t n) { return (R *)E->alloc(n * M * rs); };
+  auto A = [&](size_t n) { return (R *)E->alloc(n * M * rs, true); };
   R **cells[] = {&L.gd_old, &L.ghs, &L.gdS[0], &L.gdS[1], &L.acc_gd, &L.t_gd[0], &L.t_gd[1], &L.E, &L.tend_gd};
   for (R **p : cells)
     if (!(*p = A(nC))) return E->fail(TM_ENOMEM, "hipMalloc ensemble cell state");
@@ -539,10 +577,10 @@ int ens_alloc_level(tm_ens *E, int l, EnsLevelDev<R> &L) {
     if (!(*p = A(nq))) return E->fail(TM_ENOMEM, "hipMalloc ensemble tracer state");
   // partials sized for the smallest chunk the tuner may pick (32 elements)
   L.part_stride = (int)((std::max(nE, std::max(nC, nV)) + 31) / 32);
-  if (!(L.part = (double2 *)E->alloc((size_t)kNumSums * M * L.part_stride * sizeof(double2))) ||
-      !(L.sums = (double *)E->alloc((size_t)M * kNumSums * sizeof(double))) ||
-      !(L.beta = (double *)E->alloc((size_t)M * sizeof(double))) ||
-      !(L.inv = (double *)E->alloc((size_t)M * 4 * sizeof(double))))
+  if (!(L.part = (double2 *)E->alloc((size_t)kNumSums * M * L.part_stride * sizeof(double2), true)) ||
+      !(L.sums = (double *)E->alloc((size_t)M * kNumSums * sizeof(double), true)) ||
+      !(L.beta = (double *)E->alloc((size_t)M * sizeof(double), true)) ||
+      !(L.inv = (double *)E->alloc((size_t)M * 4 * sizeof(double), true)))
     return E->fail(TM_ENOMEM, "hipMalloc ensemble sums");
   ENSCHK(E, hipMemsetAsync(L.gd_old, 0, nC * M * rs, E->stream));
   ENSCHK(E, hipMemsetAsync(L.iap_old, 0, nE * M * rs, E->stream));
@@ -576,8 +614,9 @@ int put_column(tm_ens *E, int l, R *dst, const double *src, const std::vector<in
 }
 
 template <typename R>
-int ens_upload(tm_ens *E, int l, int m, const double *gd, const double *u, const double *ghs, const double *q) {
-  EnsLevelDev<R> &L = levels<R>(E)[l];
+int ens_upload(tm_ens *E, int l, int m0, const double *gd, const double *u, const double *ghs, const double *q) {
+  int m = 0;
+  EnsLevelDev<R> &L = levels<R>(E)[E->sub_of(l, m0, &m)];
   const MeshDev &md = E->md[l];
   const int M = L.M, nC = md.dm.nC, nE = md.dm.nE;
   auto scatter = [&](R *dst, const double *src, const std::vector<int> &perm, int n, int copies) -> int {
@@ -608,8 +647,9 @@ int ens_upload(tm_ens *E, int l, int m, const double *gd, const double *u, const
 }
 
 template <typename R>
-int ens_download(tm_ens *E, int l, int m, double *gd, double *iap, double *q) {
-  EnsLevelDev<R> &L = levels<R>(E)[l];
+int ens_download(tm_ens *E, int l, int m0, double *gd, double *iap, double *q) {
+  int m = 0;
+  EnsLevelDev<R> &L = levels<R>(E)[E->sub_of(l, m0, &m)];
   const MeshDev &md = E->md[l];
   const int M = L.M;
   auto gather = [&](const R *src, double *dst, const std::vector<int> &perm, int n, int copies) -> int {
@@ -634,15 +674,19 @@ template <typename R>
 int ens_launch(tm_ens *E, const int *nsteps, double *ms) {
   auto &lv = levels<R>(E);
   int T = 0;
-  for (int l = 0; l < E->nlev; ++l) {
+  for (int l = 0; l < E->nlev; ++l)
     if (nsteps[l] < 0) return E->fail(TM_EINVAL, "negative step count");
-    lv[l].nsteps = nsteps[l];
+  EnsCtl c{E->bar, E->bar + 1, E->abort_flag, E->wait_limit, 0, E->chunk, E->nsub, E->nlev, {}, {}};
+  for (int l = 0; l < E->nlev; ++l) {
+    for (int s = 0; s < E->S[l]; ++s) lv[E->first[l] + s].nsteps = nsteps[l];
+    c.first[l] = E->first[l];
+    c.nsl[l] = E->S[l];
     T = std::max(T, nsteps[l]);
   }
+  c.T = T;
   ENSCHK(E, hipMemcpyAsync(E->d_lv, lv.data(), lv.size() * sizeof(EnsLevelDev<R>), hipMemcpyHostToDevice,
                            E->stream));
   ENSCHK(E, hipMemsetAsync(E->abort_flag, 0, sizeof(int), E->stream));
-  EnsCtl c{E->bar, E->bar + 1, E->abort_flag, E->wait_limit, T, E->chunk, E->nlev};
   const int G = std::max(kXcd, (E->cus * E->bpc / kXcd) * kXcd);
   // The persistent launch is captured in a hipGraph and replayed once.
   hipGraph_t g = nullptr;
@@ -683,27 +727,35 @@ int ens_launch(tm_ens *E, const int *nsteps, double *ms) {
   return TM_OK;
 }
 
+// Storage of every slice of every level group (sub-levels in group order).
 template <typename R>
 int ens_create_levels(tm_ens *E) {
   auto &lv = levels<R>(E);
-  lv.resize(E->nlev);
+  E->first.assign(E->nlev, 0);
+  E->nsub = 0;
   for (int l = 0; l < E->nlev; ++l) {
-    int rc = ens_alloc_level<R>(E, l, lv[l]);
-    if (rc) return rc;
+    E->first[l] = E->nsub;
+    E->nsub += E->S[l];
   }
-  if (!(E->d_lv = E->alloc(sizeof(EnsLevelDev<R>) * E->nlev))) return E->fail(TM_ENOMEM, "hipMalloc levels");
+  lv.assign(E->nsub, EnsLevelDev<R>{});
+  for (int l = 0; l < E->nlev; ++l)
+    for (int s = 0; s < E->S[l]; ++s) {
+      int rc = ens_alloc_level<R>(E, l, lv[E->first[l] + s]);
+      if (rc) return rc;
+    }
+  if (!E->d_lv && !(E->d_lv = E->alloc(sizeof(EnsLevelDev<R>) * kMaxSub))) return E->fail(TM_ENOMEM, "hipMalloc levels");
   int occ[3] = {0, 0, 0};
   ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[0], k_ens_run<R, 1>, kEnsBlock, 0));
   ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[1], k_ens_run<R, 2>, kEnsBlock, 0));
   ENSCHK(E, hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ[2], k_ens_run<R, 4>, kEnsBlock, 0));
   for (int i = 0; i < 3; ++i) E->max_bpc[i] = std::max(1, occ[i]);
-  // widest vector of members that divides every level's member count:
+  // widest vector of members that divides every slice's member count:
   // 16-byte rows (4 x FP32, 2 x FP64)
   E->W = 1;
   for (int w : {2, 4}) {
     if (w * sizeof(R) > 16) continue;
     bool ok = true;
-    for (int l = 0; l < E->nlev; ++l) ok = ok && (E->M[l] % w == 0);
+    for (int l = 0; l < E->nlev; ++l) ok = ok && ((E->M[l] / E->S[l]) % w == 0);
     if (ok) E->W = w;
   }
   E->bpc = std::min(E->bpc, E->max_bpc[E->W == 4 ? 2 : E->W - 1]);
@@ -756,6 +808,7 @@ extern "C" int tm_ens_create(tm_ens **out, int nlevels, const tm_ens_level *leve
     int rc = build_mesh_dev(E->md[l], levels_in[l].mesh, levels_in[l].fit, lo, E->stream, E->allocs, e);
     if (rc) return bail(E->fail(rc, e));
     E->M.push_back(levels_in[l].members);
+    E->S.push_back(default_slices(levels_in[l].members));
     E->dt.push_back(levels_in[l].dt);
     maxn = std::max(maxn, (size_t)std::max(levels_in[l].mesh->nEdges,
                                            levels_in[l].mesh->nCells * std::max(o->ntracers, 1)));
@@ -774,6 +827,7 @@ extern "C" int tm_ens_create(tm_ens **out, int nlevels, const tm_ens_level *leve
 extern "C" int tm_ens_destroy(tm_ens *E) {
   if (!E) return TM_EINVAL;
   for (void *p : E->allocs) (void)hipFree(p);
+  for (void *p : E->level_allocs) (void)hipFree(p);
   for (void *p : E->backup) (void)hipFree(p);
   if (E->ev0) (void)hipEventDestroy(E->ev0);
   if (E->ev1) (void)hipEventDestroy(E->ev1);
@@ -789,6 +843,7 @@ extern "C" int tm_ens_upload(tm_ens *E, int l, int m, const double *gd, const do
   if (!E) return TM_EINVAL;
   if (l < 0 || l >= E->nlev || m < 0 || m >= E->M[l]) return E->fail(TM_EINVAL, "level/member out of range");
   if (!gd || !u || !ghs || (E->o.ntracers > 0 && !q)) return E->fail(TM_EINVAL, "missing state array");
+  E->uploaded = true;
   const MeshDev &md = E->md[l];
   for (int i = 0; i < md.dm.nC; ++i)
     if (!(gd[i] > 0.0) || !std::isfinite(gd[i]) || !std::isfinite(ghs[i]))
@@ -804,6 +859,31 @@ extern "C" int tm_ens_download(tm_ens *E, int l, int m, double *gd, double *iap,
   return E->rsz == 8 ? ens_download<double>(E, l, m, gd, iap, q) : ens_download<float>(E, l, m, gd, iap, q);
 }
 
+extern "C" int tm_ens_set_slices(tm_ens *E, int l, int slices) {
+  if (!E) return TM_EINVAL;
+  if (l < 0 || l >= E->nlev) return E->fail(TM_EINVAL, "level out of range");
+  if (slices < 1 || slices > kMaxSlices || E->M[l] % slices)
+    return E->fail(TM_EINVAL, "slices must be 1..8 and divide the level's members");
+  if (E->uploaded) return E->fail(TM_EINVAL, "slices are set before the first upload");
+  ENSCHK(E, hipStreamSynchronize(E->stream));
+  for (void *p : E->level_allocs) (void)hipFree(p);
+  E->level_allocs.clear();
+  for (void *p : E->backup) (void)hipFree(p);
+  E->backup.clear();
+  E->S[l] = slices;
+  int rc = E->rsz == 8 ? ens_create_levels<double>(E) : ens_create_levels<float>(E);
+  if (rc) return rc;
+  ENSCHK(E, hipStreamSynchronize(E->stream));
+  return TM_OK;
+}
+
+extern "C" int tm_ens_get_slices(tm_ens *E, int l, int *slices) {
+  if (!E || !slices) return TM_EINVAL;
+  if (l < 0 || l >= E->nlev) return E->fail(TM_EINVAL, "level out of range");
+  *slices = E->S[l];
+  return TM_OK;
+}
+
 extern "C" int tm_ens_run(tm_ens *E, const int *nsteps, double *ms) {
   if (!E || !nsteps) return TM_EINVAL;
   return E->rsz == 8 ? ens_launch<double>(E, nsteps, ms) : ens_launch<float>(E, nsteps, ms);
@@ -812,8 +892,10 @@ extern "C" int tm_ens_run(tm_ens *E, const int *nsteps, double *ms) {
 extern "C" int tm_ens_invariants(tm_ens *E, int l, int m, double out[4]) {
   if (!E || !out) return TM_EINVAL;
   if (l < 0 || l >= E->nlev || m < 0 || m >= E->M[l]) return E->fail(TM_EINVAL, "level/member out of range");
-  const double *src = E->rsz == 8 ? E->lv64[l].inv : E->lv32[l].inv;
-  ENSCHK(E, hipMemcpy(out, src + (size_t)m * 4, 4 * sizeof(double), hipMemcpyDeviceToHost));
+  int j = 0;
+  const int sl = E->sub_of(l, m, &j);
+  const double *src = E->rsz == 8 ? E->lv64[sl].inv : E->lv32[sl].inv;
+  ENSCHK(E, hipMemcpy(out, src + (size_t)j * 4, 4 * sizeof(double), hipMemcpyDeviceToHost));
   return TM_OK;
 }
 
@@ -823,7 +905,7 @@ bool w_valid(const tm_ens *E, int w) {
   if (w != 1 && w != 2 && w != 4) return false;
   if ((size_t)w * E->rsz > 16) return false;
   for (int l = 0; l < E->nlev; ++l)
-    if (E->M[l] % w) return false;
+    if ((E->M[l] / E->S[l]) % w) return false;
   return true;
 }
 }  // namespace
@@ -832,7 +914,8 @@ extern "C" int tm_ens_set_tuning(tm_ens *E, int chunk, int bpc, int members_per_
   if (!E) return TM_EINVAL;
   if (chunk < 32 || chunk > 4096 || bpc < 1) return E->fail(TM_EINVAL, "chunk must be 32..4096, blocks_per_cu >= 1");
   if (!w_valid(E, members_per_thread))
-    return E->fail(TM_EINVAL, "members_per_thread must be 1, 2 or 4, divide every level's members and span <= 16 bytes");
+    return E->fail(TM_EINVAL,
+                   "members_per_thread must be 1, 2 or 4, divide every slice's members and span <= 16 bytes");
   if (bpc > E->max_bpc[wslot(members_per_thread)])
     return E->fail(TM_EINVAL, "blocks_per_cu exceeds the occupancy of the persistent kernel");
   E->chunk = chunk;
@@ -846,7 +929,7 @@ template <typename R>
 int ens_state_copy(tm_ens *E, bool save) {
   auto &lv = levels<R>(E);
   if (save && E->backup.empty()) {
-    for (int l = 0; l < E->nlev; ++l) {
+    for (int l = 0; l < E->nsub; ++l) {
       const size_t M = lv[l].M, nC = lv[l].m.nC, nE = lv[l].m.nE, nq = std::max<size_t>((size_t)lv[l].nT * nC, 1);
       for (size_t n : {nC, nE, nq}) {
         void *p = nullptr;
@@ -855,7 +938,7 @@ int ens_state_copy(tm_ens *E, bool save) {
       }
     }
   }
-  for (int l = 0; l < E->nlev; ++l) {
+  for (int l = 0; l < E->nsub; ++l) {
     const size_t M = lv[l].M, nC = lv[l].m.nC, nE = lv[l].m.nE, nq = std::max<size_t>((size_t)lv[l].nT * nC, 1);
     void *dev[3] = {lv[l].gd_old, lv[l].iap_old, lv[l].q_old};
     const size_t n[3] = {nC, nE, nq};

@@ -35,18 +35,22 @@ def _bind(L):
     L.tm_ens_invariants.argtypes = [_vp, C.c_int, C.c_int, _dp]
     L.tm_ens_set_tuning.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.tm_ens_autotune.argtypes = [_vp, C.c_int, C.POINTER(C.c_int), _dp]
+    L.tm_ens_set_slices.argtypes = [_vp, C.c_int, C.c_int]
+    L.tm_ens_get_slices.argtypes = [_vp, C.c_int, C.POINTER(C.c_int)]
     for f in ("tm_ens_create", "tm_ens_destroy", "tm_ens_upload", "tm_ens_download", "tm_ens_run",
-              "tm_ens_invariants", "tm_ens_set_tuning", "tm_ens_autotune"):
+              "tm_ens_invariants", "tm_ens_set_tuning", "tm_ens_autotune", "tm_ens_set_slices",
+              "tm_ens_get_slices"):
         getattr(L, f).restype = C.c_int
     L._ens_bound = True
     return L
 
 
 class Ensemble:
-    """levels: list of (mesh, fit, members, dt)."""
+    """levels: list of (mesh, fit, members, dt).  slices: member slices per
+    level (tm_ens_set_slices), None for the library's default."""
 
     def __init__(self, levels, *, precision: int = 32, ntracers: int = 0, radius: float = RADIUS,
-                 interp_pv_order: int = 3, device: int = 0):
+                 interp_pv_order: int = 3, device: int = 0, slices=None):
         L = _bind(gpu_lib())
         o = Opts()
         L.tm_opts_default(C.byref(o))
@@ -65,6 +69,22 @@ class Ensemble:
             raise _ffi.TmError(rc, "tm_ens_create")
         self.h = h
         self.nT = ntracers
+        if slices is not None:
+            for lev, s in enumerate(slices):
+                self.set_slices(lev, s)
+
+    def set_slices(self, level, slices):
+        """tm_ens_set_slices: before the first upload."""
+        self._check(gpu_lib().tm_ens_set_slices(self.h, level, int(slices)), "tm_ens_set_slices")
+
+    @property
+    def slices(self):
+        out = []
+        for lev in range(len(self.levels)):
+            s = C.c_int()
+            self._check(gpu_lib().tm_ens_get_slices(self.h, lev, C.byref(s)), "tm_ens_get_slices")
+            out.append(s.value)
+        return out
 
     def _check(self, rc, where):
         if rc != 0:

@@ -1,12 +1,13 @@
 """Batched-ensemble benchmark (one persistent launch for every member).
 
     python -m tmcore_amd.ensemble_bench [--days 2]
-    python -m tmcore_amd.ensemble_bench --group 6:128:300 [--steps 20] [--tuning 64,1,4]
+    python -m tmcore_amd.ensemble_bench --group 6:128:300 [--steps 20] [--tuning 64,1,4] [--slices 8]
 
 128 members on level 6 (dt 300 s), 32 on level 7 (dt 150 s) and 8 on level 8
 (dt 90 s), FP32 state with FP64 invariants, member m seeded 1000 + m, advanced
 `days` simulated days after the autotuner has picked chunk size, workgroups
-per CU and members per thread.  Prints one JSON line: member-days per hour of
+per CU and members per thread (member slices per level: the library's
+default unless given).  Prints one JSON line: member-days per hour of
 the whole run and the step latency of each level group run alone.
 """
 from __future__ import annotations
@@ -41,10 +42,10 @@ def step_bytes(mesh, fit, members, ntracers=0, rs=4.0, fsz=4.0):
     return 4 * (cst + 7 * rs * prog) + 3 * rs * prog
 
 
-def config4(days=2.0, spec=SPEC):
+def config4(days=2.0, spec=SPEC, slices=None):
     meshes = [Mesh(L) for L, _, _ in spec]
     fits = [m.fit() for m in meshes]
-    ens = Ensemble([(m, f, M, dt) for m, f, (L, M, dt) in zip(meshes, fits, spec)], precision=32)
+    ens = Ensemble([(m, f, M, dt) for m, f, (L, M, dt) in zip(meshes, fits, spec)], precision=32, slices=slices)
     seed = 1000
     for li, (m, (L, M, dt)) in enumerate(zip(meshes, spec)):
         for k in range(M):
@@ -58,6 +59,7 @@ def config4(days=2.0, spec=SPEC):
     inv = np.array([ens.invariants(li, k) for li, (_, M, _) in enumerate(spec) for k in range(M)])
     out = {"member_days_per_hour": member_days / (ms / 3.6e6), "run_s": ms / 1e3, "member_days": member_days,
            "members": {f"L{L}": M for L, M, _ in spec}, "steps": {f"L{L}": s for (L, _, _), s in zip(spec, steps)},
+           "slices": {f"L{L}": s for (L, _, _), s in zip(spec, ens.slices)},
            "autotune": {"chunk": chunk, "blocks_per_cu": bpc, "members_per_thread": width,
                         "ms_per_tick": round(tick_ms, 4)},
            "finite_invariants": bool(np.isfinite(inv).all())}
@@ -84,15 +86,17 @@ def config4(days=2.0, spec=SPEC):
     return out
 
 
-def group_steps(spec, steps, tuning=None):
+def group_steps(spec, steps, tuning=None, slices=None):
     """One level group alone (spec = (level, members, dt)) for `steps` steps
     after two warm-up steps, at the given (chunk, blocks_per_cu,
-    members_per_thread) or the autotuner's choice: ms per step and the
-    compulsory-bytes roofline of the step."""
+    members_per_thread) or the autotuner's choice and the given member slices
+    or the library's default: ms per step and the compulsory-bytes roofline
+    of the step."""
     L, M, dt = spec
     m = Mesh(L)
     f = m.fit()
-    ens = Ensemble([(m, f, M, dt)], precision=32)
+    ens = Ensemble([(m, f, M, dt)], precision=32, slices=None if slices is None else [slices])
+    used = ens.slices[0]
     for k in range(M):
         gd, u, ghs, _ = seeded_state(m, 1000 + k)
         ens.upload(0, k, gd, u, ghs)
@@ -106,6 +110,7 @@ def group_steps(spec, steps, tuning=None):
     ens.close()
     b = step_bytes(m, f, M)
     return {"group": f"L{L}x{M}", "ms_per_step": round(ms, 4), "bytes_per_step": b, "tuning": tuning or tuned,
+            "slices": used,
             "GBps": round(b / (ms * 1e-3) / 1e9, 1), "frac": round(b / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
@@ -115,13 +120,17 @@ def main():
     ap.add_argument("--group", default=None, help="level:members:dt of one group run alone (e.g. 6:128:300)")
     ap.add_argument("--steps", type=int, default=20, help="steps of --group")
     ap.add_argument("--tuning", default=None, help="chunk,blocks_per_cu,members_per_thread (default: autotune)")
+    ap.add_argument("--slices", default=None,
+                    help="member slices: one number for --group, comma-separated per level otherwise")
     args = ap.parse_args()
+    sl = [int(x) for x in args.slices.split(",")] if args.slices else None
     if args.group:
         L, M, dt = args.group.split(":")
         tuning = tuple(int(x) for x in args.tuning.split(",")) if args.tuning else None
-        print(json.dumps(group_steps((int(L), int(M), float(dt)), args.steps, tuning)), flush=True)
+        print(json.dumps(group_steps((int(L), int(M), float(dt)), args.steps, tuning, sl[0] if sl else None)),
+              flush=True)
     else:
-        print(json.dumps(config4(args.days)), flush=True)
+        print(json.dumps(config4(args.days, slices=sl)), flush=True)
 
 
 if __name__ == "__main__":
