@@ -17,8 +17,8 @@
 //
 // Schedule: the run is T = max(nsteps) ticks.  Level l takes a step in tick t
 // when floor((t+1) S_l / T) > floor(t S_l / T), so its S_l steps are spread
-// evenly.  A tick is 4 stages x 4 phases + reduce + finalize; phases are
-// separated by a grid barrier.  The barrier waits are bounded by a wall-clock
+// evenly.  A tick is an edge pass, 4 stages x 3 phases + reduce + finalize;
+// phases are separated by a grid barrier.  The barrier waits are bounded by a wall-clock
 // limit: a wave that waits longer raises the abort flag, every other wave
 // sees it at its next wait, and the kernel returns with an error instead of
 // hanging.
@@ -56,6 +56,7 @@ struct EnsLevelDev {
   R *acc_gd, *acc_iap, *acc_q;
   R *t_gd[2], *t_iap[2];
   R *gd_e, *u, *fk, *udc, *E, *tend_gd, *pv_v, *pv_e;  // fk: [2 nE][M] (StageBufs::fk)
+  R *gd_e2, *u2;  // second gd_edge/u set: stages 2 and 4 (the tendency phase forms the next stage's)
   double2 *part;  // [kNumSums][M][part_stride]
   int part_stride;
   double *sums;   // [M][kNumSums]
@@ -136,8 +137,15 @@ __device__ __forceinline__ StageBufs<R> ens_bufs(const EnsLevelDev<R> &L, int k,
   b.tprev_iap = L.t_iap[(k + 1) & 1];
   b.tcur_gd = L.t_gd[k & 1];
   b.tcur_iap = L.t_iap[k & 1];
-  b.gd_e = L.gd_e;
-  b.u = L.u;
+  // the tendency phase of stages 1-3 forms the next stage's edge values
+  // (kXTendPrep) into the other gd_edge/u set: stages 1 and 3 read set 0,
+  // stages 2 and 4 set 1
+  const int es = (k == 2 || k == 4) ? 1 : 0;
+  b.gd_e = es ? L.gd_e2 : L.gd_e;
+  b.u = es ? L.u2 : L.u;
+  b.gd_e_n = es ? L.gd_e : L.gd_e2;
+  b.u_n = es ? L.u : L.u2;
+  b.prep_next = k >= 1 && k <= 3 ? 1 : 0;
   b.fk = L.fk;
   b.udc = L.udc;
   b.E = L.E;
@@ -252,7 +260,7 @@ __device__ void ens_chunk(const EnsLevelDev<R> &L, int phase, int k, bool inv, i
     D s[2] = {D(0.0), D(0.0)};
     if (live)
       for (int e = e0 + lane; e < e1; e += lanes) {
-        const EdgeSums<D> r = edge_tend_e<R, Lay>(L.m, b, at, e);
+        const EdgeSums<D> r = edge_tend_e<R, Lay, kXTendPrep>(L.m, b, at, e);
         s[0] += r.rp;
         s[1] += r.r44;
       }
@@ -414,9 +422,13 @@ __global__ __launch_bounds__(kEnsBlock) void k_ens_run(const EnsLevelDev<R> *__r
   for (int t = 0; t < c.T; ++t) {
     for (int l = 0; l < c.nlev; ++l) act[l] = level_active(lv[l].nsteps, c.T, t);
     for (int k = 1; k <= 4; ++k) {
-      // phases: 0 edge prep, 1 cells (+ 4 vertices in the same pass), 2 PV, 3 tendency
-      for_chunks<R>(lv, c, act, 0, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 0, k, k == 1, ch, c.chunk, lds); });
-      if (!grid_sync(c)) return;
+      // phases: 0 edge prep (stage 1 only: the tendency phase of stages 1-3
+      // forms the next stage's edge values), 1 cells (+ 4 vertices in the
+      // same pass), 2 PV, 3 tendency
+      if (k == 1) {
+        for_chunks<R>(lv, c, act, 0, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 0, k, true, ch, c.chunk, lds); });
+        if (!grid_sync(c)) return;
+      }
       for_chunks<R>(lv, c, act, 1, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 1, k, k == 1, ch, c.chunk, lds); });
       for_chunks<R>(lv, c, act, 4, [&](int l, int ch) { ens_chunk<R, W>(lv[l], 4, k, k == 1, ch, c.chunk, lds); });
       if (!grid_sync(c)) return;
@@ -566,7 +578,8 @@ int ens_alloc_level(tm_ens *E, int l, EnsLevelDev<R> &L) {
   R **cells[] = {&L.gd_old, &L.ghs, &L.gdS[0], &L.gdS[1], &L.acc_gd, &L.t_gd[0], &L.t_gd[1], &L.E, &L.tend_gd};
   for (R **p : cells)
     if (!(*p = A(nC))) return E->fail(TM_ENOMEM, "hipMalloc ensemble cell state");
-  R **edges[] = {&L.iap_old, &L.iap_s, &L.acc_iap, &L.t_iap[0], &L.t_iap[1], &L.gd_e, &L.u, &L.udc, &L.pv_e};
+  R **edges[] = {&L.iap_old, &L.iap_s, &L.acc_iap, &L.t_iap[0], &L.t_iap[1], &L.gd_e, &L.u, &L.udc, &L.pv_e,
+                  &L.gd_e2, &L.u2};
   for (R **p : edges)
     if (!(*p = A(nE))) return E->fail(TM_ENOMEM, "hipMalloc ensemble edge state");
   if (!(L.fk = A(2 * nE))) return E->fail(TM_ENOMEM, "hipMalloc ensemble edge state");
