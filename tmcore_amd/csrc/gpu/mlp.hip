@@ -11,20 +11,24 @@
 //
 // Every product is formed transposed, D' = W^T H^T: the weights are the A
 // operand, the activation rows the B operand, so a lane's four results are
-// four consecutive features of one row and go to LDS or HBM as one 8-byte
-// store.  Weights are repacked once per update as BF16 "packs" in the A
-// layout (one 16-byte fragment per lane per MFMA): P(W) for the forward pass
-// (m over a layer's outputs, k over its inputs) and Q(W) for the backward
-// pass (m over inputs, k over outputs).  A workgroup (8 waves, one per CU)
-// copies the four packs it needs (84 KB) into LDS once and then loops over
-// 32-row tiles, one per wave at a time; a wave's activations stay in its own
-// LDS tile between layers.
+// four consecutive features of one row and go to HBM as one 8-byte store.
+// Weights are repacked once per update as BF16 "packs" in the A layout (one
+// 16-byte fragment per lane per MFMA): P(W) for the forward pass (m over a
+// layer's outputs, k over its inputs) and Q(W) for the backward pass (m over
+// inputs, k over outputs).  A workgroup (one per CU) copies the four packs
+// it needs (84 KB) into LDS once and then loops over row tiles, one per wave
+// at a time.  The forward pass keeps a tile's activations in registers
+// between layers (two lane swaps per dword reorder a layer's output into the
+// next layer's B operand; see k_mlp_fwd); the backward pass keeps them in
+// the wave's own LDS tile.
 //
 // Layers: Z1 = X W1 + b1, H1 = GELU(Z1); Z2 = H1 W2 + b2, H2 = GELU(Z2);
 // Z3 = H2 W3 + b3, H3 = GELU(Z3); Y = H3 W4 + b4.  GELU is the erf form of
-// Hendrycks & Gimpel (2016), x * Phi(x), with erf from Abramowitz & Stegun
-// (1964) eq. 7.1.26 (|error| <= 1.5e-7, far below the BF16 rounding every
-// activation then takes): branch-free, and GELU' reuses its exponential.
+// Hendrycks & Gimpel (2016), x * Phi(x).  The forward pass reads it from a
+// table in LDS, the chord interpolant on 1024 intervals of [-6, 6] (error
+// <= 1.37e-5); GELU' of the training forward and the table's nodes use erf
+// from Abramowitz & Stegun (1964) eq. 7.1.26 (|error| <= 1.5e-7).  Both are
+// far below the BF16 rounding every activation then takes.
 
 #include <hip/hip_runtime.h>
 
@@ -154,8 +158,8 @@ struct MlpPtrs {
 
 // Copies a pass's four packs (laid out back to back, kPackFrags fragments)
 // into LDS.
-__device__ __forceinline__ void stage_packs(bf16x8_t *lds, const bf16x8_t *g) {
-  for (int i = threadIdx.x; i < kPackFrags; i += kMlpBlock) lds[i] = g[i];
+__device__ __forceinline__ void stage_packs(bf16x8_t *lds, const bf16x8_t *g, int nthreads = kMlpBlock) {
+  for (int i = threadIdx.x; i < kPackFrags; i += nthreads) lds[i] = g[i];
 }
 
 constexpr int kNBias = 3 * kMlpH + kMlpOut;  // b1, b2, b3, b4 back to back in LDS
@@ -183,79 +187,155 @@ __device__ __forceinline__ void wave_mma(const bf16x8_t *pack, const uint16_t (*
   }
 }
 
-// Hidden-layer epilogue: bias + GELU (the LDS table) into the wave's LDS
-// tile (the next layer's B operand); when training, H and GELU'(Z) (A&S) to
-// HBM (BF16 [n][128]).
-__device__ __forceinline__ void hidden_epilogue(const f32x4_t (&acc)[kSub][8], const float *bias,
-                                                uint16_t (*act)[kActStride], int lane, int row0, int n, bool train,
-                                                uint16_t *H, uint16_t *D, const float2 *gtab) {
+// ---------------------------------------------------------------- forward
+// Forward pass: each wave takes 32-row tiles (two 16-row MFMA column blocks,
+// so every weight fragment read from LDS feeds two MFMAs) and keeps the
+// activations in registers between layers.  A layer's result in lane (q, r)
+// is features 4q + 0..3 of m-tile mt for row r; the next layer's B operand
+// for k-step ks needs features 32 ks + 8q + 0..7 of row r, i.e. m-tile
+// 2ks + q/2, features 8(q%2) + 0..7, which lanes (2(q%2), r) and
+// (2(q%2) + 1, r) hold.  v_permlane32_swap then v_permlane16_swap on the
+// packed m-tiles 2ks and 2ks+1 move exactly those: the first operand ends as
+// [2ks of rows 0, 2 | 2ks+1 of rows 0, 2], the second as [2ks of rows 1, 3 |
+// 2ks+1 of rows 1, 3] (rows of 16 lanes), the low and high halves of the B
+// fragment.  LDS holds only the P packs, the biases and the GELU table.
+// 16-row column blocks per wave tile: two for inference (126 VGPRs, 16 waves
+// per workgroup, one workgroup per CU); one for the training form, which
+// also forms GELU' and stores H and GELU'(Z) (measured: 16 waves of 16-row
+// tiles faster than 12 waves of 32-row tiles, which need 150 VGPRs)
+constexpr int fwd_sub(bool train) { return train ? 1 : 2; }
+constexpr int kFwdWaves = 16;
+constexpr int kFwdBlock = 64 * kFwdWaves;
+
+typedef uint32_t u32x4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bf16x8_t bf16x8_of(const f32x4_t &lo, const f32x4_t &hi) {
+  return __builtin_bit_cast(bf16x8_t, (u32x4_t{pk_bf16(lo[0], lo[1]), pk_bf16(lo[2], lo[3]), pk_bf16(hi[0], hi[1]),
+                                               pk_bf16(hi[2], hi[3])}));
+}
+
+// Layer-1 B fragments of one 16-row block: lane (q, r) takes X[row][32 ks +
+// 8q + 0..7] (zero past the 40 inputs and past the last row).
+__device__ __forceinline__ void x_fragments(const float *X, int n, int row, int q, bf16x8_t *b) {
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    const int c = 32 * ks + 8 * q;
+    f32x4_t lo = f32x4_t{0.f, 0.f, 0.f, 0.f}, hi = lo;
+    if (c < kMlpIn && row < n) {
+      const f32x4_t *src = reinterpret_cast<const f32x4_t *>(X + (size_t)row * kMlpIn + c);
+      lo = src[0];
+      hi = src[1];
+    }
+    b[ks] = bf16x8_of(lo, hi);
+  }
+}
+
+// acc[s][mt] = pack(ks, mt) x b[s][ks] summed over ks < KS.
+template <int KS, int MT, int SUB>
+__device__ __forceinline__ void wave_mma_r(const bf16x8_t *pack, const bf16x8_t (&b)[SUB][4],
+                                           f32x4_t (&acc)[SUB][MT], int lane) {
+#pragma unroll
+  for (int s = 0; s < SUB; ++s)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) acc[s][mt] = f32x4_t{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const bf16x8_t a = pack[(ks * MT + mt) * 64 + lane];
+#pragma unroll
+      for (int s = 0; s < SUB; ++s)
+        acc[s][mt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[s][ks], acc[s][mt], 0, 0, 0);
+    }
+}
+
+// Hidden-layer epilogue: bias + GELU (the LDS table), BF16, into the next
+// layer's B fragments b; when training, H and GELU'(Z) (A&S) to HBM (BF16
+// [n][128]).
+template <bool TRAIN, int SUB = fwd_sub(TRAIN)>
+__device__ __forceinline__ void fwd_epilogue(const f32x4_t (&acc)[SUB][8], const float *bias, const float2 *gtab,
+                                             int lane, int row0, int n, uint16_t *H, uint16_t *D,
+                                             bf16x8_t (&b)[SUB][4]) {
   const int q = lane >> 4, r = lane & 15;
+  uint32_t t[SUB][8][2];
 #pragma unroll
   for (int mt = 0; mt < 8; ++mt) {
     const int f = mt * 16 + 4 * q;
-    const f32x4_t b = *reinterpret_cast<const f32x4_t *>(bias + f);
+    const f32x4_t bv = *reinterpret_cast<const f32x4_t *>(bias + f);
 #pragma unroll
-    for (int s = 0; s < kSub; ++s) {
+    for (int s = 0; s < SUB; ++s) {
       float g[4], gd[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const float z = acc[s][mt][i] + b[i];
+        const float z = acc[s][mt][i] + bv[i];
         g[i] = gelu_tab(gtab, z);
-        if (train) gd[i] = gelu_d(z);
+        if constexpr (TRAIN) gd[i] = gelu_d(z);
       }
-      const uint2 h = make_uint2(pk_bf16(g[0], g[1]), pk_bf16(g[2], g[3]));
-      const int row = s * 16 + r;
-      *reinterpret_cast<uint2 *>(&act[row][f]) = h;
-      if (train && row0 + row < n) {
-        *reinterpret_cast<uint2 *>(&H[(size_t)(row0 + row) * kMlpH + f]) = h;
-        *reinterpret_cast<uint2 *>(&D[(size_t)(row0 + row) * kMlpH + f]) =
+      t[s][mt][0] = pk_bf16(g[0], g[1]);
+      t[s][mt][1] = pk_bf16(g[2], g[3]);
+      const int row = row0 + 16 * s + r;
+      if (TRAIN && row < n) {
+        *reinterpret_cast<uint2 *>(&H[(size_t)row * kMlpH + f]) = make_uint2(t[s][mt][0], t[s][mt][1]);
+        *reinterpret_cast<uint2 *>(&D[(size_t)row * kMlpH + f]) =
             make_uint2(pk_bf16(gd[0], gd[1]), pk_bf16(gd[2], gd[3]));
       }
     }
   }
+#pragma unroll
+  for (int s = 0; s < SUB; ++s)
+#pragma unroll
+    for (int ks = 0; ks < 4; ++ks) {
+      uint32_t w[4];
+#pragma unroll
+      for (int d = 0; d < 2; ++d) {
+        const auto p32 = __builtin_amdgcn_permlane32_swap(t[s][2 * ks][d], t[s][2 * ks + 1][d], false, false);
+        const auto p16 = __builtin_amdgcn_permlane16_swap(p32[0], p32[1], false, false);
+        w[d] = p16[0];
+        w[2 + d] = p16[1];
+      }
+      b[s][ks] = __builtin_bit_cast(bf16x8_t, (u32x4_t{w[0], w[1], w[2], w[3]}));
+    }
 }
 
-__global__ __launch_bounds__(kMlpBlock) void k_mlp_fwd(MlpPtrs p, const float *__restrict__ X, int n,
-                                                       float *__restrict__ Y, int train, uint16_t *H, uint16_t *D) {
+template <bool TRAIN>
+__global__ __launch_bounds__(kFwdBlock) void k_mlp_fwd(MlpPtrs p, const float *__restrict__ X, int n,
+                                                       float *__restrict__ Y, uint16_t *H, uint16_t *D) {
+  constexpr int kFwdSub = fwd_sub(TRAIN), kFwdRows = 16 * kFwdSub;
   __shared__ bf16x8_t packs[kPackFrags];
-  __shared__ __attribute__((aligned(16))) uint16_t act[kMlpWaves][kTileRows][kActStride];
   __shared__ __attribute__((aligned(16))) float bias[kNBias];
   __shared__ float2 gtab[kGeluN];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  stage_packs(packs, p.p1);
-  gelu_table(gtab, threadIdx.x, kMlpBlock);
-  for (int i = threadIdx.x; i < kNBias; i += kMlpBlock) {
+  stage_packs(packs, p.p1, kFwdBlock);
+  gelu_table(gtab, threadIdx.x, kFwdBlock);
+  for (int i = threadIdx.x; i < kNBias; i += kFwdBlock) {
     const int l = i / kMlpH;
     bias[i] = p.params[(l == 0 ? kOB1 : l == 1 ? kOB2 : l == 2 ? kOB3 : kOB4) + i % kMlpH];
   }
   __syncthreads();
   const bf16x8_t *P1 = packs, *P2 = P1 + kP1, *P3 = P2 + kP2, *P4 = P3 + kP2;
-  uint16_t(*A)[kActStride] = act[wave];
   const size_t plane = (size_t)n * kMlpH;
-  const int ntiles = (n + kTileRows - 1) / kTileRows;
-  for (int tile = blockIdx.x * kMlpWaves + wave; tile < ntiles; tile += gridDim.x * kMlpWaves) {
-    const int row0 = tile * kTileRows;
-    // X (FP32, 40 per row) -> BF16 tile columns 0..39, zeros to 63
-    for (int i = lane; i < kTileRows * (kMlpIn / 4); i += 64) {
-      const int r = i / (kMlpIn / 4), c = 4 * (i % (kMlpIn / 4));
-      f32x4_t x = f32x4_t{0.f, 0.f, 0.f, 0.f};
-      if (row0 + r < n) x = *reinterpret_cast<const f32x4_t *>(&X[(size_t)(row0 + r) * kMlpIn + c]);
-      *reinterpret_cast<uint2 *>(&A[r][c]) = make_uint2(pk_bf16(x[0], x[1]), pk_bf16(x[2], x[3]));
-    }
-    for (int i = lane; i < kTileRows * 3; i += 64)
-      *reinterpret_cast<f32x4_t *>(&A[i / 3][kMlpIn + 8 * (i % 3)]) = f32x4_t{0.f, 0.f, 0.f, 0.f};
-    f32x4_t acc[kSub][8];
-    wave_mma<2, 8>(P1, A, acc, lane);
-    hidden_epilogue(acc, bias, A, lane, row0, n, train, H, D, gtab);
-    wave_mma<4, 8>(P2, A, acc, lane);
-    hidden_epilogue(acc, bias + kMlpH, A, lane, row0, n, train, H + plane, D + plane, gtab);
-    wave_mma<4, 8>(P3, A, acc, lane);
-    hidden_epilogue(acc, bias + 2 * kMlpH, A, lane, row0, n, train, H + 2 * plane, D + 2 * plane, gtab);
-    f32x4_t out[kSub][1];
-    wave_mma<4, 1>(P4, A, out, lane);
+  const int ntiles = (n + kFwdRows - 1) / kFwdRows;
+  for (int tile = blockIdx.x * kFwdWaves + wave; tile < ntiles; tile += gridDim.x * kFwdWaves) {
+    // the weight fragments are loop-invariant LDS reads: without this
+    // compiler barrier they are hoisted out of the tile loop (84 fragments,
+    // 336 VGPRs) and spill; read per tile, they stream through a few VGPRs
+    __asm__ volatile("" ::: "memory");
+    const int row0 = tile * kFwdRows;
+    bf16x8_t b[kFwdSub][4];
+#pragma unroll
+    for (int s = 0; s < kFwdSub; ++s) x_fragments(X, n, row0 + 16 * s + (lane & 15), lane >> 4, b[s]);
+    f32x4_t acc[kFwdSub][8];
+    wave_mma_r<2, 8>(P1, b, acc, lane);
+    fwd_epilogue<TRAIN>(acc, bias, gtab, lane, row0, n, H, D, b);
+    wave_mma_r<4, 8>(P2, b, acc, lane);
+    fwd_epilogue<TRAIN>(acc, bias + kMlpH, gtab, lane, row0, n, H + plane, D + plane, b);
+    wave_mma_r<4, 8>(P3, b, acc, lane);
+    fwd_epilogue<TRAIN>(acc, bias + 2 * kMlpH, gtab, lane, row0, n, H + 2 * plane, D + 2 * plane, b);
+    f32x4_t out[kFwdSub][1];
+    wave_mma_r<4, 1>(P4, b, out, lane);
     if (lane < 16) {  // outputs 0 and 1 are features 4q + 0, 1 of q = 0
 #pragma unroll
-      for (int s = 0; s < kSub; ++s) {
+      for (int s = 0; s < kFwdSub; ++s) {
         const int row = row0 + s * 16 + lane;
         if (row < n) {
           float2 y;
@@ -553,6 +633,11 @@ int mlp_blocks(const MlpState &s, int rows) {
   return std::max(1, std::min(s.ncu, (tiles + kMlpWaves - 1) / kMlpWaves));
 }
 
+int fwd_blocks(const MlpState &s, int rows, bool train) {
+  const int tiles = (rows + 16 * fwd_sub(train) - 1) / (16 * fwd_sub(train));
+  return std::max(1, std::min(s.ncu, (tiles + kFwdWaves - 1) / kFwdWaves));
+}
+
 // Row ranges of the gradient partials: one per CU, at least 64 rows each.
 int grad_parts(const MlpState &s, int rows) { return std::max(1, std::min(s.ncu, (rows + 255) / 256)); }
 
@@ -607,8 +692,12 @@ int mlp_pack(MlpState &s, hipStream_t st) {
 int mlp_forward(MlpState &s, const float *X, int n, float *Y, bool train, hipStream_t st) {
   if (n <= 0) return TM_OK;
   if (train && n > s.cap) return fail(s, TM_EINVAL, "batch exceeds the reserved MLP batch buffers");
-  hipLaunchKernelGGL(k_mlp_fwd, dim3(mlp_blocks(s, n)), dim3(kMlpBlock), 0, st, ptrs(s), X, n, Y, train ? 1 : 0,
-                     s.H, s.D);
+  if (train)
+    hipLaunchKernelGGL(k_mlp_fwd<true>, dim3(fwd_blocks(s, n, true)), dim3(kFwdBlock), 0, st, ptrs(s), X, n, Y, s.H,
+                       s.D);
+  else
+    hipLaunchKernelGGL(k_mlp_fwd<false>, dim3(fwd_blocks(s, n, false)), dim3(kFwdBlock), 0, st, ptrs(s), X, n, Y, s.H,
+                       s.D);
   return hipGetLastError() == hipSuccess ? TM_OK : fail(s, TM_EDEVICE, "k_mlp_fwd launch");
 }
 
