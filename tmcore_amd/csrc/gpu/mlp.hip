@@ -157,9 +157,22 @@ struct MlpPtrs {
 };
 
 // Copies a pass's four packs (laid out back to back, kPackFrags fragments)
-// into LDS.
-__device__ __forceinline__ void stage_packs(bf16x8_t *lds, const bf16x8_t *g, int nthreads = kMlpBlock) {
-  for (int i = threadIdx.x; i < kPackFrags; i += nthreads) lds[i] = g[i];
+// into LDS with NT threads: every load is issued before the first store, so
+// the ~6 loads of a thread wait once, not one after another.
+template <int NT>
+__device__ __forceinline__ void stage_packs(bf16x8_t *lds, const bf16x8_t *g) {
+  constexpr int kPer = (kPackFrags + NT - 1) / NT;
+  bf16x8_t v[kPer];
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < kPackFrags) v[j] = g[i];
+  }
+#pragma unroll
+  for (int j = 0; j < kPer; ++j) {
+    const int i = threadIdx.x + j * NT;
+    if (i < kPackFrags) lds[i] = v[j];
+  }
 }
 
 constexpr int kNBias = 3 * kMlpH + kMlpOut;  // b1, b2, b3, b4 back to back in LDS
@@ -305,7 +318,7 @@ __global__ __launch_bounds__(kFwdBlock) void k_mlp_fwd(MlpPtrs p, const float *_
   __shared__ __attribute__((aligned(16))) float bias[kNBias];
   __shared__ float2 gtab[kGeluN];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  stage_packs(packs, p.p1, kFwdBlock);
+  stage_packs<kFwdBlock>(packs, p.p1);
   gelu_table(gtab, threadIdx.x, kFwdBlock);
   for (int i = threadIdx.x; i < kNBias; i += kFwdBlock) {
     const int l = i / kMlpH;
@@ -378,7 +391,7 @@ __global__ __launch_bounds__(kMlpBlock) void k_mlp_bwd(MlpPtrs p, const float *_
   __shared__ bf16x8_t packs[kPackFrags];
   __shared__ __attribute__((aligned(16))) uint16_t act[kMlpWaves][kTileRows][kActStride];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  stage_packs(packs, p.q1);
+  stage_packs<kMlpBlock>(packs, p.q1);
   __syncthreads();
   const bf16x8_t *Q1 = packs, *Q2 = Q1 + kQ1, *Q3 = Q2 + kQ2, *Q4 = Q3 + kQ2;
   uint16_t(*A)[kActStride] = act[wave];
