@@ -431,25 +431,27 @@ def test_nonfinite_upload_is_rejected():
                                                    (64, 0, {"time_scheme": 1, "pv_scheme": 1})])
 def test_kernel_layouts_are_bit_identical(precision, ntracers, kw):
     """edgesOnEdge derived from the cells' edge rows (in the PV kernel, the
-    tendency kernel, both) gives the slots of the table in the same order,
-    and the next stage's edge values formed in the tendency kernel are the
-    edge-prep pass's: the stage diagnostics, ten steps, beta and the
+    tendency kernel, both) gives the slots of the table in the same order;
+    the next stage's edge values formed in the tendency kernel are the
+    edge-prep pass's; the tendency kernel's gathers from its LDS tile read
+    the same values: the stage diagnostics, ten steps, beta and the
     invariants are bit-identical in every mode the scheme allows.  The
-    schemes outside plain CRK4 refuse bit 4."""
+    schemes outside plain CRK4 refuse bits 4 and 8; bit 8 supersedes bit 2."""
     m = mesh(4)
     gd, u, ghs, q = seeded_state(m, 5, [201, 202][:ntracers])
     crk4 = not kw
-    modes = range(8) if crk4 else range(4)
+    modes = range(16) if crk4 else range(4)
     runs = {}
     for mode in modes:
         g = SwmGpu(m, m.fit(), dt=600.0, ntracers=ntracers, precision=precision, **kw)
-        assert g.layout & 3  # the generated mesh keeps the MPAS order
-        assert bool(g.layout & 4) == crk4
+        assert g.layout & 1  # the generated mesh keeps the MPAS order
+        assert bool(g.layout & 4) == crk4 and bool(g.layout & 8) == crk4
         if not crk4:
-            with pytest.raises(Exception, match="CRK4"):
-                g.layout = mode | 4
+            for extra in (4, 8):
+                with pytest.raises(Exception, match="CRK4"):
+                    g.layout = mode | extra
         g.layout = mode
-        assert g.layout == mode
+        assert g.layout == (mode & ~2 if mode & 8 else mode)
         g.upload(gd, ghs, u=u, tracers=q if ntracers else None)
         st = g.stage()
         inv, beta = g.step(10)
@@ -472,7 +474,7 @@ def test_upwind_correction_refuses_the_layout_modes():
     m = mesh(3)
     g = SwmGpu(m, m.fit(max_fit=31), dt=600.0, ntracers=1, adv_order=3)
     assert g.layout == 0
-    for mode in (1, 2, 4):
+    for mode in (1, 2, 4, 8):
         with pytest.raises(Exception, match="adv_order"):
             g.layout = mode
     g.close()
@@ -490,7 +492,7 @@ def test_mesh_without_mpas_edge_order_keeps_the_table_kernels():
     m.weightsOnEdge[e, :n] = np.roll(m.weightsOnEdge[e, :n], 1)
     gd, u, ghs, q = seeded_state(m, 2, [201])
     g = SwmGpu(m, m.fit(), dt=600.0, ntracers=1)
-    assert g.layout == 4  # the edge values from the tendency kernel need no MPAS order
+    assert g.layout == 12  # the tendency kernel's edge values and LDS tile need no MPAS order
     with pytest.raises(Exception, match="MPAS"):
         g.layout = 3
     o = Oracle(m, m.fit(), dt=600.0)

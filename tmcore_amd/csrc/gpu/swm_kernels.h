@@ -370,6 +370,11 @@ constexpr int kXEoePv = 8, kXEoeTend = 16;
 // stages 2-4 have no edge-prep pass.  gd_edge and u ping-pong between two
 // buffers (the tendency reads its neighbours' current values).
 constexpr int kXTendPrep = 32;
+// kXTendLds (CRK4 without the upwind correction): the tendency kernel stages
+// u, gd_edge and pv_edge of its tile and the tile's halo in LDS and gathers
+// the edgesOnEdge values from there (DMesh::eoeL, haloE); the arithmetic is
+// unchanged.
+constexpr int kXTendLds = 64;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (f at offset base; padded slots add an exact 0).
@@ -791,37 +796,69 @@ struct EdgeSums {
 // calc_u_tend_on_edge (tmcore_swm_mod.F90:155-201, flux_4th_order_correct
 // off) from E = ke + gd + ghs in b.E, gd_edge, u, the PV flux s and the edge
 // thickness tendency tge: returns u_tend, sets the IAP tendency.
+// gde and ue: the edge's own gd_edge and u.
 template <typename R, typename L>
 __device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
-                                                    val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it) {
+                                                    val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it,
+                                                    val_t<R, L::W> gde, val_t<R, L::W> ue) {
   typedef val_t<R, L::W> V;
   const int c1 = nt_ld(&m.coe[e]), c2 = nt_ld(&m.coe[m.nE + e]);
   const V dEdx = (at.ld(b.E, c2) - at.ld(b.E, c1)) / geo_at<R, true>(m.dc, m.dc32, e);
   const V ut = s - dEdx;
-  const V sg = tm_sqrt<R, L::W>(at.ld(b.gd_e, e));
-  it = sg * ut + ((R(0.5) * at.ld(b.u, e)) / sg) * tge;
+  const V sg = tm_sqrt<R, L::W>(gde);
+  it = sg * ut + ((R(0.5) * ue) / sg) * tge;
   return ut;
 }
+template <typename R, typename L>
+__device__ __forceinline__ val_t<R, L::W> u_tend_at(const DMesh &m, const StageBufs<R> &b, L at, int e,
+                                                    val_t<R, L::W> s, val_t<R, L::W> tge, val_t<R, L::W> &it) {
+  return u_tend_at<R, L>(m, b, at, e, s, tge, it, at.ld(b.gd_e, e), at.ld(b.u, e));
+}
+
+// The tile's staged rows (kXTendLds): u, gd_edge, pv_edge at local rows, the
+// thread's own edge at row tid.
+template <typename R>
+struct TendTile {
+  const R *u, *gd_e, *pv_e;
+  int tid;
+};
 
 // TRiSK PV flux, edge gradient, IAP tendency and the RK update of iap_u.
 template <typename R, typename L, int X = 0>
 __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh &m, const StageBufs<R> &b, L at,
-                                                                     int e) {
+                                                                     int e, TendTile<R> tl = {}) {
   typedef val_t<R, L::W> V;
   typedef val_t<double, L::W> D;
-  const V pve = at.ld(b.pv_e, e);
-  int ep[kMaxEoE];
-  V uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE];
+  constexpr bool kLds = (X & kXTendLds) != 0;
+  static_assert(!kLds || L::W == 1, "the LDS tile holds one member");
+  V pve, uv[kMaxEoE], gv[kMaxEoE], pv[kMaxEoE], gde_own, ue_own;
   R w[kMaxEoE];
-  if constexpr ((X & kXEoeTend) != 0)
-    eoe_row_cells(m, e, nt_ld(&m.coe[e]), nt_ld(&m.coe[m.nE + e]), ep);
-  else
-    eoe_row<true>(m, e, ep);
+  if constexpr (kLds) {
+    pve = tl.pv_e[tl.tid];
+    gde_own = tl.gd_e[tl.tid];
+    ue_own = tl.u[tl.tid];
 #pragma unroll
-  for (int q = 0; q < kMaxEoE; ++q) {
-    uv[q] = at.ld(b.u, ep[q]);
-    gv[q] = at.ld(b.gd_e, ep[q]);
-    pv[q] = at.ld(b.pv_e, ep[q]);
+    for (int p = 0; p < kMaxEoE / 2; ++p) {
+      const uint32_t pr = nt_ld(&m.eoeL[(size_t)p * m.nE + e]);
+      const int l0 = (int)(pr & 0xffffu), l1 = (int)(pr >> 16);
+      uv[2 * p] = tl.u[l0], gv[2 * p] = tl.gd_e[l0], pv[2 * p] = tl.pv_e[l0];
+      uv[2 * p + 1] = tl.u[l1], gv[2 * p + 1] = tl.gd_e[l1], pv[2 * p + 1] = tl.pv_e[l1];
+    }
+  } else {
+    pve = at.ld(b.pv_e, e);
+    int ep[kMaxEoE];
+    if constexpr ((X & kXEoeTend) != 0)
+      eoe_row_cells(m, e, nt_ld(&m.coe[e]), nt_ld(&m.coe[m.nE + e]), ep);
+    else
+      eoe_row<true>(m, e, ep);
+#pragma unroll
+    for (int q = 0; q < kMaxEoE; ++q) {
+      uv[q] = at.ld(b.u, ep[q]);
+      gv[q] = at.ld(b.gd_e, ep[q]);
+      pv[q] = at.ld(b.pv_e, ep[q]);
+    }
+    gde_own = at.ld(b.gd_e, e);
+    ue_own = at.ld(b.u, e);
   }
 #pragma unroll
   for (int p = 0; p < kMaxEoE / 2; ++p) geo_pair<R, true>(m.woe, m.woe32, p, e, m.nE, w[2 * p], w[2 * p + 1]);
@@ -833,7 +870,7 @@ __device__ __forceinline__ EdgeSums<val_t<double, L::W>> edge_tend_e(const DMesh
                                        R(0.5) * (at.ld(b.tend_gd, c1) + at.ld(b.tend_gd, c2)),
                                        sign1v<R, L::W>(at.ld(b.iap, e)));
   V it;
-  const V ut = u_tend_at<R, L>(m, b, at, e, s, tge, it);
+  const V ut = u_tend_at<R, L>(m, b, at, e, s, tge, it, gde_own, ue_own);
   if (b.d_pvflx) at.st(b.d_pvflx, e, s);
   if (b.d_utend) at.st(b.d_utend, e, ut);
   if (b.d_tiap) at.st(b.d_tiap, e, it);
@@ -965,7 +1002,27 @@ __global__ __launch_bounds__(kBlock) void k_edge_tend(DMesh m, StageBufs<R> b) {
   const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
   const int e = tile * kBlock + threadIdx.x;
   EdgeSums<double> r{0.0, 0.0, 0.0};
-  if (e < m.nE) r = edge_tend_e<R, Solo, X>(m, b, Solo{}, e);
+  if constexpr ((X & kXTendLds) != 0) {
+    static_assert(kBlock == kTileE, "a tendency tile is one workgroup");
+    // the tile's rows, then its halo's, of the three gathered edge fields
+    __shared__ R su[kTileE + kTendHaloMax], sg[kTileE + kTendHaloMax], sp[kTileE + kTendHaloMax];
+    if (e < m.nE) {
+      su[threadIdx.x] = b.u[e];
+      sg[threadIdx.x] = b.gd_e[e];
+      sp[threadIdx.x] = b.pv_e[e];
+    }
+    const int h0 = m.haloOff[tile], h1 = m.haloOff[tile + 1];
+    for (int h = h0 + (int)threadIdx.x; h < h1; h += kBlock) {
+      const int g = m.haloE[h];
+      su[kTileE + h - h0] = b.u[g];
+      sg[kTileE + h - h0] = b.gd_e[g];
+      sp[kTileE + h - h0] = b.pv_e[g];
+    }
+    __syncthreads();
+    if (e < m.nE) r = edge_tend_e<R, Solo, X>(m, b, Solo{}, e, TendTile<R>{su, sg, sp, (int)threadIdx.x});
+  } else {
+    if (e < m.nE) r = edge_tend_e<R, Solo, X>(m, b, Solo{}, e);
+  }
   if constexpr ((X & kXExt) != 0) {
     if (b.k >= 1) rk_sums_ext<R>(b, r.rp, r.rf, r.r44, 0, tile);
   } else if (b.k >= 2) {

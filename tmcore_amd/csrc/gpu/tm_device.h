@@ -39,6 +39,9 @@ enum SumId {
 // Mesh constants, new numbering, 0-based.  Slot-major: array[slot * n + i].
 // Signed connectivity packs nSignEdge / tSignEdge (mesh_mod.F90:182-203) into
 // the index: e for +1, ~e for -1.
+constexpr int kTileE = 256;         // edges per tendency-kernel tile (= the kernels' workgroup size)
+constexpr int kTendHaloMax = 512;   // halo rows one tile may stage
+
 struct DMesh {
   int nC, nE, nV;
   int nbC, nbE, nbV;  // workgroups per element kind, multiples of kXcd
@@ -72,6 +75,14 @@ struct DMesh {
   const double *d4v;      // [2][21][nE] deriv4OnVertex slots 1..21
   const int *eocR;        // [nC][8] cell-major edge rows: signed edgesOnCell slots 0-5, nEdgesOnCell, 0
                           // (null when the mesh breaks the MPAS orderings they stand for)
+  // LDS tiles of the tendency kernel (kXTendLds; null when a tile's halo
+  // exceeds kTendHaloMax): tile t is edges [kTileE t, kTileE (t+1)) of the
+  // Hilbert order, staged with its halo (the other edges its edgesOnEdge
+  // name, ascending) at local rows kTileE.. ; eoeL holds every edgesOnEdge
+  // slot as a local row, two 16-bit slots per word
+  const uint32_t *eoeL;   // [5][nE]
+  const int *haloE;       // halo edges of tile t at [haloOff[t], haloOff[t + 1])
+  const int *haloOff;     // [nbE + 1]
   const int *vocS;        // [6][nC] verticesOnCell (for the v2c gather)
   const int *cocN;        // [6][nC] cellsOnCell, unused slots = the cell itself
   const double *kiteC;    // [6][nC] kite area of (cell, verticesOnCell(slot))

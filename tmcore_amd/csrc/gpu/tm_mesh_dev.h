@@ -22,6 +22,10 @@ struct MeshDev {
   // every edgesOnEdge row is derivable from its cells' edgesOnCell rows (the
   // MPAS order the cell-row kernels rely on); dm.eocR is set
   bool cell_rows = false;
+  // every tendency-kernel tile's halo fits kTendHaloMax; dm.eoeL, haloE and
+  // haloOff are set.  haloTotal: halo rows over all tiles
+  bool tend_lds = false;
+  double haloTotal = 0.0;
 };
 
 // Validates the reference-layout mesh (and the fit when o.interp_pv_order

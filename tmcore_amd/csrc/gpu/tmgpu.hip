@@ -408,6 +408,10 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
     case kXTendPrep | kXEoeTend: launch_part_x<R, kXTendPrep | kXEoeTend>(c, which, b); break;
     case kXTendPrep | kXEoePv | kXEoeTend: launch_part_x<R, kXTendPrep | kXEoePv | kXEoeTend>(c, which, b); break;
     case kXEoeTend: launch_part_x<R, kXEoeTend>(c, which, b); break;
+    case kXTendLds: launch_part_x<R, kXTendLds>(c, which, b); break;
+    case kXTendLds | kXEoePv: launch_part_x<R, kXTendLds | kXEoePv>(c, which, b); break;
+    case kXTendLds | kXTendPrep: launch_part_x<R, kXTendLds | kXTendPrep>(c, which, b); break;
+    case kXTendLds | kXTendPrep | kXEoePv: launch_part_x<R, kXTendLds | kXTendPrep | kXEoePv>(c, which, b); break;
     case kXEoePv | kXEoeTend: launch_part_x<R, kXEoePv | kXEoeTend>(c, which, b); break;
     case kXExt | kXEoePv: launch_part_x<R, kXExt | kXEoePv>(c, which, b); break;
     case kXExt | kXEoeTend: launch_part_x<R, kXExt | kXEoeTend>(c, which, b); break;
@@ -907,6 +911,46 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     }
   }
 
+  // ---- LDS tiles of the tendency kernel: tile t = edges [kTileE t,
+  // kTileE (t+1)) of the Hilbert order; its halo is the set of other edges
+  // its edgesOnEdge slots name, ascending.  eoeL gives every slot as a local
+  // row: the tile's edges at 0..kTileE-1, the halo's after them.  A mesh with
+  // a halo beyond kTendHaloMax keeps the table (md.tend_lds stays false).
+  std::vector<uint32_t> eoeL;
+  std::vector<int> haloE, haloOff;
+  {
+    static_assert(kTileE + kTendHaloMax <= 65536, "local rows fit 16 bits");
+    const int nb = round_blocks(nE);
+    haloOff.assign((size_t)nb + 1, 0);
+    eoeL.assign((size_t)(kMaxEoE / 2) * nE, 0u);
+    std::vector<int> halo;
+    bool ok = true;
+    for (int t = 0; t < nb && ok; ++t) {
+      const int e0 = std::min(nE, t * kTileE), e1 = std::min(nE, e0 + kTileE);
+      halo.clear();
+      for (int ne = e0; ne < e1; ++ne)
+        for (int q = 0; q < kMaxEoE; ++q) {
+          const int g = eoe[pair_at(q, ne, nE)];
+          if (g < e0 || g >= e1) halo.push_back(g);
+        }
+      std::sort(halo.begin(), halo.end());
+      halo.erase(std::unique(halo.begin(), halo.end()), halo.end());
+      ok = (int)halo.size() <= kTendHaloMax;
+      for (int ne = e0; ne < e1 && ok; ++ne)
+        for (int q = 0; q < kMaxEoE; ++q) {
+          const int g = eoe[pair_at(q, ne, nE)];
+          const int l = (g >= e0 && g < e1)
+                            ? g - e0
+                            : kTileE + (int)(std::lower_bound(halo.begin(), halo.end(), g) - halo.begin());
+          eoeL[(size_t)(q / 2) * nE + ne] |= (uint32_t)l << (16 * (q % 2));
+        }
+      haloE.insert(haloE.end(), halo.begin(), halo.end());
+      haloOff[(size_t)t + 1] = (int)haloE.size();
+    }
+    md.tend_lds = ok;
+    md.haloTotal = ok ? (double)haloE.size() : 0.0;
+  }
+
   DMesh &d = md.dm;
   d.nC = nC;
   d.nE = nE;
@@ -1033,6 +1077,14 @@ int build_mesh_dev(MeshDev &md, const tm_mesh_desc *m, const tm_fit_desc *f, con
     UP(eocR, eocR, int);
   else
     d.eocR = nullptr;
+  if (md.tend_lds) {
+    UP(eoeL, eoeL, uint32_t);
+    UP(haloE, haloE, int);
+    UP(haloOff, haloOff, int);
+  } else {
+    d.eoeL = nullptr;
+    d.haloE = d.haloOff = nullptr;
+  }
 #undef UP
 
   md.sumEoE = md.sumFitE = md.sumFitV = md.sumEoC = 0.0;
@@ -1107,8 +1159,11 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   // edgesOnEdge derived from the cells' edge rows where the mesh keeps the
   // MPAS order (not in the upwind-correction variants)
   if (c->md.cell_rows && o.adv_order < 3) c->xmask |= cell_rows_default(c->rsz);
-  // next-stage edge values from the tendency kernel (plain CRK4, no upwind correction)
+  // next-stage edge values from the tendency kernel, and its gathers from an
+  // LDS tile (plain CRK4, no upwind correction); the tile replaces the
+  // cell-row derivation in the tendency kernel
   if ((c->xmask & (kXAdv | kXExt)) == 0) c->xmask |= kXTendPrep;
+  if (c->md.tend_lds && (c->xmask & (kXAdv | kXExt)) == 0) c->xmask = (c->xmask & ~kXEoeTend) | kXTendLds;
   c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
   if (o.pv_scheme == 2 &&
@@ -1151,7 +1206,11 @@ static void kernel_bytes(const tm_ctx *c, double *kb) {
   const double prog = nE + nC * (1 + T);
   // edgesOnEdge: the table, or (cellsOnEdge plus) the cells' 32-byte edge rows
   const double eoe_pv = (c->xmask & kXEoePv) ? nE * 8.0 + nC * 32.0 : 4.0 * sumEoE;
-  const double eoe_tend = (c->xmask & kXEoeTend) ? nC * 32.0 : 4.0 * sumEoE;
+  // the tendency kernel's edgesOnEdge: the table, the cells' rows, or the LDS
+  // tile's local rows (16-bit) with the halo lists
+  const double eoe_tend = (c->xmask & kXTendLds) ? nE * 20.0 + 4.0 * c->md.haloTotal + 4.0 * (c->dm.nbE + 1)
+                          : (c->xmask & kXEoeTend) ? nC * 32.0
+                                                   : 4.0 * sumEoE;
   kb[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs) + nC * rs;
   // cells, vertices, the edge products gathered (fk, u dcEdge) and the
   // tracers' cellsOnCell, u and dvEdge
@@ -1185,22 +1244,27 @@ extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_d
 
 extern "C" int tm_ctx_set_layout(tm_ctx *c, int mode) {
   if (!c) return TM_EINVAL;
-  if (mode < 0 || mode > 7) return c->fail(TM_EINVAL, "layout mode must be 0..7");
-  const int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0) | ((mode & 4) ? kXTendPrep : 0);
+  if (mode < 0 || mode > 15) return c->fail(TM_EINVAL, "layout mode must be 0..15");
+  int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0) | ((mode & 4) ? kXTendPrep : 0) |
+             ((mode & 8) ? kXTendLds : 0);
+  if (want & kXTendLds) want &= ~kXEoeTend;  // the LDS tile supersedes the cell rows in the tendency kernel
+  if ((want & kXTendLds) && !c->md.tend_lds)
+    return c->fail(TM_EINVAL, "a tendency tile's edgesOnEdge halo exceeds the LDS tile");
   if ((want & (kXEoePv | kXEoeTend)) && !c->md.cell_rows)
     return c->fail(TM_EINVAL, "the mesh's edgesOnEdge rows do not follow the MPAS order");
   if (want && (c->xmask & kXAdv)) return c->fail(TM_EINVAL, "the layout modes do not run with adv_order >= 3");
-  if ((want & kXTendPrep) && (c->xmask & kXExt))
-    return c->fail(TM_EINVAL, "edge values from the tendency kernel need CRK4 with pv_scheme 3 order <= 4");
+  if ((want & (kXTendPrep | kXTendLds)) && (c->xmask & kXExt))
+    return c->fail(TM_EINVAL, "edge values from the tendency kernel and its LDS tile need CRK4 with pv_scheme 3 order <= 4");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);  // captured steps hold the previous variant
-  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend | kXTendPrep)) | want;
+  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend | kXTendPrep | kXTendLds)) | want;
   return TM_OK;
 }
 
 extern "C" int tm_ctx_get_layout(tm_ctx *c, int *mode) {
   if (!c || !mode) return TM_EINVAL;
-  *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0) | ((c->xmask & kXTendPrep) ? 4 : 0);
+  *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0) | ((c->xmask & kXTendPrep) ? 4 : 0) |
+          ((c->xmask & kXTendLds) ? 8 : 0);
   return TM_OK;
 }
 
