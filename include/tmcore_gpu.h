@@ -72,9 +72,10 @@ int tm_ctx_destroy(tm_ctx *ctx);
  * stages 2-4 run no edge-prep pass.  Bit 8: the tendency kernel stages u,
  * gd_edge and pv_edge of its 256-edge tile and of the tile's edgesOnEdge
  * halo in LDS and gathers from there (supersedes bit 2; needs every tile's
- * halo within 512 edges, checked by tm_ctx_create).  Bits 4 and 8 need
- * time_scheme 2 (CRK4) with pv_scheme 3 and interp_pv_order <= 4.  Every
- * bit needs adv_order 2.
+ * halo within 512 edges, checked by tm_ctx_create).  Bit 16: the PV kernel
+ * does the same with u for the tangent wind (supersedes bit 1).  Bits 4, 8
+ * and 16 need time_scheme 2 (CRK4) with pv_scheme 3 and interp_pv_order
+ * <= 4.  Every bit needs adv_order 2.
  * tm_ctx_create picks the fastest mode measured for the context's precision
  * that the mesh and options allow.  Setting a mode they do not allow
  * returns TM_EINVAL.  For tests and timing comparisons. */

@@ -433,25 +433,28 @@ def test_kernel_layouts_are_bit_identical(precision, ntracers, kw):
     """edgesOnEdge derived from the cells' edge rows (in the PV kernel, the
     tendency kernel, both) gives the slots of the table in the same order;
     the next stage's edge values formed in the tendency kernel are the
-    edge-prep pass's; the tendency kernel's gathers from its LDS tile read
-    the same values: the stage diagnostics, ten steps, beta and the
-    invariants are bit-identical in every mode the scheme allows.  The
-    schemes outside plain CRK4 refuse bits 4 and 8; bit 8 supersedes bit 2."""
+    edge-prep pass's; the tendency and PV kernels' gathers from their LDS
+    tiles read the same values: the stage diagnostics, ten steps, beta and
+    the invariants are bit-identical in every mode the scheme allows.  The
+    schemes outside plain CRK4 refuse bits 4, 8 and 16; bit 8 supersedes bit
+    2 and bit 16 bit 1."""
     m = mesh(4)
     gd, u, ghs, q = seeded_state(m, 5, [201, 202][:ntracers])
     crk4 = not kw
-    modes = range(16) if crk4 else range(4)
+    modes = range(32) if crk4 else range(4)
     runs = {}
     for mode in modes:
         g = SwmGpu(m, m.fit(), dt=600.0, ntracers=ntracers, precision=precision, **kw)
-        assert g.layout & 1  # the generated mesh keeps the MPAS order
-        assert bool(g.layout & 4) == crk4 and bool(g.layout & 8) == crk4
+        # CRK4: edge values from the tendency kernel and both LDS tiles; the
+        # other schemes: the cell rows (the generated mesh keeps the MPAS order)
+        assert g.layout == 28 if crk4 else g.layout & 1
         if not crk4:
-            for extra in (4, 8):
+            for extra in (4, 8, 16):
                 with pytest.raises(Exception, match="CRK4"):
                     g.layout = mode | extra
         g.layout = mode
-        assert g.layout == (mode & ~2 if mode & 8 else mode)
+        want = mode & ~(2 if mode & 8 else 0) & ~(1 if mode & 16 else 0)
+        assert g.layout == want
         g.upload(gd, ghs, u=u, tracers=q if ntracers else None)
         st = g.stage()
         inv, beta = g.step(10)
@@ -474,7 +477,7 @@ def test_upwind_correction_refuses_the_layout_modes():
     m = mesh(3)
     g = SwmGpu(m, m.fit(max_fit=31), dt=600.0, ntracers=1, adv_order=3)
     assert g.layout == 0
-    for mode in (1, 2, 4, 8):
+    for mode in (1, 2, 4, 8, 16):
         with pytest.raises(Exception, match="adv_order"):
             g.layout = mode
     g.close()
@@ -492,7 +495,7 @@ def test_mesh_without_mpas_edge_order_keeps_the_table_kernels():
     m.weightsOnEdge[e, :n] = np.roll(m.weightsOnEdge[e, :n], 1)
     gd, u, ghs, q = seeded_state(m, 2, [201])
     g = SwmGpu(m, m.fit(), dt=600.0, ntracers=1)
-    assert g.layout == 12  # the tendency kernel's edge values and LDS tile need no MPAS order
+    assert g.layout == 28  # the edge values and the LDS tiles need no MPAS order
     with pytest.raises(Exception, match="MPAS"):
         g.layout = 3
     o = Oracle(m, m.fit(), dt=600.0)

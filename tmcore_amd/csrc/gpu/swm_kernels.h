@@ -375,6 +375,10 @@ constexpr int kXTendPrep = 32;
 // the edgesOnEdge values from there (DMesh::eoeL, haloE); the arithmetic is
 // unchanged.
 constexpr int kXTendLds = 64;
+// kXPvLds (same schemes): the PV kernel stages u of its tile and of the
+// tile's edgesOnEdge halo (the tendency kernel's tiles and lists) in LDS for
+// the tangent wind.
+constexpr int kXPvLds = 128;
 
 // sum_j w(j, side, e) * (f(fit(j, c)) - f(c)) over the NS padded slots of a
 // fit stencil of element c (f at offset base; padded slots add an exact 0).
@@ -766,19 +770,30 @@ __device__ __forceinline__ val_t<R, L::W> pv_edge_at(const DMesh &m, const Stage
 
 // Tangent wind (calc_tangent_wind) + scheme-3 PV on the edge.
 template <typename R, typename L, int X = 0>
-__device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e) {
+__device__ __forceinline__ void edge_pv_e(const DMesh &m, const StageBufs<R> &b, L at, int e,
+                                          const R *su = nullptr) {
   typedef val_t<R, L::W> V;
   V v = V(R(0));
   {
-    int ep[kMaxEoE];
     V uv[kMaxEoE];
     R w[kMaxEoE];
-    if constexpr ((X & kXEoePv) != 0)
-      eoe_row_cells(m, e, m.coe[e], m.coe[m.nE + e], ep);
-    else
-      eoe_row(m, e, ep);
+    if constexpr ((X & kXPvLds) != 0) {
+      static_assert(L::W == 1, "the LDS tile holds one member");
 #pragma unroll
-    for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
+      for (int p = 0; p < kMaxEoE / 2; ++p) {
+        const uint32_t pr = nt_ld(&m.eoeL[(size_t)p * m.nE + e]);
+        uv[2 * p] = su[pr & 0xffffu];
+        uv[2 * p + 1] = su[pr >> 16];
+      }
+    } else {
+      int ep[kMaxEoE];
+      if constexpr ((X & kXEoePv) != 0)
+        eoe_row_cells(m, e, m.coe[e], m.coe[m.nE + e], ep);
+      else
+        eoe_row(m, e, ep);
+#pragma unroll
+      for (int s = 0; s < kMaxEoE; ++s) uv[s] = at.ld(b.u, ep[s]);
+    }
 #pragma unroll
     for (int p = 0; p < kMaxEoE / 2; ++p) geo_pair<R>(m.woe, m.woe32, p, e, m.nE, w[2 * p], w[2 * p + 1]);
 #pragma unroll
@@ -993,8 +1008,19 @@ __global__ __launch_bounds__(kBlock) void k_cell_vertex(DMesh m, StageBufs<R> b)
 
 template <typename R, int X>
 __global__ __launch_bounds__(kBlock) void k_edge_pv(DMesh m, StageBufs<R> b) {
-  const int e = xcd_tile(blockIdx.x, m.nbE, b.rev) * kBlock + threadIdx.x;
-  if (e < m.nE) edge_pv_e<R, Solo, X>(m, b, Solo{}, e);
+  const int tile = xcd_tile(blockIdx.x, m.nbE, b.rev);
+  const int e = tile * kBlock + threadIdx.x;
+  if constexpr ((X & kXPvLds) != 0) {
+    // u of the tile's edges, then of its halo (the tendency kernel's tiles)
+    __shared__ R su[kTileE + kTendHaloMax];
+    if (e < m.nE) su[threadIdx.x] = b.u[e];
+    const int h0 = m.haloOff[tile], h1 = m.haloOff[tile + 1];
+    for (int h = h0 + (int)threadIdx.x; h < h1; h += kBlock) su[kTileE + h - h0] = b.u[m.haloE[h]];
+    __syncthreads();
+    if (e < m.nE) edge_pv_e<R, Solo, X>(m, b, Solo{}, e, su);
+  } else {
+    if (e < m.nE) edge_pv_e<R, Solo, X>(m, b, Solo{}, e);
+  }
 }
 
 template <typename R, int X>

@@ -413,6 +413,12 @@ void launch_part(tm_ctx *c, int which, const StageBufs<R> &b) {
     case kXTendLds | kXTendPrep: launch_part_x<R, kXTendLds | kXTendPrep>(c, which, b); break;
     case kXTendLds | kXTendPrep | kXEoePv: launch_part_x<R, kXTendLds | kXTendPrep | kXEoePv>(c, which, b); break;
     case kXEoePv | kXEoeTend: launch_part_x<R, kXEoePv | kXEoeTend>(c, which, b); break;
+    case kXPvLds: launch_part_x<R, kXPvLds>(c, which, b); break;
+    case kXPvLds | kXTendPrep: launch_part_x<R, kXPvLds | kXTendPrep>(c, which, b); break;
+    case kXPvLds | kXEoeTend: launch_part_x<R, kXPvLds | kXEoeTend>(c, which, b); break;
+    case kXPvLds | kXEoeTend | kXTendPrep: launch_part_x<R, kXPvLds | kXEoeTend | kXTendPrep>(c, which, b); break;
+    case kXPvLds | kXTendLds: launch_part_x<R, kXPvLds | kXTendLds>(c, which, b); break;
+    case kXPvLds | kXTendLds | kXTendPrep: launch_part_x<R, kXPvLds | kXTendLds | kXTendPrep>(c, which, b); break;
     case kXExt | kXEoePv: launch_part_x<R, kXExt | kXEoePv>(c, which, b); break;
     case kXExt | kXEoeTend: launch_part_x<R, kXExt | kXEoeTend>(c, which, b); break;
     case kXExt | kXEoePv | kXEoeTend: launch_part_x<R, kXExt | kXEoePv | kXEoeTend>(c, which, b); break;
@@ -1159,11 +1165,12 @@ static int create_impl(tm_ctx *c, const tm_mesh_desc *m, const tm_fit_desc *f) {
   // edgesOnEdge derived from the cells' edge rows where the mesh keeps the
   // MPAS order (not in the upwind-correction variants)
   if (c->md.cell_rows && o.adv_order < 3) c->xmask |= cell_rows_default(c->rsz);
-  // next-stage edge values from the tendency kernel, and its gathers from an
-  // LDS tile (plain CRK4, no upwind correction); the tile replaces the
-  // cell-row derivation in the tendency kernel
+  // next-stage edge values from the tendency kernel, and the tendency and PV
+  // kernels' edgesOnEdge gathers from LDS tiles (plain CRK4, no upwind
+  // correction); the tiles replace the cell-row derivation
   if ((c->xmask & (kXAdv | kXExt)) == 0) c->xmask |= kXTendPrep;
-  if (c->md.tend_lds && (c->xmask & (kXAdv | kXExt)) == 0) c->xmask = (c->xmask & ~kXEoeTend) | kXTendLds;
+  if (c->md.tend_lds && (c->xmask & (kXAdv | kXExt)) == 0)
+    c->xmask = (c->xmask & ~(kXEoeTend | kXEoePv)) | kXTendLds | kXPvLds;
   c->nstages = o.time_scheme == 2 ? 4 : 3;
   if (o.pv_scheme != 3 && !(c->pv_c = c->alloc_r(nC))) return c->fail(TM_ENOMEM, "hipMalloc pv_cell");
   if (o.pv_scheme == 2 &&
@@ -1205,12 +1212,11 @@ static void kernel_bytes(const tm_ctx *c, double *kb) {
   const double fsz = c->rsz == 4 ? 4.0 : 8.0, rs = (double)c->rsz;
   const double prog = nE + nC * (1 + T);
   // edgesOnEdge: the table, or (cellsOnEdge plus) the cells' 32-byte edge rows
-  const double eoe_pv = (c->xmask & kXEoePv) ? nE * 8.0 + nC * 32.0 : 4.0 * sumEoE;
+  const double tiles = nE * 20.0 + 4.0 * c->md.haloTotal + 4.0 * (c->dm.nbE + 1);  // 16-bit local rows + halo lists
+  const double eoe_pv = (c->xmask & kXPvLds) ? tiles : (c->xmask & kXEoePv) ? nE * 8.0 + nC * 32.0 : 4.0 * sumEoE;
   // the tendency kernel's edgesOnEdge: the table, the cells' rows, or the LDS
   // tile's local rows (16-bit) with the halo lists
-  const double eoe_tend = (c->xmask & kXTendLds) ? nE * 20.0 + 4.0 * c->md.haloTotal + 4.0 * (c->dm.nbE + 1)
-                          : (c->xmask & kXEoeTend) ? nC * 32.0
-                                                   : 4.0 * sumEoE;
+  const double eoe_tend = (c->xmask & kXTendLds) ? tiles : (c->xmask & kXEoeTend) ? nC * 32.0 : 4.0 * sumEoE;
   kb[TM_K_EDGE_PREP] = nE * (8.0 + rs + 24.0 + 5.0 * rs) + nC * rs;
   // cells, vertices, the edge products gathered (fk, u dcEdge) and the
   // tracers' cellsOnCell, u and dvEdge
@@ -1244,27 +1250,28 @@ extern "C" int tm_ctx_create(tm_ctx **out, const tm_mesh_desc *m, const tm_fit_d
 
 extern "C" int tm_ctx_set_layout(tm_ctx *c, int mode) {
   if (!c) return TM_EINVAL;
-  if (mode < 0 || mode > 15) return c->fail(TM_EINVAL, "layout mode must be 0..15");
+  if (mode < 0 || mode > 31) return c->fail(TM_EINVAL, "layout mode must be 0..31");
   int want = ((mode & 1) ? kXEoePv : 0) | ((mode & 2) ? kXEoeTend : 0) | ((mode & 4) ? kXTendPrep : 0) |
-             ((mode & 8) ? kXTendLds : 0);
-  if (want & kXTendLds) want &= ~kXEoeTend;  // the LDS tile supersedes the cell rows in the tendency kernel
-  if ((want & kXTendLds) && !c->md.tend_lds)
-    return c->fail(TM_EINVAL, "a tendency tile's edgesOnEdge halo exceeds the LDS tile");
+             ((mode & 8) ? kXTendLds : 0) | ((mode & 16) ? kXPvLds : 0);
+  if (want & kXTendLds) want &= ~kXEoeTend;  // the LDS tiles supersede the cell rows
+  if (want & kXPvLds) want &= ~kXEoePv;
+  if ((want & (kXTendLds | kXPvLds)) && !c->md.tend_lds)
+    return c->fail(TM_EINVAL, "a tile's edgesOnEdge halo exceeds the LDS tile");
   if ((want & (kXEoePv | kXEoeTend)) && !c->md.cell_rows)
     return c->fail(TM_EINVAL, "the mesh's edgesOnEdge rows do not follow the MPAS order");
   if (want && (c->xmask & kXAdv)) return c->fail(TM_EINVAL, "the layout modes do not run with adv_order >= 3");
-  if ((want & (kXTendPrep | kXTendLds)) && (c->xmask & kXExt))
+  if ((want & (kXTendPrep | kXTendLds | kXPvLds)) && (c->xmask & kXExt))
     return c->fail(TM_EINVAL, "edge values from the tendency kernel and its LDS tile need CRK4 with pv_scheme 3 order <= 4");
   HIPCHK(c, hipStreamSynchronize(c->stream));
   drop_graphs(c);  // captured steps hold the previous variant
-  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend | kXTendPrep | kXTendLds)) | want;
+  c->xmask = (c->xmask & ~(kXEoePv | kXEoeTend | kXTendPrep | kXTendLds | kXPvLds)) | want;
   return TM_OK;
 }
 
 extern "C" int tm_ctx_get_layout(tm_ctx *c, int *mode) {
   if (!c || !mode) return TM_EINVAL;
   *mode = ((c->xmask & kXEoePv) ? 1 : 0) | ((c->xmask & kXEoeTend) ? 2 : 0) | ((c->xmask & kXTendPrep) ? 4 : 0) |
-          ((c->xmask & kXTendLds) ? 8 : 0);
+          ((c->xmask & kXTendLds) ? 8 : 0) | ((c->xmask & kXPvLds) ? 16 : 0);
   return TM_OK;
 }
 

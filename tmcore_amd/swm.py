@@ -215,7 +215,8 @@ class SwmGpu:
         """Kernel layout (tm_ctx_set_layout): bit 1 and bit 2, edgesOnEdge
         derived from the cells' edge rows in the PV and tendency kernels;
         bit 4, the tendency kernel forms the next RK stage's edge values;
-        bit 8, the tendency kernel gathers from an LDS tile."""
+        bit 8, the tendency kernel gathers from an LDS tile; bit 16, the PV
+        kernel does."""
         m = C.c_int()
         self._check(gpu_lib().tm_ctx_get_layout(self.h, C.byref(m)), "tm_ctx_get_layout")
         return m.value
