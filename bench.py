@@ -6,9 +6,9 @@ one MI355X, at 655,362 cells (icosahedral level 8).
 A step is one CRK4 time step (4 fused RK stages + beta reduction + update) of
 the level-8 mesh (655,362 cells, 1,966,080 edges, 1,310,720 vertices), dt =
 90 s, 4 passive tracers, seeded synthetic initial state, all inputs resident
-in HBM.  Prints one JSON line.  The CPU baseline times the FP64 oracle
-(oracle/, the C restatement of the reference algorithm) on a bounded sample of
-the same workload on this host.
+in HBM.  Prints one JSON line.  The CPU baseline times the FP64 CPU path in
+Fortran (tmcore_amd/fortran/tm_swm_cpu_mod.F90, OpenMP) on a bounded sample of
+the same workload on this host, with the oracle's C restatement beside it.
 """
 from __future__ import annotations
 
@@ -51,33 +51,42 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(mesh, fit, state, dt, budget_s=12.0):
-    """The FP64 oracle (C restatement of the reference's path, gcc -O2) on a
-    bounded number of steps of the same workload: once on one thread, once on
-    every host thread available to this process (OpenMP over the element
-    loops, at most 16), as the reference ran with GCC auto-parallelisation."""
+def cpu_baseline(mesh, fit, state, dt, budget_s=8.0):
+    """The FP64 CPU path in Fortran (tmcore_amd/fortran/tm_swm_cpu_mod.F90,
+    amdflang -O3 -fopenmp, bit-identical to the oracle) on a bounded number of
+    steps of the same workload: once on one thread, once on every host thread
+    available to this process (OpenMP over the element loops, at most 16), as
+    the reference ran with GCC auto-parallelisation.  The oracle's C
+    restatement (gcc -O2, OpenMP) is timed beside it on all threads."""
     from oracle.oracle import Oracle, set_threads
+    from tmcore_amd.cpu_path import cpu_run
     gd, u, ghs, q = state
     o = Oracle(mesh, fit, dt=dt)
     iap = o.initial_iap(gd, u)
     nthreads = min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count())
+    days_h = lambda n, el: (n * dt / 86400.0) / (el / 3600.0)
     res = {}
     for threads in (1, nthreads):
-        set_threads(threads)
-        t0 = time.perf_counter()
-        o.run(1, gd, iap, ghs, q)
-        t1 = time.perf_counter() - t0
+        t1 = cpu_run(mesh, fit, gd, iap, ghs, q, dt, 1, threads)[-1]
         n = int(max(2, min(30, budget_s / max(t1, 1e-3))))
-        t0 = time.perf_counter()
-        o.run(n, gd, iap, ghs, q)
-        el = time.perf_counter() - t0
-        res[threads] = (n, el, (n * dt / 86400.0) / (el / 3600.0))
+        el = cpu_run(mesh, fit, gd, iap, ghs, q, dt, n, threads)[-1]
+        res[threads] = (n, el, days_h(n, el))
+    set_threads(nthreads)
+    t0 = time.perf_counter()
+    o.run(1, gd, iap, ghs, q)
+    n_c = int(max(2, min(30, budget_s / max(time.perf_counter() - t0, 1e-3))))
+    t0 = time.perf_counter()
+    o.run(n_c, gd, iap, ghs, q)
+    el_c = time.perf_counter() - t0
     n, el, v = res[nthreads]
     n1, el1, v1 = res[1]
     return {"value": v, "unit": "sim-days/hour", "cores": nthreads, "kind": "port",
-            "sample": f"{n} CRK4 steps (dt={dt:g} s, {q.shape[0]} tracers) of the same level-8 workload, "
+            "language": "Fortran (amdflang -O3 -fopenmp -ffp-contract=off), bit-identical to the oracle",
+            "sample": f"{n} CRK4 steps (dt={dt:g} s, {q.shape[0]} tracers) of the same level-{mesh.level} workload, "
                       f"{el:.1f} s wall on {nthreads} threads; single thread: {n1} steps in {el1:.1f} s",
             "single_thread_value": v1, "s_per_step": el / n, "s_per_step_single_thread": el1 / n1,
+            "c_port": {"value": days_h(n_c, el_c), "s_per_step": el_c / n_c, "threads": nthreads,
+                       "build": "gcc -O2 -ffp-contract=off -fopenmp (oracle/)"},
             "cpu_model": cpu_model(), "nproc": os.cpu_count(), "threads": nthreads}
 
 

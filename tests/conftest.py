@@ -19,6 +19,9 @@ def _built_libs():
     if not os.path.exists(os.path.join(_ffi.LIB_DIR, "libtmhost.so")):
         from tmcore_amd import build
         build.build_host()
+    if not os.path.exists(os.path.join(_ffi.LIB_DIR, "libtmcpu.so")):
+        from tmcore_amd import build
+        build.build_fortran()
     oracle_so = os.path.join(ROOT, "oracle", "build", "liboracle.so")
     if not os.path.exists(oracle_so):
         from oracle import oracle

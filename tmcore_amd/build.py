@@ -5,6 +5,7 @@
 libtmhost.so  g++     mesh generator, poly_fit weights, initial states
 libtmgpu.so   hipcc   gfx950 kernels + C-ABI (include/tmcore_gpu.h)
 test_tmcore_sw amdflang  Fortran namelist driver over the C-ABI (if amdflang exists)
+libtmcpu.so   amdflang  Fortran FP64 CPU path (OpenMP), the CPU baseline of bench.py
 """
 from __future__ import annotations
 
@@ -58,7 +59,7 @@ def build_fortran():
         print("amdflang not found; skipping the Fortran driver", flush=True)
         return
     fdir = os.path.join(PKG, "fortran")
-    srcs = ["tm_c_binding_mod.F90", "tm_params_mod.F90", "tm_swm_gpu_mod.F90", "test_tmcore_sw.F90"]
+    srcs = ["tm_desc_mod.F90", "tm_c_binding_mod.F90", "tm_params_mod.F90", "tm_swm_gpu_mod.F90", "test_tmcore_sw.F90"]
     if not all(os.path.exists(os.path.join(fdir, s)) for s in srcs):
         return
     bdir = os.path.join(fdir, "build")
@@ -69,6 +70,13 @@ def build_fortran():
     # the operator wrappers under their reference names, for the parity test
     _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs[:-1]],
           os.path.join(fdir, "test_tmcore_ops.F90"), "-o", os.path.join(LIB, "test_tmcore_ops"), *link], cwd=bdir)
+    # the FP64 CPU path: -O3 with OpenMP, no FMA contraction (the reference's
+    # association order, as the oracle keeps it)
+    cdir = os.path.join(bdir, "cpu")
+    os.makedirs(cdir, exist_ok=True)
+    _run([FLANG, "-O3", "-fopenmp", "-ffp-contract=off", "-fPIC", "-shared", "-J", cdir,
+          os.path.join(fdir, "tm_desc_mod.F90"), os.path.join(fdir, "tm_swm_cpu_mod.F90"),
+          "-o", os.path.join(LIB, "libtmcpu.so")], cwd=cdir)
 
 
 def build_all():
