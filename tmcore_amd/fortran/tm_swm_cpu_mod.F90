@@ -29,13 +29,13 @@ module tm_swm_cpu_mod
   ! of mesh_mod.F90:169-224
   type cpu_mesh_t
     integer :: nC, nE, nV, me, me2, mf
-    integer, pointer :: nEoC(:), nEoE(:), eoc(:,:), coe(:,:), voe(:,:), eoe(:,:), cov(:,:), eov(:,:)
-    real(8), pointer :: woe(:,:), kite(:,:)
+    integer, pointer, contiguous :: nEoC(:), nEoE(:), eoc(:,:), coe(:,:), voe(:,:), eoe(:,:), cov(:,:), eov(:,:)
+    real(8), pointer, contiguous :: woe(:,:), kite(:,:)
     real(8), allocatable :: dv(:), dc(:), aE(:), aC(:), aT(:), fV(:)
     integer, allocatable :: nsgn(:,:), tsgn(:,:)
     real(8) :: total_area
-    integer, pointer :: nfit(:), fit(:,:)
-    real(8), pointer :: d2v(:,:,:)
+    integer, pointer, contiguous :: nfit(:), fit(:,:)
+    real(8), pointer, contiguous :: d2v(:,:,:)
   end type cpu_mesh_t
 
   ! work arrays of one stage
@@ -124,7 +124,7 @@ contains
     type(tm_fit_desc), intent(in) :: fd
     real(8), intent(in) :: radius
     logical, intent(in) :: want_fit
-    real(8), pointer :: p1(:), lat(:)
+    real(8), pointer, contiguous :: p1(:), lat(:)
     integer :: i, k, e, vx
 
     m%nC = md%nCells; m%nE = md%nEdges; m%nV = md%nVertices
@@ -208,8 +208,9 @@ contains
 
   ! out = old + a * t (update_state, tmcore_swm_mod.F90:203-213)
   subroutine axpy(old, a, t, out)
-    real(8), intent(in) :: old(:), t(:), a
-    real(8), intent(inout) :: out(:)
+    real(8), intent(in), contiguous :: old(:), t(:)
+    real(8), intent(in) :: a
+    real(8), intent(inout), contiguous :: out(:)
     integer :: i
     !$omp parallel do
     do i = 1, size(old)
@@ -220,8 +221,8 @@ contains
   ! div_operator (operators_mod.F90:40-55): minus the divergence
   subroutine divergence(m, f, d)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: f(:)
-    real(8), intent(out) :: d(:)
+    real(8), intent(in), contiguous :: f(:)
+    real(8), intent(out), contiguous :: d(:)
     integer :: i, k, e
     real(8) :: s
     !$omp parallel do private(k, e, s)
@@ -238,8 +239,8 @@ contains
   ! curl_operator (operators_mod.F90:76-91)
   subroutine curl(m, f, r)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: f(:)
-    real(8), intent(out) :: r(:)
+    real(8), intent(in), contiguous :: f(:)
+    real(8), intent(out), contiguous :: r(:)
     integer :: vx, k, e
     real(8) :: s
     !$omp parallel do private(k, e, s)
@@ -256,8 +257,8 @@ contains
   ! scalar_c2e_interp_operator (operators_mod.F90:93-104)
   subroutine cell_to_edge(m, f, fe)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: f(:)
-    real(8), intent(out) :: fe(:)
+    real(8), intent(in), contiguous :: f(:)
+    real(8), intent(out), contiguous :: fe(:)
     integer :: e
     !$omp parallel do
     do e = 1, m%nE
@@ -268,8 +269,8 @@ contains
   ! scalar_c2v_interp_operator (operators_mod.F90:106-123)
   subroutine cell_to_vertex(m, f, fv)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: f(:)
-    real(8), intent(out) :: fv(:)
+    real(8), intent(in), contiguous :: f(:)
+    real(8), intent(out), contiguous :: fv(:)
     integer :: vx, k
     real(8) :: s
     !$omp parallel do private(k, s)
@@ -286,8 +287,8 @@ contains
   ! the vertices' cells, serial in vertex order as the reference's loop
   subroutine vertex_to_cell(m, fv, f)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: fv(:)
-    real(8), intent(out) :: f(:)
+    real(8), intent(in), contiguous :: fv(:)
+    real(8), intent(out), contiguous :: f(:)
     integer :: vx, k, i
     f = 0.0d0
     do vx = 1, m%nV
@@ -301,8 +302,8 @@ contains
   ! calc_kinetic_energy (operators_mod.F90:206-221)
   subroutine kinetic_energy(m, ue, kc)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: ue(:)
-    real(8), intent(out) :: kc(:)
+    real(8), intent(in), contiguous :: ue(:)
+    real(8), intent(out), contiguous :: kc(:)
     integer :: i, k, e
     real(8) :: s
     !$omp parallel do private(k, e, s)
@@ -319,8 +320,8 @@ contains
   ! calc_tangent_wind (operators_mod.F90:223-237)
   subroutine tangent_wind(m, ue, ve)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: ue(:)
-    real(8), intent(out) :: ve(:)
+    real(8), intent(in), contiguous :: ue(:)
+    real(8), intent(out), contiguous :: ve(:)
     integer :: e, k
     real(8) :: s
     !$omp parallel do private(k, s)
@@ -336,8 +337,8 @@ contains
   ! calc_tangent_vor_flux (operators_mod.F90:239-260)
   subroutine tangent_vor_flux(m, ue, ge, pe, fl)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: ue(:), ge(:), pe(:)
-    real(8), intent(out) :: fl(:)
+    real(8), intent(in), contiguous :: ue(:), ge(:), pe(:)
+    real(8), intent(out), contiguous :: fl(:)
     integer :: e, k, j
     real(8) :: s
     !$omp parallel do private(k, j, s)
@@ -355,8 +356,8 @@ contains
   subroutine pv_on_edge(m, order, ve, pv, pe)
     type(cpu_mesh_t), intent(in) :: m
     integer, intent(in) :: order
-    real(8), intent(in) :: ve(:), pv(:)
-    real(8), intent(out) :: pe(:)
+    real(8), intent(in), contiguous :: ve(:), pv(:)
+    real(8), intent(out), contiguous :: pe(:)
     integer :: e, v1, v2, j
     real(8) :: d1, d2, coef2, dv
     coef2 = merge(1.0d0, 0.0d0, order == 3)
@@ -385,8 +386,8 @@ contains
   subroutine stage(m, order, gd, iap, ghs, nT, q, tend_gd, tend_iap, tend_q)
     type(cpu_mesh_t), intent(in) :: m
     integer, intent(in) :: order, nT
-    real(8), intent(in) :: gd(:), iap(:), ghs(:), q(:,:)
-    real(8), intent(out) :: tend_gd(:), tend_iap(:), tend_q(:,:)
+    real(8), intent(in), contiguous :: gd(:), iap(:), ghs(:), q(:,:)
+    real(8), intent(out), contiguous :: tend_gd(:), tend_iap(:), tend_q(:,:)
     integer :: e, i, t
     real(8) :: sg, dEdx
     call cell_to_edge(m, gd, gd_e)
@@ -437,7 +438,7 @@ contains
   ! inner_product_tend (operators_mod.F90:187-194): serial sums
   real(8) function inner(m, ag, ai, bg, bi)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: ag(:), ai(:), bg(:), bi(:)
+    real(8), intent(in), contiguous :: ag(:), ai(:), bg(:), bi(:)
     real(8) :: s1, s2
     integer :: i
     s1 = 0.0d0
@@ -455,7 +456,7 @@ contains
   ! mass, total energy, potential enstrophy, mean absolute vorticity
   subroutine invariants(m, gd, iap, ghs, out)
     type(cpu_mesh_t), intent(in) :: m
-    real(8), intent(in) :: gd(:), iap(:), ghs(:)
+    real(8), intent(in), contiguous :: gd(:), iap(:), ghs(:)
     real(8), intent(out) :: out(4)
     real(8) :: mass, e1, e2, ens, av, at
     integer :: i, e
