@@ -83,6 +83,69 @@ def test_fortran_driver_rejects_missing_namelist(tmp_path):
     assert "cannot open namelist" in r.stdout
 
 
+def test_fortran_driver_cpu_path_matches_oracle_bitwise(tmp_path):
+    """use_gpu = .false.: the same namelist run takes its steps on the host in
+    the Fortran FP64 CPU path; every printed invariant and beta (17
+    significant digits) equals the oracle's."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang absent when building)")
+    level, dt, hours = 4, 600.0, 2
+    nml = tmp_path / "namelist.tmcore_sw"
+    nml.write_text(NAMELIST.format(level=level, dt=int(dt), hours=hours, extra="use_gpu = .false.\n", tmcore="",
+                                   io=""))
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "on the host CPU" in r.stdout
+    rows = parse(r.stdout)
+    nsteps = int(hours * 3600 / dt)
+    assert sorted(rows) == list(range(0, nsteps + 1))
+    m = Mesh(level)
+    gd, u, ghs, q = seeded_state(m, 7, [77])
+    o = Oracle(m, m.fit(), dt=dt)
+    iap = o.initial_iap(gd, u)
+    inv0 = o.invariants(gd, iap, ghs)
+    _, _, _, inv, beta = o.run(nsteps, gd, iap, ghs, q)
+    np.testing.assert_array_equal(rows[0], inv0)
+    for s in range(1, nsteps + 1):
+        np.testing.assert_array_equal(rows[s][:4], inv[s - 1])
+        assert rows[s][4] == beta[s - 1]
+
+
+@pytest.mark.parametrize("tmcore,extra", [("", "precision = 32\n"), ("adv_order = 3\n", ""),
+                                          ("time_scheme = 1\n", ""), ("interp_pv_order = 5\n", "")])
+def test_fortran_driver_cpu_path_refuses_what_it_does_not_implement(tmp_path, tmcore, extra):
+    """A later namelist assignment overrides the template's: each case asks the
+    CPU path for a scheme or form it does not have."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built")
+    nml = tmp_path / "nml"
+    nml.write_text(NAMELIST.format(level=2, dt=600, hours=1, extra="use_gpu = .false.\n" + extra, tmcore=tmcore,
+                                   io=""))
+    r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "use_gpu = .false. runs" in r.stdout
+
+
+@pytest.mark.gpu
+def test_fortran_driver_cpu_and_gpu_paths_agree(tmp_path):
+    """The same namelist with use_gpu .true. and .false.: the device run and
+    the host CPU run print the same invariants and beta to 1e-12."""
+    if not os.path.exists(EXE):
+        pytest.skip("Fortran driver not built (amdflang absent when building)")
+    outs = {}
+    for flag in (".true.", ".false."):
+        nml = tmp_path / f"nml{flag}"
+        nml.write_text(NAMELIST.format(level=5, dt=600, hours=3, extra=f"use_gpu = {flag}\n", tmcore="", io=""))
+        r = subprocess.run([EXE, str(nml)], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stdout + r.stderr
+        outs[flag] = parse(r.stdout)
+    assert sorted(outs[".true."]) == sorted(outs[".false."]) == list(range(0, 19))
+    for s in range(19):
+        np.testing.assert_allclose(outs[".true."][s][:3], outs[".false."][s][:3], rtol=1e-12)
+        if s:
+            assert abs(outs[".true."][s][4] - outs[".false."][s][4]) < 1e-12
+
+
 @pytest.mark.gpu
 def test_fortran_driver_attaches_closure_file(tmp_path):
     """closure_file: a zero-output closure leaves every diagnostic line of the

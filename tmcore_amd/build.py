@@ -54,21 +54,26 @@ def build_gpu():
 
 
 def build_fortran():
-    """Fortran host driver test_tmcore_sw (iso_c_binding over the C-ABI)."""
+    """Fortran host driver test_tmcore_sw (iso_c_binding over the C-ABI, or
+    the host CPU path with use_gpu = .false.), the operator-wrapper test
+    program, and libtmcpu.so (the Fortran FP64 CPU path alone)."""
     if not os.path.exists(FLANG):
         print("amdflang not found; skipping the Fortran driver", flush=True)
         return
     fdir = os.path.join(PKG, "fortran")
-    srcs = ["tm_desc_mod.F90", "tm_c_binding_mod.F90", "tm_params_mod.F90", "tm_swm_gpu_mod.F90", "test_tmcore_sw.F90"]
+    srcs = ["tm_desc_mod.F90", "tm_c_binding_mod.F90", "tm_params_mod.F90", "tm_swm_cpu_mod.F90", "tm_swm_gpu_mod.F90",
+            "test_tmcore_sw.F90"]
     if not all(os.path.exists(os.path.join(fdir, s)) for s in srcs):
         return
     bdir = os.path.join(fdir, "build")
     os.makedirs(bdir, exist_ok=True)
     link = ["-L" + LIB, "-ltmgpu", "-ltmhost", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
-    _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs], "-o",
+    # -fopenmp and no FMA contraction for the CPU path the host can run (use_gpu = .false.)
+    fflags = ["-O3", "-fopenmp", "-ffp-contract=off"]
+    _run([FLANG, *fflags, "-J", bdir, *[os.path.join(fdir, s) for s in srcs], "-o",
           os.path.join(LIB, "test_tmcore_sw"), *link], cwd=bdir)
     # the operator wrappers under their reference names, for the parity test
-    _run([FLANG, "-O2", "-J", bdir, *[os.path.join(fdir, s) for s in srcs[:-1]],
+    _run([FLANG, *fflags, "-J", bdir, *[os.path.join(fdir, s) for s in srcs[:-1]],
           os.path.join(fdir, "test_tmcore_ops.F90"), "-o", os.path.join(LIB, "test_tmcore_ops"), *link], cwd=bdir)
     # the FP64 CPU path: -O3 with OpenMP, no FMA contraction (the reference's
     # association order, as the oracle keeps it)

@@ -70,10 +70,13 @@ module tm_params_mod
   integer :: closure_interval = 1    ! steps between MLP evaluations
   integer :: closure_lumped = 0      ! 0: correction after every step; 1: once per interval
   real(8) :: closure_gain = 1.0d0    ! factor on the applied correction
+  logical :: use_gpu = .true.        ! .false.: the steps run on the host CPU in the Fortran FP64 CPU path
+                                     ! (tm_swm_cpu_mod: CRK4 with beta, pv_scheme 3 at interp_pv_order
+                                     ! 2-4, adv_order 2, precision 64, no closure, no history)
 
   namelist /gpu_control/ mesh_level, precision, ntracers, seed, tracer_seed0, device, renumber, print_every, &
                          closure_file, closure_interval, closure_lumped, &
-                         closure_gain
+                         closure_gain, use_gpu
 
 contains
 

@@ -8,10 +8,14 @@
 ! element's arithmetic in the reference's association order, and the global
 ! sums stay serial in element order, as the reference's are.
 !
-! tm_cpu_run (bind(c)) takes the reference-layout mesh and fits of
-! include/tmcore_host.h (1-based connectivity, unit-sphere geometry) and the
-! state in reference order, advances it nsteps steps and returns the
-! invariants and beta of every step and the wall time of the step loop.
+! A Fortran host uses it through cpu_setup / cpu_initial_iap / cpu_invariants
+! / cpu_steps / cpu_final (one mesh at a time: the module holds the mesh and
+! the work arrays, so the procedures are not re-entrant).  tm_cpu_run
+! (bind(c)) does the same for a C caller: it takes the reference-layout mesh
+! and fits of include/tmcore_host.h (1-based connectivity, unit-sphere
+! geometry) and the state in reference order, advances it nsteps steps and
+! returns the invariants and beta of every step and the wall time of the step
+! loop.
 module tm_swm_cpu_mod
 
   use iso_c_binding
@@ -21,7 +25,7 @@ module tm_swm_cpu_mod
   implicit none
 
   private
-  public tm_cpu_run
+  public tm_cpu_run, cpu_setup, cpu_initial_iap, cpu_velocity, cpu_invariants, cpu_steps, cpu_final
 
   real(8), parameter :: omega = 7.292d-5     ! const_mod.F90:13
 
@@ -38,9 +42,14 @@ module tm_swm_cpu_mod
     real(8), pointer, contiguous :: d2v(:,:,:)
   end type cpu_mesh_t
 
-  ! work arrays of one stage
+  ! the mesh of cpu_setup, its PV interpolation order, and the work arrays
+  type(cpu_mesh_t) :: cm
+  integer :: cm_order = 0
+  logical :: cm_ready = .false.
   real(8), allocatable :: gd_e(:), u(:), tgd_e(:), v(:), pv_e(:), pv_flx(:), q_e(:), flx(:)
   real(8), allocatable :: tgd_v(:), gd_v(:), vor(:), pv_v(:), ke(:), pv_c(:), en(:)
+  ! state and tendency slots of the CRK4 step (state_mod.F90:63, tend_mod.F90:44)
+  real(8), allocatable :: sg(:,:), si(:,:), sq(:,:,:), tg(:,:), ti(:,:), tq(:,:,:)
 
 contains
 
@@ -54,27 +63,95 @@ contains
     real(c_double), intent(in) :: ghs(md%nCells)
     real(c_double), intent(out) :: inv(4, *), beta(*)
     real(c_double), intent(out) :: elapsed
-    type(cpu_mesh_t) :: m
-    real(8), allocatable :: sg(:,:), si(:,:), sq(:,:,:), tg(:,:), ti(:,:), tq(:,:,:)
-    real(8) :: cf(3), t0, b, r12, r23, r34, r44
-    integer :: step, k, i, t, nT
+    real(8) :: t0
 
     tm_cpu_run = -1
-    if (interp_pv_order < 2 .or. interp_pv_order > 4 .or. ntracers < 0 .or. nsteps < 0) return
-    if (interp_pv_order >= 3 .and. .not. c_associated(fd%deriv2OnVertex)) return
+    if (ntracers < 0 .or. nsteps < 0) return
     if (nthreads > 0) call omp_set_num_threads(nthreads)
-    call mesh_setup(m, md, fd, radius, interp_pv_order >= 3)
-    nT = ntracers
-    allocate(sg(m%nC, 4), si(m%nE, 4), sq(m%nC, max(nT, 1), 4), tg(m%nC, 0:4), ti(m%nE, 0:4), &
-             tq(m%nC, max(nT, 1), 0:4))
-    cf = [0.5d0, 0.5d0, 1.0d0]
+    tm_cpu_run = cpu_setup(md, fd, radius, interp_pv_order, ntracers)
+    if (tm_cpu_run /= 0) return
     t0 = omp_get_wtime()
+    call cpu_steps(nsteps, dt, gd, iap, ghs, ntracers, q, inv, beta)
+    elapsed = omp_get_wtime() - t0
+    call cpu_final()
+  end function tm_cpu_run
+
+  ! mesh_init + poly_fit_init's tables for the CPU path (tmcore_swm_mod.F90:37-41):
+  ! 0, or -1 when the order is not 2-4, the fits are missing or a mesh is set up
+  integer function cpu_setup(md, fd, radius, interp_pv_order, ntracers) result(rc)
+    type(tm_mesh_desc), intent(in) :: md
+    type(tm_fit_desc), intent(in) :: fd
+    real(8), intent(in) :: radius
+    integer, intent(in) :: interp_pv_order, ntracers
+
+    rc = -1
+    if (cm_ready .or. interp_pv_order < 2 .or. interp_pv_order > 4 .or. ntracers < 0) return
+    if (interp_pv_order >= 3 .and. .not. c_associated(fd%deriv2OnVertex)) return
+    call mesh_setup(cm, md, fd, radius, interp_pv_order >= 3)
+    allocate(sg(cm%nC, 4), si(cm%nE, 4), sq(cm%nC, max(ntracers, 1), 4), tg(cm%nC, 0:4), ti(cm%nE, 0:4), &
+             tq(cm%nC, max(ntracers, 1), 0:4))
+    cm_order = interp_pv_order
+    cm_ready = .true.
+    rc = 0
+  end function cpu_setup
+
+  subroutine cpu_final()
+    if (.not. cm_ready) return
+    call work_free()
+    call mesh_free(cm)
+    deallocate(sg, si, sq, tg, ti, tq)
+    cm_ready = .false.
+  end subroutine cpu_final
+
+  ! iap_u = sqrt(c2e(gd)) * u, the run prologue (tmcore_swm_mod.F90:62-64)
+  subroutine cpu_initial_iap(gd, uu, iap)
+    real(8), intent(in), contiguous :: gd(:), uu(:)
+    real(8), intent(out), contiguous :: iap(:)
+    integer :: e
+    call cell_to_edge(cm, gd, gd_e)
+    !$omp parallel do
+    do e = 1, cm%nE
+      iap(e) = sqrt(gd_e(e)) * uu(e)
+    end do
+  end subroutine cpu_initial_iap
+
+  ! u = iap_u / sqrt(c2e(gd)), the refresh of tmcore_swm_mod.F90:78-80
+  subroutine cpu_velocity(gd, iap, uu)
+    real(8), intent(in), contiguous :: gd(:), iap(:)
+    real(8), intent(out), contiguous :: uu(:)
+    integer :: e
+    call cell_to_edge(cm, gd, gd_e)
+    !$omp parallel do
+    do e = 1, cm%nE
+      uu(e) = iap(e) / sqrt(gd_e(e))
+    end do
+  end subroutine cpu_velocity
+
+  ! the invariants of a state (the refresh and diag_run of tmcore_swm_mod.F90:78-85)
+  subroutine cpu_invariants(gd, iap, ghs, out)
+    real(8), intent(in), contiguous :: gd(:), iap(:), ghs(:)
+    real(8), intent(out) :: out(4)
+    call invariants(cm, gd, iap, ghs, out)
+  end subroutine cpu_invariants
+
+  ! nsteps CRK4 steps with beta (time_scheme_mod.F90:109-154); the tracers
+  ! q(:, 1:ntracers) with plain RK4 weights; inv(:, s) and beta(s) of step s
+  subroutine cpu_steps(nsteps, dt, gd, iap, ghs, nT, q, inv, beta)
+    integer, intent(in) :: nsteps, nT
+    real(8), intent(in) :: dt
+    real(8), intent(inout) :: gd(cm%nC), iap(cm%nE), q(cm%nC, *)
+    real(8), intent(in) :: ghs(cm%nC)
+    real(8), intent(out) :: inv(4, *), beta(*)
+    real(8) :: cf(3), b, r12, r23, r34, r44
+    integer :: step, k, i, t
+
+    cf = [0.5d0, 0.5d0, 1.0d0]
     do step = 1, nsteps
       sg(:,1) = gd
       si(:,1) = iap
       if (nT > 0) sq(:,1:nT,1) = q(:,1:nT)
       do k = 1, 4
-        call stage(m, interp_pv_order, sg(:,k), si(:,k), ghs, nT, sq(:,:,k), tg(:,k), ti(:,k), tq(:,:,k))
+        call stage(cm, cm_order, sg(:,k), si(:,k), ghs, nT, sq(:,:,k), tg(:,k), ti(:,k), tq(:,:,k))
         if (k < 4) then
           call axpy(sg(:,1), cf(k) * dt, tg(:,k), sg(:,k+1))
           call axpy(si(:,1), cf(k) * dt, ti(:,k), si(:,k+1))
@@ -84,37 +161,33 @@ contains
         end if
       end do
       !$omp parallel do
-      do i = 1, m%nC
+      do i = 1, cm%nC
         tg(i,0) = (((tg(i,1) + 2.0d0 * tg(i,2)) + 2.0d0 * tg(i,3)) + tg(i,4)) / 6.0d0
       end do
       !$omp parallel do
-      do i = 1, m%nE
+      do i = 1, cm%nE
         ti(i,0) = (((ti(i,1) + 2.0d0 * ti(i,2)) + 2.0d0 * ti(i,3)) + ti(i,4)) / 6.0d0
       end do
       do t = 1, nT
         !$omp parallel do
-        do i = 1, m%nC
+        do i = 1, cm%nC
           tq(i,t,0) = (((tq(i,t,1) + 2.0d0 * tq(i,t,2)) + 2.0d0 * tq(i,t,3)) + tq(i,t,4)) / 6.0d0
         end do
       end do
-      r44 = inner(m, tg(:,0), ti(:,0), tg(:,0), ti(:,0))
-      r12 = inner(m, tg(:,1), ti(:,1), tg(:,2), ti(:,2))
-      r23 = inner(m, tg(:,2), ti(:,2), tg(:,3), ti(:,3))
-      r34 = inner(m, tg(:,3), ti(:,3), tg(:,4), ti(:,4))
+      r44 = inner(cm, tg(:,0), ti(:,0), tg(:,0), ti(:,0))
+      r12 = inner(cm, tg(:,1), ti(:,1), tg(:,2), ti(:,2))
+      r23 = inner(cm, tg(:,2), ti(:,2), tg(:,3), ti(:,3))
+      r34 = inner(cm, tg(:,3), ti(:,3), tg(:,4), ti(:,4))
       b = (r12 + r23 + r34) / (3.0d0 * r44)
-      call axpy(gd, b * dt, tg(:,0), gd)
-      call axpy(iap, b * dt, ti(:,0), iap)
+      call add_scaled(gd, b * dt, tg(:,0))
+      call add_scaled(iap, b * dt, ti(:,0))
       do t = 1, nT
-        call axpy(q(:,t), dt, tq(:,t,0), q(:,t))
+        call add_scaled(q(:,t), dt, tq(:,t,0))
       end do
       beta(step) = b
-      call invariants(m, gd, iap, ghs, inv(:,step))
+      call invariants(cm, gd, iap, ghs, inv(:,step))
     end do
-    elapsed = omp_get_wtime() - t0
-    call work_free()
-    call mesh_free(m)
-    tm_cpu_run = 0
-  end function tm_cpu_run
+  end subroutine cpu_steps
 
   ! mesh_init's scaling and the derived arrays (mesh_mod.F90:169-224); the
   ! caller's connectivity is viewed in place
@@ -217,6 +290,18 @@ contains
       out(i) = old(i) + a * t(i)
     end do
   end subroutine axpy
+
+  ! x = x + a * t (the final update of update_state on state(old))
+  subroutine add_scaled(x, a, t)
+    real(8), intent(inout), contiguous :: x(:)
+    real(8), intent(in), contiguous :: t(:)
+    real(8), intent(in) :: a
+    integer :: i
+    !$omp parallel do
+    do i = 1, size(x)
+      x(i) = x(i) + a * t(i)
+    end do
+  end subroutine add_scaled
 
   ! div_operator (operators_mod.F90:40-55): minus the divergence
   subroutine divergence(m, f, d)

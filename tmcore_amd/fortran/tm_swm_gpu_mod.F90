@@ -12,12 +12,16 @@
 ! The operators of operators_mod keep their names and dummy arguments
 ! (operators_mod.F90:40-260) and run on the device.  A non-zero return code
 ! from the library prints its message and stops, as the reference's
-! log_error does.
+! log_error does.  With use_gpu = .false. (namelist gpu_control) the same
+! run takes its steps on the host CPU through the Fortran FP64 CPU path
+! (tm_swm_cpu_mod) instead, for the configuration that path implements.
 module tm_swm_gpu_mod
 
   use iso_c_binding
   use tm_c_binding_mod
   use tm_params_mod
+  use tm_swm_cpu_mod, only: cpu_setup, cpu_initial_iap, cpu_velocity, cpu_invariants, cpu_steps, cpu_final
+  use omp_lib, only: omp_get_wtime
 
   implicit none
 
@@ -177,10 +181,25 @@ contains
       write(*, '(A)') '[Error]: flux_4th_order_correct = .true. is not reproducible in the reference (deriv3OnCell)'
       stop 1
     end if
-    call check(tm_ctx_create(ctx, mesh, fit, opts), 'tm_ctx_create')
-    call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
-    if (trim(history_interval(1)) /= 'N/A') call history_init()
-    if (trim(closure_file) /= 'N/A') call attach_closure(trim(closure_file))
+    if (use_gpu) then
+      call check(tm_ctx_create(ctx, mesh, fit, opts), 'tm_ctx_create')
+      call check(tm_state_upload(ctx, c_loc(gd), c_loc(u), c_null_ptr, c_loc(ghs), c_loc(tracers)), 'tm_state_upload')
+      if (trim(history_interval(1)) /= 'N/A') call history_init()
+      if (trim(closure_file) /= 'N/A') call attach_closure(trim(closure_file))
+    else
+      if (time_scheme /= 2 .or. .not. conserve_energy .or. pv_scheme /= 3 .or. interp_pv_order < 2 .or. &
+          interp_pv_order > 4 .or. adv_order /= 2 .or. precision /= 64 .or. trim(closure_file) /= 'N/A' .or. &
+          trim(history_interval(1)) /= 'N/A') then
+        write(*, '(A)') '[Error]: use_gpu = .false. runs CRK4 with conserve_energy, pv_scheme 3 at interp_pv_order' // &
+          ' 2-4, adv_order 2 and precision 64, without closure_file or history'
+        stop 1
+      end if
+      if (cpu_setup(mesh, fit, radius, interp_pv_order, ntracers) /= 0) then
+        write(*, '(A)') '[Error]: the CPU path could not set up the mesh'
+        stop 1
+      end if
+      call cpu_initial_iap(gd, u, iap_u)
+    end if
 
     if (trim(mesh_file_path) /= 'N/A') then
       write(*, '(A,I0,A,I0,A,I0,A,F8.2,A,I0)') '[Notice]: mesh ' // trim(mesh_file_path) // ': ', nCells, ' cells, ', &
@@ -237,6 +256,10 @@ contains
 
     nsteps = run_steps()
     allocate(run_inv(4, nsteps), run_beta(nsteps))
+    if (.not. use_gpu) then
+      call cpu_run(nsteps)
+      return
+    end if
     call check(tm_invariants(ctx, inv0), 'tm_invariants')
     write(*, '(A,I8,4(A,ES24.16))') 'step=', 0, ' total_mass=', inv0(1), ' total_energy=', inv0(2), &
       ' total_potential_enstropy=', inv0(3), ' total_absolute_vorticity=', inv0(4)
@@ -269,6 +292,43 @@ contains
       (nsteps * dt / 86400.0_r8) / (max(run_ms, 1.0e-9_r8) / 3.6e6_r8)
 
   end subroutine tm_swm_run
+
+  ! tm_swm_run on the host CPU: the same print intervals and lines, the steps
+  ! of the Fortran FP64 CPU path, the wall time of the steps as run_ms.
+  subroutine cpu_run(nsteps)
+
+    integer, intent(in) :: nsteps
+    integer :: done, n, i
+    real(c_double) :: inv0(4), t0
+    real(c_double), allocatable :: inv(:,:), beta(:)
+
+    call cpu_invariants(gd, iap_u, ghs, inv0)
+    write(*, '(A,I8,4(A,ES24.16))') 'step=', 0, ' total_mass=', inv0(1), ' total_energy=', inv0(2), &
+      ' total_potential_enstropy=', inv0(3), ' total_absolute_vorticity=', inv0(4)
+    done = 0
+    run_ms = 0
+    do while (done < nsteps)
+      n = min(max(print_every, 1), nsteps - done)
+      allocate(inv(4, n), beta(n))
+      t0 = omp_get_wtime()
+      call cpu_steps(n, dt, gd, iap_u, ghs, ntracers, tracers, inv, beta)
+      run_ms = run_ms + 1000.0_r8 * (omp_get_wtime() - t0)
+      run_inv(:, done+1:done+n) = inv
+      run_beta(done+1:done+n) = beta
+      do i = 1, n
+        if (mod(done + i, max(print_every, 1)) == 0 .or. done + i == nsteps) then
+          write(*, '(A,I8,5(A,ES24.16))') 'step=', done + i, ' total_mass=', inv(1,i), ' total_energy=', inv(2,i), &
+            ' total_potential_enstropy=', inv(3,i), ' total_absolute_vorticity=', inv(4,i), ' beta=', beta(i)
+        end if
+      end do
+      deallocate(inv, beta)
+      done = done + n
+    end do
+    call cpu_velocity(gd, iap_u, u)
+    write(*, '(A,I0,A,F12.3,A,F14.3)') '[Notice]: ', nsteps, ' steps on the host CPU, time ', run_ms, &
+      ' ms, sim-days/hour ', (nsteps * dt / 86400.0_r8) / (max(run_ms, 1.0e-9_r8) / 3.6e6_r8)
+
+  end subroutine cpu_run
 
   ! The learned closure of a raw weights file (see tm_params_mod closure_file),
   ! applied in every following step.
@@ -366,6 +426,7 @@ contains
     ctx = c_null_ptr
     if (c_associated(mlp)) rc = tm_mlp_destroy(mlp)
     mlp = c_null_ptr
+    call cpu_final()
 
   end subroutine tm_swm_final
 
