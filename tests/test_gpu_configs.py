@@ -105,6 +105,17 @@ def test_level5_fifteen_days_matches_oracle():
 
 
 @pytest.mark.timeout(900)
+def test_reference_shipped_run_length_matches_oracle():
+    """The reference's shipped run settings (run/namelist.tmcore_swm:1-3,13-21):
+    20 simulated days at dt 240 s (7,200 steps) on a 40,962-cell mesh (the size
+    of its x1.40962 grid, here icosahedral level 6), CRK4 with beta, pv_scheme 3
+    order 3, with the seeded state: the per-step invariant log and the final
+    state against the oracle."""
+    dev, ora, inv0 = _long_run_vs_oracle(6, 240.0, 7200, 4, [], chunk=1200)
+    _check_parity(dev, ora, inv0, 0, 1e-9)
+
+
+@pytest.mark.timeout(900)
 def test_level10_precision_form_against_fp64():
     """Level 10 (10,485,762 cells), seed 3, dt 22.5 s.  One FP64 step against
     the oracle (fields <= 1e-12, beta and invariants <= 1e-10); then 960
