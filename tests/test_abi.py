@@ -19,12 +19,13 @@ def declared(header):
     return sorted(set(re.findall(r"\b(tm_[a-z0-9_]+)\s*\(", txt)))
 
 
-@pytest.mark.parametrize("header,libname", [("tmcore_host.h", "libtmhost.so"), ("tmcore_gpu.h", "libtmgpu.so")])
+@pytest.mark.parametrize("header,libname", [("tmcore_host.h", "libtmhost.so"), ("tmcore_gpu.h", "libtmgpu.so"),
+                                            ("tmcore_cpu.h", "libtmcpu.so")])
 def test_every_declared_symbol_is_exported(header, libname):
     path = os.path.join(_ffi.LIB_DIR, libname)
     if not os.path.exists(path):
         from tmcore_amd import build
-        build.build_host() if "host" in libname else build.build_gpu()
+        {"libtmhost.so": build.build_host, "libtmgpu.so": build.build_gpu, "libtmcpu.so": build.build_fortran}[libname]()
     lib = C.CDLL(path)
     names = declared(header)
     assert names, header
