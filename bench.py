@@ -271,13 +271,31 @@ def main():
         wall = time.perf_counter() - t0
         ev_ms = g.last_elapsed_ms()
         st = g.download()
-        return g, wall, ev_ms, st
+        # a second download of the same state into the same (already touched)
+        # arrays, timed: what a host with preallocated state arrays sees
+        t0 = time.perf_counter()
+        g.download(out=st)
+        down = time.perf_counter() - t0
+        # a second upload of the initial state, timed: the host-buffer boundary
+        # (validation, renumbering, host-to-device copies, the iap_u prologue)
+        t0 = time.perf_counter()
+        g.upload(gd, ghs, u=u, tracers=q)
+        up = time.perf_counter() - t0
+        return g, wall, ev_ms, st, up, down
 
-    g64, wall, ev_ms, st64 = run_form(64)
+    g64, wall, ev_ms, st64, up64, down64 = run_form(64)
     ms_per_step = 1000.0 * wall / args.steps
     sim_days_h = (args.steps * args.dt / 86400.0) / (wall / 3600.0)
     log(f"FP64: {ms_per_step:.4f} ms/step wall, {ev_ms / args.steps:.4f} ms/step events, "
         f"{sim_days_h:.1f} sim-days/h")
+    # the same run with the state handed over in host memory: one upload
+    # before it and one download (gd, u, iap_u, tracers) after it
+    host_boundary = {
+        "upload_ms": round(1000.0 * up64, 3), "download_ms": round(1000.0 * down64, 3),
+        "sim_days_per_hour_incl_transfers": round((args.steps * args.dt / 86400.0) /
+                                                  ((wall + up64 + down64) / 3600.0), 2),
+        "ms_per_step_with_a_download_every_step": round(ms_per_step + 1000.0 * down64, 4)}
+    log(f"FP64 host boundary: {host_boundary}")
     sb = g64.step_bytes()
     step_gbs = sb["step"] / (ev_ms / args.steps * 1e-3) / 1e9
 
@@ -296,9 +314,10 @@ def main():
     copy_gbs = hbm_copy_gbps(2.0 * 2**30, 20)
     log(f"HBM copy probe (2 GiB, read+write): {copy_gbs:.1f} GB/s")
 
-    forms = {"fp64": {"sim_days_per_hour": round(sim_days_h, 2), "ms_per_step": round(ms_per_step, 4)}}
+    forms = {"fp64": {"sim_days_per_hour": round(sim_days_h, 2), "ms_per_step": round(ms_per_step, 4),
+                      "host_boundary": host_boundary}}
     if not args.no_fp32:
-        g32, wall32, ev32, st32 = run_form(32)
+        g32, wall32, ev32, st32, _, _ = run_form(32)
         g32.close()
         from numpy.linalg import norm
         h64 = (st64["gd"] + ghs)

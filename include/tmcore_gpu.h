@@ -86,7 +86,10 @@ const char *tm_last_error(const tm_ctx *ctx);
 /* Prognostic upload (reference state(old), tmcore_swm_mod.F90:62-64):
  * gd[nCells] and ghs[nCells]; either iap_u[nEdges], or u[nEdges] from which
  * iap_u = sqrt(gd_edge)*u is formed on the device as the reference's run
- * prologue does; tracers[ntracers][nCells] may be NULL when ntracers == 0. */
+ * prologue does; tracers[ntracers][nCells] may be NULL when ntracers == 0.
+ * Upload and download are synchronous; both stage through a pinned host
+ * buffer the context allocates at its first transfer (sized to the largest
+ * state array) and keeps until tm_ctx_destroy. */
 int tm_state_upload(tm_ctx *ctx, const double *gd, const double *u, const double *iap_u,
                     const double *ghs, const double *tracers);
 /* Download of state(old): any pointer may be NULL.  u is the inverse IAP

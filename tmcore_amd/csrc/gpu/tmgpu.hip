@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "closure_kernels.h"
@@ -167,6 +169,10 @@ struct tm_ctx {
   hipGraphExec_t gexec_k[5] = {};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *noalloc_scratch = nullptr;  // this device's `scratch` (tm_invariants_noalloc)
+  // pinned host buffer the state transfers stage through (tm_state_upload /
+  // tm_state_download), allocated at the first transfer, grown on demand
+  void *host_stage = nullptr;
+  size_t host_stage_bytes = 0;
   bool have_state = false;
   bool timed = false;
   int serpentine = 1;  // alternate the tile sweep direction between kernels
@@ -235,34 +241,79 @@ int upload(tm_ctx *c, T *dst, const std::vector<T> &h) {
   return TM_OK;
 }
 
-// Convert a reference-order FP64 host array to new order and the state type.
-int upload_state(tm_ctx *c, void *dst, const double *src, const std::vector<int> &perm, int ncopies, int n) {
-  if (c->rsz == 8) {
-    std::vector<double> h((size_t)n * ncopies);
-    for (int t = 0; t < ncopies; ++t)
-      for (int i = 0; i < n; ++i) h[(size_t)t * n + i] = src[(size_t)t * n + perm[i]];
-    return upload(c, (double *)dst, h);
+// f(lo, hi) over [0, n) in contiguous chunks on up to 16 host threads (one
+// thread below 64k elements).
+template <typename F>
+void par_for(size_t n, F f) {
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  const size_t nt = n < (size_t(1) << 16) ? 1 : std::min<size_t>(hw, n >> 15);
+  if (nt <= 1) {
+    f(size_t(0), n);
+    return;
   }
-  std::vector<float> h((size_t)n * ncopies);
-  for (int t = 0; t < ncopies; ++t)
-    for (int i = 0; i < n; ++i) h[(size_t)t * n + i] = (float)src[(size_t)t * n + perm[i]];
-  return upload(c, (float *)dst, h);
+  std::vector<std::thread> th;
+  th.reserve(nt - 1);
+  for (size_t k = 1; k < nt; ++k) th.emplace_back(f, n * k / nt, n * (k + 1) / nt);
+  f(size_t(0), n / nt);
+  for (auto &t : th) t.join();
+}
+
+// The context's pinned staging buffer, at least `bytes` long (null if the
+// pinned allocation fails).
+void *host_stage(tm_ctx *c, size_t bytes) {
+  if (c->host_stage_bytes >= bytes) return c->host_stage;
+  if (c->host_stage) (void)hipHostFree(c->host_stage);
+  c->host_stage = nullptr;
+  c->host_stage_bytes = 0;
+  if (hipHostMalloc(&c->host_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+    c->host_stage = nullptr;
+    return nullptr;
+  }
+  c->host_stage_bytes = bytes;
+  return c->host_stage;
+}
+
+// Convert a reference-order FP64 host array to new order and the state type,
+// through the pinned staging buffer.
+int upload_state(tm_ctx *c, void *dst, const double *src, const std::vector<int> &perm, int ncopies, int n) {
+  const size_t len = (size_t)n * ncopies;
+  void *h = host_stage(c, len * c->rsz);
+  if (!h) return c->fail(TM_EDEVICE, "hipHostMalloc of the transfer staging buffer failed");
+  const int *pm = perm.data();
+  if (c->rsz == 8) {
+    double *hd = (double *)h;
+    par_for(len, [=](size_t lo, size_t hi) {
+      for (size_t j = lo; j < hi; ++j) hd[j] = src[j - j % n + pm[j % n]];
+    });
+  } else {
+    float *hf = (float *)h;
+    par_for(len, [=](size_t lo, size_t hi) {
+      for (size_t j = lo; j < hi; ++j) hf[j] = (float)src[j - j % n + pm[j % n]];
+    });
+  }
+  HIPCHK(c, hipMemcpyAsync(dst, h, len * c->rsz, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return TM_OK;
 }
 
 int download_state(tm_ctx *c, double *dst, const void *src, const std::vector<int> &perm, int ncopies, int n) {
-  if (c->rsz == 8) {
-    std::vector<double> h((size_t)n * ncopies);
-    HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(c, hipStreamSynchronize(c->stream));
-    for (int t = 0; t < ncopies; ++t)
-      for (int i = 0; i < n; ++i) dst[(size_t)t * n + perm[i]] = h[(size_t)t * n + i];
-    return TM_OK;
-  }
-  std::vector<float> h((size_t)n * ncopies);
-  HIPCHK(c, hipMemcpyAsync(h.data(), src, h.size() * 4, hipMemcpyDeviceToHost, c->stream));
+  const size_t len = (size_t)n * ncopies;
+  void *h = host_stage(c, len * c->rsz);
+  if (!h) return c->fail(TM_EDEVICE, "hipHostMalloc of the transfer staging buffer failed");
+  HIPCHK(c, hipMemcpyAsync(h, src, len * c->rsz, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  for (int t = 0; t < ncopies; ++t)
-    for (int i = 0; i < n; ++i) dst[(size_t)t * n + perm[i]] = h[(size_t)t * n + i];
+  const int *pm = perm.data();
+  if (c->rsz == 8) {
+    const double *hd = (const double *)h;
+    par_for(len, [=](size_t lo, size_t hi) {
+      for (size_t j = lo; j < hi; ++j) dst[j - j % n + pm[j % n]] = hd[j];
+    });
+  } else {
+    const float *hf = (const float *)h;
+    par_for(len, [=](size_t lo, size_t hi) {
+      for (size_t j = lo; j < hi; ++j) dst[j - j % n + pm[j % n]] = hf[j];
+    });
+  }
   return TM_OK;
 }
 
@@ -640,9 +691,13 @@ int run_steps(tm_ctx *c, int nsteps, bool refresh) {
 }
 
 bool finite_all(const double *a, size_t n) {
-  for (size_t i = 0; i < n; ++i)
-    if (!std::isfinite(a[i])) return false;
-  return true;
+  std::atomic<int> bad{0};
+  par_for(n, [&](size_t lo, size_t hi) {
+    int b = 0;
+    for (size_t i = lo; i < hi; ++i) b |= !std::isfinite(a[i]);
+    if (b) bad.store(1, std::memory_order_relaxed);
+  });
+  return bad.load() == 0;
 }
 
 }  // namespace
@@ -679,6 +734,7 @@ extern "C" int tm_ctx_destroy(tm_ctx *c) {
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   for (void *p : c->allocs) (void)hipFree(p);
+  if (c->host_stage) (void)hipHostFree(c->host_stage);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return TM_OK;

@@ -159,12 +159,16 @@ class SwmGpu:
         self._check(gpu_lib().tm_state_upload(self.h, _p(gd), _p(u), _p(iap_u), _p(ghs), _p(tracers)),
                     "tm_state_upload")
 
-    def download(self):
-        """Returns dict gd, u, iap_u, tracers (reference order)."""
-        gd = np.zeros(self.nC)
-        u = np.zeros(self.nE)
-        iap = np.zeros(self.nE)
-        tr = np.zeros((self.nT, self.nC))
+    def download(self, out=None):
+        """Returns dict gd, u, iap_u, tracers (reference order); with `out` (a
+        dict an earlier download returned) the arrays are filled in place."""
+        if out is None:
+            out = {"gd": np.zeros(self.nC), "u": np.zeros(self.nE), "iap_u": np.zeros(self.nE),
+                   "tracers": np.zeros((self.nT, self.nC))}
+        gd, u, iap, tr = out["gd"], out["u"], out["iap_u"], out["tracers"]
+        for a, n in ((gd, (self.nC,)), (u, (self.nE,)), (iap, (self.nE,)), (tr, (self.nT, self.nC))):
+            if a.shape != n or a.dtype != np.float64 or not a.flags.c_contiguous:
+                raise ValueError(f"download: out array of shape {a.shape} / {a.dtype}, expected {n} float64")
         self._check(gpu_lib().tm_state_download(self.h, _p(gd), _p(u), _p(iap), _p(tr) if self.nT else None),
                     "tm_state_download")
         return {"gd": gd, "u": u, "iap_u": iap, "tracers": tr}
