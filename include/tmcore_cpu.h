@@ -29,7 +29,8 @@ extern "C" {
  * advanced in place; inv
  * [nsteps][4] and beta[nsteps] receive every step's invariants and beta;
  * *seconds the wall time of the step loop.  nthreads > 0 sets the OpenMP
- * threads.  Returns 0, or -1 for an order outside 2-4, missing vertex fits
+ * threads (omp_set_num_threads: it stays the process's default for later
+ * OpenMP regions of this runtime); 0 keeps the current one.  Returns 0, or -1 for an order outside 2-4, missing vertex fits
  * at orders 3-4, or negative counts.  Not re-entrant (one mesh at a time).
  */
 int tm_cpu_run(const tm_mesh_desc *mesh, const tm_fit_desc *fit, double radius, double dt, int interp_pv_order,

@@ -18,6 +18,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <system_error>
 #include <thread>
 #include <vector>
 
@@ -253,8 +254,14 @@ void par_for(size_t n, F f) {
   }
   std::vector<std::thread> th;
   th.reserve(nt - 1);
-  for (size_t k = 1; k < nt; ++k) th.emplace_back(f, n * k / nt, n * (k + 1) / nt);
+  size_t k = 1;
+  try {
+    for (; k < nt; ++k) th.emplace_back(f, n * k / nt, n * (k + 1) / nt);
+  } catch (const std::system_error &) {
+    // no more threads: the chunks not handed out run on this thread
+  }
   f(size_t(0), n / nt);
+  for (size_t r = k; r < nt; ++r) f(n * r / nt, n * (r + 1) / nt);
   for (auto &t : th) t.join();
 }
 
